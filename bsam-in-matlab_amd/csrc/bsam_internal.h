@@ -1,0 +1,107 @@
+// Internal layout of the device-resident patch hierarchy.
+//
+// Every level stores its patches packed back to back in one buffer per field,
+// each patch column-major (x index fastest), with a per-patch offset record:
+//   Q     (mx+2)(my+2)  padded solution            PatchInfo::qoff
+//   F, FT, CC, ID, vis/tgt maps   mx*my            PatchInfo::coff
+//   DeW   (mx+1)*my     x-faces                    PatchInfo::xoff
+//   DnS   mx*(my+1)     y-faces                    PatchInfo::yoff
+//   QC    (mx/2+2)(my/2+2) coarse shadow + ring    PatchInfo::qcoff
+//   G     boundary data W(my) E(my) S(mx) N(mx)    PatchInfo::goff
+// All table indices are absolute element indices into a level's buffers, so a
+// launch sweeps every patch of a level through one offset table.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+
+namespace bsam {
+
+constexpr int kTile = 256;  // cells per work tile (4 wave64)
+
+struct PatchInfo {
+  int mx, my, par0, cf;   // cf: patch has an internal face or CF corner
+  int bc[4];              // W E S N: 0 internal, 1 Dirichlet, 2 Neumann
+  int qoff, coff, xoff, yoff, qcoff, goff, ccoff, pad;
+};
+
+// ghost record kinds (one record per ghost cell of a level)
+enum GhostKind : int {
+  G_COPY = 0,      // Q[s0] (same-level sibling interior)
+  G_QIF = 1,       // (a1 Q[s0] + a2 Q[s1]) + a3 P[s2]
+  G_QIF_END = 2,   // (a1 Q[s0] + a2 Q[s1]) + a3 ((b1 P[s2] + b2 P[s3]) + b3 P[s4])
+  G_LIF = 3,       // (2/3 P[s2] + 2/3 Q[s0]) - 1/3 Q[s1]
+  G_LIF_CORNER = 4,// 2/3 P[s2] + 1/3 Q[s0]
+  G_DIR = 5,       // (8/3 g - 2 Q[s0]) + 1/3 Q[s1]
+  G_NEU = 6,       // Q[s0] + (h g)
+  G_PCORNER = 7,   // (side(s0) + side(s1)) - Q[s2]; s0,s1 are record ids
+};
+
+struct GhostRec {
+  int dst, kind;
+  int s[5];
+  int pad;
+  double c;  // g (Dirichlet) or h*g (Neumann)
+};
+
+struct RingRec {
+  int dst, src;  // QC index <- parent Q index
+};
+
+// per coarse footprint cell: parent indices for restriction / FAS load
+struct FootRec {
+  int p, ci, cj;       // fine patch and coarse cell (0-based, interior)
+  int pq;              // parent Q index (padded) of the coarse cell
+  int pc;              // parent cell index (F/CC layout)
+  int dew_w, dew_e;    // parent DeW indices of the cell's west/east faces
+  int dns_s, dns_n;    // parent DnS indices of the cell's south/north faces
+};
+
+// flux-balance (reflux) contribution of one coarse CF face
+struct CorrRec {
+  int d1, d2;          // face diffusivities (DeW or DnS)
+  int in1, g1, in2, g2;// fine interior / ghost Q indices
+  int ring, qcin;      // QC ring / QC interior indices
+  int face, pad;
+  double cf;           // coarse interface-face diffusivity
+};
+
+// device tables and buffers of one level
+template <typename T>
+struct LevelDev {
+  int n[2];
+  double h[2];
+  int np, ncell, nq, nx, ny, nqc, ng;
+  int ntile, nctile, nghost, nring_all, nring_cf, nfoot, ncorr, ntgt;
+  PatchInfo* P;
+  int2* tiles;       // (patch, first cell) per 256-cell tile
+  int2* ctiles;      // (patch, first coarse cell) per tile of the footprint
+  GhostRec* ghost;
+  RingRec* ring_all;
+  RingRec* ring_cf;
+  FootRec* foot;
+  CorrRec* corr;     // this level's CF faces (targets live on level k-1)
+  int* tgt_cell;     // targets on THIS level (from level k+1's corrections)
+  int* tgt_ptr;
+  int* tgt_ent;      // entries into level k+1's corr array
+  int* tgt_of_cell;  // per cell: target row or -1
+  int* wmap;         // per QC cell: parent Q index
+  uint8_t* vis;      // per cell: 1 if not covered by the next level
+  T *Q, *F, *FT, *CC, *ID, *DeW, *DnS, *QC, *G;
+};
+
+struct CoarsestDev {
+  int N, M;          // unknowns, matrix order (N or N+1 with the KKT row)
+  double* Ainv;      // M x M column-major
+  double* bcRhs;     // N
+};
+
+// scalars of the composite residual
+struct ResidualOut {
+  double linf, l2sq, fnorm, pad;
+  long long ncell, pad2[3];
+};
+
+}  // namespace bsam

@@ -1,0 +1,1442 @@
+// Host side of libbsamgpu: hierarchy upload, descriptor tables, the solve
+// loop and the C ABI declared in include/bsamgpu.h.
+//
+// The descriptor tables are derived from the patch boxes the way the reference
+// derives its gather/scatter descriptors (EllipticBSAM/src/ebBuildOps.m:17-199,
+// ebGatherParentCoeffs.m:286-337), but flattened into per-level arrays of
+// absolute element indices so that every kernel sweeps all patches of a level
+// in one launch.  The solve loop replays the reference's call sequence
+// (ebSolveCycles.m / ebVcycle.m / ebRelaxLevel.m / ebTransfer.m) as a stream
+// of level-wide launches, optionally captured once into a HIP graph and
+// replayed per cycle.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/bsamgpu.h"
+#include "bsam_kernels.h"
+
+using namespace bsam;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(BSAM_E_HIP, "bsamgpu:hip: %s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct Box {
+  int ilo, ihi, jlo, jhi;
+};
+
+// host copy of one level: inputs, offsets and tables
+struct HLevel {
+  int n[2] = {0, 0};
+  double h[2] = {0, 0};
+  int np = 0;
+  std::vector<Box> box;
+  std::vector<PatchInfo> pi;
+  int ncell = 0, nq = 0, nx = 0, ny = 0, nqc = 0, ng = 0;
+  std::vector<double> Q, FT, DeW, DnS, CC, ID, G;
+  std::vector<int2> tiles;
+  std::vector<GhostRec> ghost;
+  std::vector<RingRec> ring_all, ring_cf;
+  std::vector<FootRec> foot;
+  std::vector<CorrRec> corr;
+  std::vector<int> corr_target;   // parent cell of each corr entry (export)
+  std::vector<int> tgt_cell, tgt_ptr, tgt_ent, tgt_of_cell, wmap, wmap_patch;
+  std::vector<uint8_t> vis;
+  std::vector<int8_t> klass;      // ghost classes per patch (export)
+  std::vector<int> klass_off;
+};
+
+template <typename T>
+struct DevBufs {
+  std::vector<void*> allocs;
+  ~DevBufs() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+  template <typename U>
+  int alloc(U** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) return 0;
+    if (hipMalloc((void**)p, n * sizeof(U)) != hipSuccess)
+      return fail(BSAM_E_NOMEM, "bsamgpu:nomem: device allocation of %zu bytes failed", n * sizeof(U));
+    allocs.push_back(*p);
+    return 0;
+  }
+};
+
+}  // namespace
+
+struct bsam_ctx {
+  int device = 0;
+};
+
+struct bsam_hier {
+  bsam_ctx* ctx = nullptr;
+  int nlev = 0, root = 0;
+  double dom[4] = {0, 0, 0, 0};
+  bsam_solve_opts opts{};
+  std::vector<HLevel> H;                   // index 0 = level 1
+  std::vector<LevelDev<double>> d64;
+  std::vector<LevelDev<float>> d32;
+  CoarsestDev cst{};
+  bool singular = false;
+  bool have32 = false;
+  hipStream_t stream = nullptr;
+  DevBufs<char> mem;
+  ResidualOut* parts = nullptr;            // residual partials, all levels
+  std::vector<int> part_off;               // per level
+  int nparts = 0;
+  ResidualOut* dres = nullptr;
+  ResidualOut* hres = nullptr;             // pinned
+  double* Uo = nullptr;                    // parabolic history (packed cells, all levels)
+  double* Uoo = nullptr;
+  double* fbuf = nullptr;
+  std::vector<long long> cell_base;        // per level offset in the history arrays
+  long long ncell_all = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_key = -1;
+  double graph_omega = 1.0;
+  ~bsam_hier() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (hres) (void)hipHostFree(hres);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// ------------------------------------------------------------------ tables
+inline int qidx(const PatchInfo& p, int li, int lj) { return p.qoff + li + lj * (p.mx + 2); }
+inline int cidx(const PatchInfo& p, int i, int j) { return p.coff + i + j * p.mx; }
+
+void make_tiles(HLevel& L) {
+  L.tiles.clear();
+  for (int p = 0; p < L.np; ++p) {
+    const int nc = L.pi[p].mx * L.pi[p].my;
+    for (int s = 0; s < nc; s += kTile) L.tiles.push_back(make_int2(p, s));
+  }
+}
+
+// Physical-face and corner records shared by every level; internal-face
+// records need the owner map and parent window (build_level_tables).
+GhostRec phys_rec(const HLevel& L, const PatchInfo& P, int side, int t) {
+  GhostRec r{};
+  const int mx = P.mx, my = P.my;
+  int dst, s0, s1;
+  double h;
+  switch (side) {
+    case 0: dst = qidx(P, 0, t + 1); s0 = qidx(P, 1, t + 1); s1 = qidx(P, 2, t + 1); h = L.h[0]; break;
+    case 1: dst = qidx(P, mx + 1, t + 1); s0 = qidx(P, mx, t + 1); s1 = qidx(P, mx - 1, t + 1); h = L.h[0]; break;
+    case 2: dst = qidx(P, t + 1, 0); s0 = qidx(P, t + 1, 1); s1 = qidx(P, t + 1, 2); h = L.h[1]; break;
+    default: dst = qidx(P, t + 1, my + 1); s0 = qidx(P, t + 1, my); s1 = qidx(P, t + 1, my - 1); h = L.h[1]; break;
+  }
+  const int goff = side == 0 ? 0 : side == 1 ? my : side == 2 ? 2 * my : 2 * my + mx;
+  const double g = L.G[P.goff + goff + t];
+  r.dst = dst;
+  if (P.bc[side] == 1) {
+    r.kind = G_DIR; r.s[0] = s0; r.s[1] = s1; r.c = g;
+  } else {
+    r.kind = G_NEU; r.s[0] = s0; r.c = h * g;
+  }
+  return r;
+}
+
+// interpolation record of a coarse-fine ghost (ebInterpCf.m:45-153).
+// win(wi,wj) returns the parent Q index of QC window cell (wi,wj).
+template <typename Win>
+GhostRec cf_rec(const PatchInfo& P, int side, int t, bool qif, Win win) {
+  GhostRec r{};
+  const int mx = P.mx, my = P.my, cmx = mx / 2, cmy = my / 2;
+  // side-local frame: position u along the face (1..len), depth 1/2 cells
+  const int len = (side < 2) ? my : mx;
+  const int u = t + 1;
+  auto q = [&](int depth, int along) {  // padded Q index `depth` cells inside
+    switch (side) {
+      case 0: return qidx(P, depth, along);
+      case 1: return qidx(P, mx + 1 - depth, along);
+      case 2: return qidx(P, along, depth);
+      default: return qidx(P, along, my + 1 - depth);
+    }
+  };
+  auto w = [&](int along) {  // ring window cell on this side
+    switch (side) {
+      case 0: return win(0, along);
+      case 1: return win(cmx + 1, along);
+      case 2: return win(along, 0);
+      default: return win(along, cmy + 1);
+    }
+  };
+  const int clen = (side < 2) ? cmy : cmx;
+  switch (side) {
+    case 0: r.dst = qidx(P, 0, u); break;
+    case 1: r.dst = qidx(P, mx + 1, u); break;
+    case 2: r.dst = qidx(P, u, 0); break;
+    default: r.dst = qidx(P, u, my + 1); break;
+  }
+  if (qif) {
+    if (u == len) {
+      r.kind = G_QIF_END; r.s[0] = q(2, len); r.s[1] = q(1, len);
+      r.s[2] = w(clen + 1); r.s[3] = w(clen); r.s[4] = w(clen - 1);
+    } else if (u == 1) {
+      r.kind = G_QIF_END; r.s[0] = q(2, 1); r.s[1] = q(1, 1);
+      r.s[2] = w(0); r.s[3] = w(1); r.s[4] = w(2);
+    } else if ((u & 1) == 0) {
+      r.kind = G_QIF; r.s[0] = q(2, u + 2); r.s[1] = q(1, u + 1); r.s[2] = w(u / 2);
+    } else {
+      r.kind = G_QIF; r.s[0] = q(2, u - 2); r.s[1] = q(1, u - 1); r.s[2] = w((u + 1) / 2);
+    }
+  } else {
+    if ((u & 1) == 0) {
+      r.kind = G_LIF; r.s[0] = q(1, u); r.s[1] = q(1, u - 1); r.s[2] = w(u / 2);
+    } else {
+      r.kind = G_LIF; r.s[0] = q(1, u); r.s[1] = q(1, u + 1); r.s[2] = w((u + 1) / 2);
+    }
+  }
+  return r;
+}
+
+struct OwnerMap {
+  int n0 = 0, n1 = 0;
+  std::vector<int> o;  // p+1, 0 = none
+  void build(const HLevel& L) {
+    n0 = L.n[0]; n1 = L.n[1];
+    o.assign((size_t)n0 * n1, 0);
+    for (int p = 0; p < L.np; ++p) {
+      const Box& b = L.box[p];
+      for (int j = b.jlo; j <= b.jhi; ++j)
+        for (int i = b.ilo; i <= b.ihi; ++i) o[(size_t)(i - 1) + (size_t)(j - 1) * n0] = p + 1;
+    }
+  }
+  int at(int gi, int gj) const {  // 1-based, 0 outside
+    if (gi < 1 || gi > n0 || gj < 1 || gj > n1) return 0;
+    return o[(size_t)(gi - 1) + (size_t)(gj - 1) * n0];
+  }
+};
+
+// Ghost records of a level without a parent (level 1): all faces physical,
+// corners extrapolated (ebFillGhosts.m:53-89).
+void build_phys_level(HLevel& L) {
+  L.ghost.clear();
+  L.klass.clear();
+  L.klass_off.clear();
+  for (int p = 0; p < L.np; ++p) {
+    const PatchInfo& P = L.pi[p];
+    L.klass_off.push_back((int)L.klass.size());
+    int first[4];
+    for (int side = 0; side < 4; ++side) {
+      first[side] = (int)L.ghost.size();
+      const int len = side < 2 ? P.my : P.mx;
+      for (int t = 0; t < len; ++t) {
+        L.ghost.push_back(phys_rec(L, P, side, t));
+        L.klass.push_back(1);
+      }
+    }
+    const int mx = P.mx, my = P.my;
+    const int cdst[4] = {qidx(P, 0, 0), qidx(P, mx + 1, 0), qidx(P, 0, my + 1), qidx(P, mx + 1, my + 1)};
+    const int cs0[4] = {first[2], first[2] + mx - 1, first[3], first[3] + mx - 1};
+    const int cs1[4] = {first[0], first[1], first[0] + my - 1, first[1] + my - 1};
+    const int cs2[4] = {qidx(P, 1, 1), qidx(P, mx, 1), qidx(P, 1, my), qidx(P, mx, my)};
+    for (int c = 0; c < 4; ++c) {
+      GhostRec r{};
+      r.dst = cdst[c]; r.kind = G_PCORNER; r.s[0] = cs0[c]; r.s[1] = cs1[c]; r.s[2] = cs2[c];
+      L.ghost.push_back(r);
+      L.klass.push_back(1);
+    }
+  }
+}
+
+// Tables of level k (F) against its parent level k-1 (C): ebBuildOps.m:17-199
+// and ebGatherParentCoeffs.m:286-337.
+int build_level_tables(HLevel& F, HLevel& C, bool qif) {
+  OwnerMap OF, OC;
+  OF.build(F);
+  OC.build(C);
+  F.ghost.clear(); F.ring_all.clear(); F.ring_cf.clear(); F.foot.clear(); F.corr.clear();
+  F.corr_target.clear(); F.klass.clear(); F.klass_off.clear();
+  F.wmap.assign(F.nqc, -1);
+  F.wmap_patch.assign(F.nqc, -1);
+  std::vector<uint8_t> cov(C.ncell, 0);
+  for (int p = 0; p < F.np; ++p) {
+    const Box& b = F.box[p];
+    const PatchInfo& P = F.pi[p];
+    if (b.ilo % 2 != 1 || b.ihi % 2 != 0 || b.jlo % 2 != 1 || b.jhi % 2 != 0)
+      return fail(BSAM_E_ALIGN, "eb_build_ops:align: patch not aligned to parent cells.");
+    const int mx = P.mx, my = P.my;
+    const int Il = (b.ilo + 1) / 2, Ih = b.ihi / 2, Jl = (b.jlo + 1) / 2, Jh = b.jhi / 2;
+    const int cmx = Ih - Il + 1, cmy = Jh - Jl + 1;
+    const int ldc = cmx + 2;
+    // ---- parent window: ring (with overhang into parent ghosts) + interior
+    auto set_w = [&](int wi, int wj, int q, int li, int lj) {
+      const PatchInfo& Q = C.pi[q];
+      F.wmap[P.qcoff + wi + wj * ldc] = qidx(Q, li, lj);
+      F.wmap_patch[P.qcoff + wi + wj * ldc] = q;
+    };
+    for (int wj = 0; wj < cmy + 2; ++wj) {
+      for (int wi = 0; wi < cmx + 2; ++wi) {
+        const int gi = Il - 1 + wi, gj = Jl - 1 + wj;
+        const bool ringcell = wi == 0 || wj == 0 || wi == cmx + 1 || wj == cmy + 1;
+        if (ringcell) {
+          const int gic = std::min(std::max(gi, 1), C.n[0]);
+          const int gjc = std::min(std::max(gj, 1), C.n[1]);
+          const int q = OC.at(gic, gjc);
+          if (q == 0)
+            return fail(BSAM_E_RING, "eb_build_ops:ring: parent cell (%d,%d) not covered - proper nesting violated.", gi, gj);
+          const Box& qb = C.box[q - 1];
+          const int li = gi - qb.ilo + 1, lj = gj - qb.jlo + 1;
+          if (li < 0 || li > qb.ihi - qb.ilo + 2 || lj < 0 || lj > qb.jhi - qb.jlo + 2)
+            return fail(BSAM_E_OVERHANG, "eb_build_ops:overhang: ring cell overhang exceeds ghost ring.");
+          set_w(wi, wj, q - 1, li, lj);
+        } else {
+          const int q = OC.at(gi, gj);
+          if (q == 0)
+            return fail(BSAM_E_NESTING,
+                        "eb_build_ops:nesting: patch %d at level: parent window not fully covered - proper nesting violated.", p + 1);
+          const Box& qb = C.box[q - 1];
+          set_w(wi, wj, q - 1, gi - qb.ilo + 1, gj - qb.jlo + 1);
+          cov[cidx(C.pi[q - 1], gi - qb.ilo, gj - qb.jlo)] = 1;
+        }
+      }
+    }
+    auto win = [&](int wi, int wj) { return F.wmap[P.qcoff + wi + wj * ldc]; };
+    // ---- ghost classes and records (land / cornerClass / same-level)
+    F.klass_off.push_back((int)F.klass.size());
+    int first[4];
+    bool anycf = false;
+    for (int side = 0; side < 4; ++side) {
+      first[side] = (int)F.ghost.size();
+      const int len = side < 2 ? my : mx;
+      for (int t = 0; t < len; ++t) {
+        if (P.bc[side] != 0) {
+          F.ghost.push_back(phys_rec(F, P, side, t));
+          F.klass.push_back(1);
+          continue;
+        }
+        int gi, gj;
+        switch (side) {
+          case 0: gi = b.ilo - 1; gj = b.jlo + t; break;
+          case 1: gi = b.ihi + 1; gj = b.jlo + t; break;
+          case 2: gi = b.ilo + t; gj = b.jlo - 1; break;
+          default: gi = b.ilo + t; gj = b.jhi + 1; break;
+        }
+        const int s = OF.at(gi, gj);
+        if (s > 0) {
+          const Box& sb = F.box[s - 1];
+          GhostRec r{};
+          r.kind = G_COPY;
+          r.s[0] = qidx(F.pi[s - 1], gi - sb.ilo + 1, gj - sb.jlo + 1);
+          const GhostRec tmp = cf_rec(P, side, t, qif, win);  // for dst only
+          r.dst = tmp.dst;
+          F.ghost.push_back(r);
+          F.klass.push_back(2);
+        } else {
+          F.ghost.push_back(cf_rec(P, side, t, qif, win));
+          F.klass.push_back(3);
+        }
+      }
+    }
+    const int cgi[4] = {b.ilo - 1, b.ihi + 1, b.ilo - 1, b.ihi + 1};
+    const int cgj[4] = {b.jlo - 1, b.jlo - 1, b.jhi + 1, b.jhi + 1};
+    const int cdst[4] = {qidx(P, 0, 0), qidx(P, mx + 1, 0), qidx(P, 0, my + 1), qidx(P, mx + 1, my + 1)};
+    const int cs0[4] = {first[2], first[2] + mx - 1, first[3], first[3] + mx - 1};
+    const int cs1[4] = {first[0], first[1], first[0] + my - 1, first[1] + my - 1};
+    const int cs2[4] = {qidx(P, 1, 1), qidx(P, mx, 1), qidx(P, 1, my), qidx(P, mx, my)};
+    // diagonal stencil sources of a coarse-fine corner (ebInterpCf.m:123-153)
+    const int cq2[4] = {qidx(P, 2, 2), qidx(P, mx - 1, 2), qidx(P, 2, my - 1), qidx(P, mx - 1, my - 1)};
+    const int cwi[4] = {0, cmx + 1, 0, cmx + 1};
+    const int cwj[4] = {0, 0, cmy + 1, cmy + 1};
+    for (int c = 0; c < 4; ++c) {
+      GhostRec r{};
+      r.dst = cdst[c];
+      const int gi = cgi[c], gj = cgj[c];
+      int klass;
+      if (gi < 1 || gi > F.n[0] || gj < 1 || gj > F.n[1]) {
+        klass = 1;
+        r.kind = G_PCORNER; r.s[0] = cs0[c]; r.s[1] = cs1[c]; r.s[2] = cs2[c];
+      } else {
+        const int s = OF.at(gi, gj);
+        if (s > 0) {
+          klass = 2;
+          const Box& sb = F.box[s - 1];
+          r.kind = G_COPY;
+          r.s[0] = qidx(F.pi[s - 1], gi - sb.ilo + 1, gj - sb.jlo + 1);
+        } else {
+          klass = 3;
+          anycf = true;
+          if (qif) {
+            r.kind = G_QIF; r.s[0] = cq2[c]; r.s[1] = cs2[c]; r.s[2] = win(cwi[c], cwj[c]);
+          } else {
+            r.kind = G_LIF_CORNER; r.s[0] = cs2[c]; r.s[2] = win(cwi[c], cwj[c]);
+          }
+        }
+      }
+      F.ghost.push_back(r);
+      F.klass.push_back((int8_t)klass);
+    }
+    const bool fint = P.bc[0] == 0 || P.bc[1] == 0 || P.bc[2] == 0 || P.bc[3] == 0;
+    F.pi[p].cf = (fint || anycf) ? 1 : 0;
+    // ---- ring gathers (ebBuildOps.m:120-122)
+    for (int wj = 0; wj < cmy + 2; ++wj)
+      for (int wi = 0; wi < cmx + 2; ++wi) {
+        if (!(wi == 0 || wj == 0 || wi == cmx + 1 || wj == cmy + 1)) continue;
+        RingRec rr{P.qcoff + wi + wj * ldc, win(wi, wj)};
+        F.ring_all.push_back(rr);
+        if (F.pi[p].cf) F.ring_cf.push_back(rr);
+      }
+    // ---- footprint: parent cell, F index, parent face coefficients
+    //      (ebBuildOps.m:144-156, ebGatherParentCoeffs.m:294-324; the last
+    //      parent patch in order wins on shared faces, as in the reference)
+    std::vector<int> pdew((size_t)(cmx + 1) * cmy, -1), pdns((size_t)cmx * (cmy + 1), -1);
+    for (int q = 0; q < C.np; ++q) {
+      const Box& qb = C.box[q];
+      const PatchInfo& Q = C.pi[q];
+      const int mqx = qb.ihi - qb.ilo + 1;
+      for (int gj = std::max(Jl, qb.jlo); gj <= std::min(Jh, qb.jhi); ++gj)
+        for (int f = std::max(Il, qb.ilo); f <= std::min(Ih + 1, qb.ihi + 1); ++f)
+          pdew[(f - Il) + (size_t)(gj - Jl) * (cmx + 1)] = Q.xoff + (f - qb.ilo) + (gj - qb.jlo) * (mqx + 1);
+      for (int g = std::max(Jl, qb.jlo); g <= std::min(Jh + 1, qb.jhi + 1); ++g)
+        for (int gi = std::max(Il, qb.ilo); gi <= std::min(Ih, qb.ihi); ++gi)
+          pdns[(gi - Il) + (size_t)(g - Jl) * cmx] = Q.yoff + (gi - qb.ilo) + (g - qb.jlo) * mqx;
+    }
+    for (int v : pdew)
+      if (v < 0) return fail(BSAM_E_COVER, "eb_gather_parent_coeffs:cover: footprint of patch %d not covered by parent level.", p + 1);
+    for (int v : pdns)
+      if (v < 0) return fail(BSAM_E_COVER, "eb_gather_parent_coeffs:cover: footprint of patch %d not covered by parent level.", p + 1);
+    for (int cj = 0; cj < cmy; ++cj)
+      for (int ci = 0; ci < cmx; ++ci) {
+        FootRec fr{};
+        fr.p = p; fr.ci = ci; fr.cj = cj;
+        fr.pq = win(ci + 1, cj + 1);
+        const int q = OC.at(Il + ci, Jl + cj) - 1;
+        const Box& qb = C.box[q];
+        fr.pc = cidx(C.pi[q], Il + ci - qb.ilo, Jl + cj - qb.jlo);
+        fr.dew_w = pdew[ci + (size_t)cj * (cmx + 1)];
+        fr.dew_e = pdew[ci + 1 + (size_t)cj * (cmx + 1)];
+        fr.dns_s = pdns[ci + (size_t)cj * cmx];
+        fr.dns_n = pdns[ci + (size_t)(cj + 1) * cmx];
+        F.foot.push_back(fr);
+      }
+    // ---- reflux entries (ebBuildOps.m:158-166, ebFaceCorrections.m:107-133)
+    const int ko = F.klass_off.back();
+    for (int side = 0; side < 4; ++side) {
+      const int clen = side < 2 ? cmy : cmx;
+      const int koff = ko + (side == 0 ? 0 : side == 1 ? my : side == 2 ? 2 * my : 2 * my + mx);
+      for (int j0 = 0; j0 < clen; ++j0) {
+        if (F.klass[koff + 2 * j0 + 1] != 3) continue;
+        int gi, gj;
+        switch (side) {
+          case 0: gi = Il - 1; gj = Jl + j0; break;
+          case 1: gi = Ih + 1; gj = Jl + j0; break;
+          case 2: gi = Il + j0; gj = Jl - 1; break;
+          default: gi = Il + j0; gj = Jh + 1; break;
+        }
+        const int q = OC.at(gi, gj);
+        if (q == 0) return fail(BSAM_E_CORR, "eb_build_ops:corr: mass-correction target cell uncovered.");
+        const Box& qb = C.box[q - 1];
+        CorrRec e{};
+        e.face = side;
+        const int a = 2 * j0, bb = 2 * j0 + 1;  // fine rows/cols of the two sub-faces (0-based)
+        switch (side) {
+          case 0:
+            e.d1 = P.xoff + 0 + a * (mx + 1); e.d2 = P.xoff + 0 + bb * (mx + 1);
+            e.in1 = qidx(P, 1, a + 1); e.g1 = qidx(P, 0, a + 1); e.in2 = qidx(P, 1, bb + 1); e.g2 = qidx(P, 0, bb + 1);
+            e.ring = P.qcoff + 0 + (j0 + 1) * ldc; e.qcin = P.qcoff + 1 + (j0 + 1) * ldc;
+            e.cf = C.DeW[pdew[0 + (size_t)j0 * (cmx + 1)]];
+            break;
+          case 1:
+            e.d1 = P.xoff + mx + a * (mx + 1); e.d2 = P.xoff + mx + bb * (mx + 1);
+            e.in1 = qidx(P, mx, a + 1); e.g1 = qidx(P, mx + 1, a + 1); e.in2 = qidx(P, mx, bb + 1); e.g2 = qidx(P, mx + 1, bb + 1);
+            e.ring = P.qcoff + (cmx + 1) + (j0 + 1) * ldc; e.qcin = P.qcoff + cmx + (j0 + 1) * ldc;
+            e.cf = C.DeW[pdew[cmx + (size_t)j0 * (cmx + 1)]];
+            break;
+          case 2:
+            e.d1 = P.yoff + a + 0 * mx; e.d2 = P.yoff + bb + 0 * mx;
+            e.in1 = qidx(P, a + 1, 1); e.g1 = qidx(P, a + 1, 0); e.in2 = qidx(P, bb + 1, 1); e.g2 = qidx(P, bb + 1, 0);
+            e.ring = P.qcoff + (j0 + 1) + 0 * ldc; e.qcin = P.qcoff + (j0 + 1) + 1 * ldc;
+            e.cf = C.DnS[pdns[j0 + (size_t)0 * cmx]];
+            break;
+          default:
+            e.d1 = P.yoff + a + my * mx; e.d2 = P.yoff + bb + my * mx;
+            e.in1 = qidx(P, a + 1, my); e.g1 = qidx(P, a + 1, my + 1); e.in2 = qidx(P, bb + 1, my); e.g2 = qidx(P, bb + 1, my + 1);
+            e.ring = P.qcoff + (j0 + 1) + (cmy + 1) * ldc; e.qcin = P.qcoff + (j0 + 1) + cmy * ldc;
+            e.cf = C.DnS[pdns[j0 + (size_t)cmy * cmx]];
+            break;
+        }
+        F.corr.push_back(e);
+        F.corr_target.push_back(cidx(C.pi[q - 1], gi - qb.ilo, gj - qb.jlo));
+      }
+    }
+  }
+  // ---- targets on the parent level (sequential order per target cell)
+  C.tgt_cell.clear(); C.tgt_ptr.clear(); C.tgt_ent.clear();
+  C.tgt_of_cell.assign(C.ncell, -1);
+  std::vector<std::vector<int>> rows;
+  for (int e = 0; e < (int)F.corr.size(); ++e) {
+    const int cell = F.corr_target[e];
+    int row = C.tgt_of_cell[cell];
+    if (row < 0) {
+      row = (int)rows.size();
+      C.tgt_of_cell[cell] = row;
+      C.tgt_cell.push_back(cell);
+      rows.emplace_back();
+    }
+    rows[row].push_back(e);
+  }
+  C.tgt_ptr.push_back(0);
+  for (auto& r : rows) {
+    for (int e : r) C.tgt_ent.push_back(e);
+    C.tgt_ptr.push_back((int)C.tgt_ent.size());
+  }
+  C.vis.assign(C.ncell, 1);
+  for (int c = 0; c < C.ncell; ++c)
+    if (cov[c]) C.vis[c] = 0;
+  return 0;
+}
+
+// ------------------------------------------------------------------ coarsest
+// Dense ghost-eliminated level-1 matrix (ebCoarsest.m:14-58, duplicates summed
+// in triplet order) and its explicit inverse by LU with partial pivoting.
+int coarsest_assemble(const HLevel& L, bool singular, std::vector<double>& Ainv,
+                      std::vector<double>& bcRhs, int& M) {
+  const int n1 = L.n[0], n2 = L.n[1], N = n1 * n2;
+  const double hx = L.h[0], hy = L.h[1], hx2 = hx * hx, hy2 = hy * hy;
+  const PatchInfo& P = L.pi[0];
+  M = singular ? N + 1 : N;
+  std::vector<double> A((size_t)M * M, 0.0);  // row-major during elimination
+  auto at = [&](int r, int c) -> double& { return A[(size_t)r * M + c]; };
+  bcRhs.assign(N, 0.0);
+  auto idx = [&](int i, int j) { return i + j * n1; };
+  const double* DeW = &L.DeW[P.xoff];
+  const double* DnS = &L.DnS[P.yoff];
+  const double* CC = &L.CC[P.coff];
+  for (int c = 0; c < N; ++c) at(c, c) = at(c, c) + CC[c];
+  // internal x-faces: (iL,iL,+c) all, (iR,iR,+c) all, (iL,iR,-c) all, (iR,iL,-c) all
+  {
+    std::vector<double> cv;
+    std::vector<int> iL, iR;
+    for (int j = 0; j < n2; ++j)
+      for (int f = 1; f < n1; ++f) {
+        cv.push_back(DeW[f + j * (n1 + 1)] / hx2);
+        iL.push_back(idx(f - 1, j));
+        iR.push_back(idx(f, j));
+      }
+    for (size_t t = 0; t < cv.size(); ++t) at(iL[t], iL[t]) = at(iL[t], iL[t]) + cv[t];
+    for (size_t t = 0; t < cv.size(); ++t) at(iR[t], iR[t]) = at(iR[t], iR[t]) + cv[t];
+    for (size_t t = 0; t < cv.size(); ++t) at(iL[t], iR[t]) = at(iL[t], iR[t]) + (-cv[t]);
+    for (size_t t = 0; t < cv.size(); ++t) at(iR[t], iL[t]) = at(iR[t], iL[t]) + (-cv[t]);
+  }
+  {
+    std::vector<double> cv;
+    std::vector<int> iL, iR;
+    for (int f = 1; f < n2; ++f)
+      for (int i = 0; i < n1; ++i) {
+        cv.push_back(DnS[i + f * n1] / hy2);
+        iL.push_back(idx(i, f - 1));
+        iR.push_back(idx(i, f));
+      }
+    for (size_t t = 0; t < cv.size(); ++t) at(iL[t], iL[t]) = at(iL[t], iL[t]) + cv[t];
+    for (size_t t = 0; t < cv.size(); ++t) at(iR[t], iR[t]) = at(iR[t], iR[t]) + cv[t];
+    for (size_t t = 0; t < cv.size(); ++t) at(iL[t], iR[t]) = at(iL[t], iR[t]) + (-cv[t]);
+    for (size_t t = 0; t < cv.size(); ++t) at(iR[t], iL[t]) = at(iR[t], iL[t]) + (-cv[t]);
+  }
+  const double* G = &L.G[P.goff];
+  for (int side = 0; side < 4; ++side) {
+    const int code = P.bc[side];
+    const int len = side < 2 ? n2 : n1;
+    const double h = side < 2 ? hx : hy, h2 = side < 2 ? hx2 : hy2;
+    const double* g = G + (side == 0 ? 0 : side == 1 ? n2 : side == 2 ? 2 * n2 : 2 * n2 + n1);
+    std::vector<double> d(len);
+    std::vector<int> i1(len), i2(len);
+    for (int t = 0; t < len; ++t) {
+      switch (side) {
+        case 0: d[t] = DeW[0 + t * (n1 + 1)]; i1[t] = idx(0, t); i2[t] = idx(1, t); break;
+        case 1: d[t] = DeW[n1 + t * (n1 + 1)]; i1[t] = idx(n1 - 1, t); i2[t] = idx(n1 - 2, t); break;
+        case 2: d[t] = DnS[t + 0 * n1]; i1[t] = idx(t, 0); i2[t] = idx(t, 1); break;
+        default: d[t] = DnS[t + n2 * n1]; i1[t] = idx(t, n2 - 1); i2[t] = idx(t, n2 - 2); break;
+      }
+    }
+    if (code == 1) {
+      for (int t = 0; t < len; ++t) at(i1[t], i1[t]) = at(i1[t], i1[t]) + 3 * d[t] / h2;
+      for (int t = 0; t < len; ++t) at(i1[t], i2[t]) = at(i1[t], i2[t]) + (-d[t] / (3 * h2));
+      for (int t = 0; t < len; ++t) bcRhs[i1[t]] = bcRhs[i1[t]] + (8.0 / 3.0) * d[t] * g[t] / h2;
+    } else if (code == 2) {
+      for (int t = 0; t < len; ++t) bcRhs[i1[t]] = bcRhs[i1[t]] + d[t] * g[t] / h;
+    }
+  }
+  if (singular) {
+    for (int c = 0; c < N; ++c) {
+      at(c, N) = 1.0;
+      at(N, c) = 1.0;
+    }
+  }
+  // LU with partial pivoting, first maximum wins
+  std::vector<int> perm(M);
+  for (int i = 0; i < M; ++i) perm[i] = i;
+  for (int k = 0; k < M; ++k) {
+    int piv = k;
+    double best = std::fabs(at(k, k));
+    for (int i = k + 1; i < M; ++i)
+      if (std::fabs(at(i, k)) > best) { best = std::fabs(at(i, k)); piv = i; }
+    if (at(piv, k) == 0.0) return fail(BSAM_E_SINGULAR, "bsamgpu:coarsest: level-1 matrix is singular");
+    if (piv != k) {
+      for (int c = 0; c < M; ++c) std::swap(at(k, c), at(piv, c));
+      std::swap(perm[k], perm[piv]);
+    }
+    const double akk = at(k, k);
+    for (int i = k + 1; i < M; ++i) at(i, k) = at(i, k) / akk;
+    for (int i = k + 1; i < M; ++i) {
+      const double l = at(i, k);
+      for (int c = k + 1; c < M; ++c) at(i, c) = at(i, c) - l * at(k, c);
+    }
+  }
+  std::vector<double> X((size_t)M * M, 0.0);  // row-major
+  for (int i = 0; i < M; ++i) X[(size_t)i * M + perm[i]] = 1.0;
+  for (int k = 0; k < M; ++k)
+    for (int i = k + 1; i < M; ++i) {
+      const double l = at(i, k);
+      for (int c = 0; c < M; ++c) X[(size_t)i * M + c] = X[(size_t)i * M + c] - l * X[(size_t)k * M + c];
+    }
+  for (int k = M - 1; k >= 0; --k) {
+    const double ukk = at(k, k);
+    for (int c = 0; c < M; ++c) X[(size_t)k * M + c] = X[(size_t)k * M + c] / ukk;
+    for (int i = 0; i < k; ++i) {
+      const double u = at(i, k);
+      for (int c = 0; c < M; ++c) X[(size_t)i * M + c] = X[(size_t)i * M + c] - u * X[(size_t)k * M + c];
+    }
+  }
+  Ainv.assign((size_t)M * M, 0.0);  // column-major for the device
+  for (int i = 0; i < M; ++i)
+    for (int c = 0; c < M; ++c) Ainv[i + (size_t)c * M] = X[(size_t)i * M + c];
+  return 0;
+}
+
+// ------------------------------------------------------------------ validation
+// Every index a kernel dereferences is checked against its buffer before any
+// upload, so a table bug is reported as an error instead of reaching the GPU.
+int validate_level(const HLevel& L, const HLevel* C, const HLevel* Fn) {
+  auto bad = [&](const char* what, long long v, long long n) {
+    return fail(BSAM_E_ARG, "bsamgpu:internal: table %s index %lld outside [0,%lld)", what, v, n);
+  };
+  auto in = [](long long v, long long n) { return v >= 0 && v < n; };
+  for (const int2& t : L.tiles) {
+    if (!in(t.x, L.np)) return bad("tile patch", t.x, L.np);
+    if (!in(t.y, (long long)L.pi[t.x].mx * L.pi[t.x].my)) return bad("tile cell", t.y, (long long)L.pi[t.x].mx * L.pi[t.x].my);
+  }
+  for (const PatchInfo& P : L.pi) {
+    if (P.qoff + (P.mx + 2) * (P.my + 2) > L.nq || P.coff + P.mx * P.my > L.ncell ||
+        P.xoff + (P.mx + 1) * P.my > L.nx || P.yoff + P.mx * (P.my + 1) > L.ny || P.goff + 2 * (P.mx + P.my) > L.ng)
+      return fail(BSAM_E_ARG, "bsamgpu:internal: patch offsets exceed level buffers");
+    if (C && P.qcoff + (P.mx / 2 + 2) * (P.my / 2 + 2) > L.nqc)
+      return fail(BSAM_E_ARG, "bsamgpu:internal: QC offsets exceed level buffer");
+  }
+  const long long nrec = (long long)L.ghost.size();
+  const long long pq = C ? C->nq : 0;
+  for (const GhostRec& r : L.ghost) {
+    if (!in(r.dst, L.nq)) return bad("ghost dst", r.dst, L.nq);
+    switch (r.kind) {
+      case G_COPY: case G_NEU: if (!in(r.s[0], L.nq)) return bad("ghost src", r.s[0], L.nq); break;
+      case G_DIR: if (!in(r.s[0], L.nq) || !in(r.s[1], L.nq)) return bad("ghost src", r.s[0], L.nq); break;
+      case G_QIF: case G_LIF:
+        if (!in(r.s[0], L.nq) || !in(r.s[1], L.nq)) return bad("ghost src", r.s[1], L.nq);
+        if (!in(r.s[2], pq)) return bad("ghost parent src", r.s[2], pq);
+        break;
+      case G_LIF_CORNER:
+        if (!in(r.s[0], L.nq)) return bad("ghost src", r.s[0], L.nq);
+        if (!in(r.s[2], pq)) return bad("ghost parent src", r.s[2], pq);
+        break;
+      case G_QIF_END:
+        if (!in(r.s[0], L.nq) || !in(r.s[1], L.nq)) return bad("ghost src", r.s[0], L.nq);
+        for (int i = 2; i < 5; ++i) if (!in(r.s[i], pq)) return bad("ghost parent src", r.s[i], pq);
+        break;
+      case G_PCORNER:
+        if (!in(r.s[0], nrec) || !in(r.s[1], nrec)) return bad("corner record", r.s[0], nrec);
+        if (L.ghost[r.s[0]].kind == G_PCORNER || L.ghost[r.s[1]].kind == G_PCORNER)
+          return fail(BSAM_E_ARG, "bsamgpu:internal: corner refers to a corner");
+        if (!in(r.s[2], L.nq)) return bad("corner src", r.s[2], L.nq);
+        break;
+      default: return fail(BSAM_E_ARG, "bsamgpu:internal: ghost kind %d", r.kind);
+    }
+  }
+  for (const auto* rl : {&L.ring_all, &L.ring_cf})
+    for (const RingRec& r : *rl) {
+      if (!in(r.dst, L.nqc)) return bad("ring dst", r.dst, L.nqc);
+      if (!in(r.src, pq)) return bad("ring src", r.src, pq);
+    }
+  for (int v : L.wmap) if (!in(v, pq)) return bad("window", v, pq);
+  for (const FootRec& f : L.foot) {
+    if (!in(f.p, L.np)) return bad("foot patch", f.p, L.np);
+    if (!in(f.ci, L.pi[f.p].mx / 2) || !in(f.cj, L.pi[f.p].my / 2)) return bad("foot cell", f.ci, L.pi[f.p].mx / 2);
+    if (!in(f.pq, pq)) return bad("foot parent Q", f.pq, pq);
+    if (!in(f.pc, C->ncell)) return bad("foot parent cell", f.pc, C->ncell);
+    if (!in(f.dew_w, C->nx) || !in(f.dew_e, C->nx)) return bad("foot DeW", f.dew_w, C->nx);
+    if (!in(f.dns_s, C->ny) || !in(f.dns_n, C->ny)) return bad("foot DnS", f.dns_s, C->ny);
+  }
+  for (const CorrRec& e : L.corr) {
+    const long long nd = e.face < 2 ? L.nx : L.ny;
+    if (!in(e.d1, nd) || !in(e.d2, nd)) return bad("corr D", e.d1, nd);
+    for (int v : {e.in1, e.g1, e.in2, e.g2}) if (!in(v, L.nq)) return bad("corr Q", v, L.nq);
+    if (!in(e.ring, L.nqc) || !in(e.qcin, L.nqc)) return bad("corr QC", e.ring, L.nqc);
+  }
+  const long long nfc = Fn ? (long long)Fn->corr.size() : 0;
+  if (!L.tgt_cell.empty() && (long long)L.tgt_ptr.size() != (long long)L.tgt_cell.size() + 1)
+    return fail(BSAM_E_ARG, "bsamgpu:internal: target row pointer size");
+  for (int c : L.tgt_cell) if (!in(c, L.ncell)) return bad("target cell", c, L.ncell);
+  for (int e : L.tgt_ent) if (!in(e, nfc)) return bad("target entry", e, nfc);
+  for (size_t i = 0; i + 1 < L.tgt_ptr.size(); ++i)
+    if (L.tgt_ptr[i] > L.tgt_ptr[i + 1] || L.tgt_ptr[i + 1] > (int)L.tgt_ent.size())
+      return fail(BSAM_E_ARG, "bsamgpu:internal: target rows not monotone");
+  for (int v : L.tgt_of_cell) if (v >= (long long)L.tgt_cell.size()) return bad("target row", v, (long long)L.tgt_cell.size());
+  if ((long long)L.vis.size() != L.ncell || (long long)L.tgt_of_cell.size() != L.ncell)
+    return fail(BSAM_E_ARG, "bsamgpu:internal: per-cell map sizes");
+  return 0;
+}
+
+// ------------------------------------------------------------------ upload
+template <typename T, typename U>
+int upload(bsam_hier* h, T** dst, const std::vector<U>& src) {
+  int rc = h->mem.alloc(dst, src.size());
+  if (rc) return rc;
+  if (!src.empty()) HIPCHK(hipMemcpy(*dst, src.data(), src.size() * sizeof(U), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename T>
+int upload_cast(bsam_hier* h, T** dst, const std::vector<double>& src) {
+  std::vector<T> tmp(src.begin(), src.end());
+  return upload(h, dst, tmp);
+}
+
+template <typename T>
+int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, const LevelDev<double>* d64) {
+  D.n[0] = L.n[0]; D.n[1] = L.n[1]; D.h[0] = L.h[0]; D.h[1] = L.h[1];
+  D.np = L.np; D.ncell = L.ncell; D.nq = L.nq; D.nx = L.nx; D.ny = L.ny; D.nqc = L.nqc; D.ng = L.ng;
+  D.ntile = (int)L.tiles.size(); D.nghost = (int)L.ghost.size();
+  D.nring_all = (int)L.ring_all.size(); D.nring_cf = (int)L.ring_cf.size();
+  D.nfoot = (int)L.foot.size(); D.ncorr = (int)L.corr.size(); D.ntgt = (int)L.tgt_cell.size();
+  int rc;
+  if (tables_from64) {  // share the index tables of the FP64 level
+    D.P = d64->P; D.tiles = d64->tiles; D.ghost = d64->ghost; D.ring_all = d64->ring_all;
+    D.ring_cf = d64->ring_cf; D.foot = d64->foot; D.corr = d64->corr; D.tgt_cell = d64->tgt_cell;
+    D.tgt_ptr = d64->tgt_ptr; D.tgt_ent = d64->tgt_ent; D.tgt_of_cell = d64->tgt_of_cell;
+    D.wmap = d64->wmap; D.vis = d64->vis;
+  } else {
+    if ((rc = upload(h, &D.P, L.pi))) return rc;
+    if ((rc = upload(h, &D.tiles, L.tiles))) return rc;
+    if ((rc = upload(h, &D.ghost, L.ghost))) return rc;
+    if ((rc = upload(h, &D.ring_all, L.ring_all))) return rc;
+    if ((rc = upload(h, &D.ring_cf, L.ring_cf))) return rc;
+    if ((rc = upload(h, &D.foot, L.foot))) return rc;
+    if ((rc = upload(h, &D.corr, L.corr))) return rc;
+    if ((rc = upload(h, &D.tgt_cell, L.tgt_cell))) return rc;
+    std::vector<int> ptr = L.tgt_ptr.empty() ? std::vector<int>{0} : L.tgt_ptr;
+    if ((rc = upload(h, &D.tgt_ptr, ptr))) return rc;
+    if ((rc = upload(h, &D.tgt_ent, L.tgt_ent))) return rc;
+    if ((rc = upload(h, &D.tgt_of_cell, L.tgt_of_cell))) return rc;
+    if ((rc = upload(h, &D.wmap, L.wmap))) return rc;
+    if ((rc = upload(h, &D.vis, L.vis))) return rc;
+  }
+  if ((rc = upload_cast(h, &D.Q, L.Q))) return rc;
+  if ((rc = upload_cast(h, &D.FT, L.FT))) return rc;
+  if ((rc = upload_cast(h, &D.F, L.FT))) return rc;
+  if ((rc = upload_cast(h, &D.CC, L.CC))) return rc;
+  if ((rc = upload_cast(h, &D.ID, L.ID))) return rc;
+  if ((rc = upload_cast(h, &D.DeW, L.DeW))) return rc;
+  if ((rc = upload_cast(h, &D.DnS, L.DnS))) return rc;
+  if ((rc = upload_cast(h, &D.G, L.G))) return rc;
+  std::vector<double> zqc(L.nqc, 0.0);
+  if ((rc = upload_cast(h, &D.QC, zqc))) return rc;
+  return 0;
+}
+
+// ------------------------------------------------------------------ sequences
+template <typename T>
+struct Seq {
+  bsam_hier* h;
+  std::vector<LevelDev<T>>& L;
+  int homog;
+  hipStream_t s;
+  const LevelDev<T>& lev(int k) { return L[k - 1]; }
+  void fill(int k) { launch_fill<T>(lev(k), k > 1 ? &L[k - 2] : nullptr, homog, s); }
+  void relax(int k, int nsweeps) {
+    for (int it = 0; it < nsweeps; ++it)
+      for (int colour = 1; colour <= 2; ++colour) {
+        launch_relax<T>(lev(k), colour, (T)h->opts.omega, homog, s);
+        fill(k);
+      }
+  }
+  void restrict_(int k) { launch_restrict<T>(lev(k), lev(k - 1), 0, s); }
+  void ring(int k) { launch_ring<T>(lev(k), lev(k - 1), s); }
+  void coarse_load(int k) { launch_coarse_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s); }
+  void correct(int k) { launch_correct<T>(lev(k), lev(k - 1), 0, s); }
+  void coarsest() {
+    launch_coarsest<T>(h->cst, lev(1), homog, s);
+    fill(1);
+  }
+  void vcycle(int k) {  // ebVcycle.m:15-35
+    if (k == 1) {
+      coarsest();
+      return;
+    }
+    relax(k, h->opts.npre);
+    restrict_(k);
+    fill(k - 1);
+    ring(k);
+    coarse_load(k);
+    vcycle(k - 1);
+    correct(k);
+    fill(k);
+    relax(k, h->opts.npost);
+  }
+  void filldown() {  // ebTransfer.m:73-79
+    for (int kk = h->nlev; kk >= 2; --kk) {
+      restrict_(kk);
+      fill(kk - 1);
+    }
+  }
+  int reset_force() {
+    for (auto& l : L)
+      if (l.ncell) {
+        hipError_t e = hipMemcpyAsync(l.F, l.FT, (size_t)l.ncell * sizeof(T), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return fail(BSAM_E_HIP, "bsamgpu:hip: F reset failed: %s", hipGetErrorString(e));
+      }
+    return 0;
+  }
+};
+
+// composite residual (ebCompResidual.m:23-84) on the FP64 hierarchy; writes
+// per-cell r into rout (FP32 inner right side) when given.
+void enqueue_residual(bsam_hier* h, float* const* rout, hipStream_t s) {
+  auto& L = h->d64;
+  const int r0 = h->root;
+  if (h->opts.mass_correction) {
+    for (int k = std::max(2, r0 + 1); k <= h->nlev; ++k) {
+      launch_ring<double>(L[k - 1], L[k - 2], s);
+      launch_restrict<double>(L[k - 1], L[k - 2], 1, s);  // QC interior only
+    }
+  }
+  for (int k = r0; k <= h->nlev; ++k) {
+    const LevelDev<double>* fine = k < h->nlev ? &L[k] : nullptr;
+    launch_residual<double, float>(L[k - 1], fine, h->opts.mass_correction, h->parts + h->part_off[k - 1],
+                                   rout ? rout[k - 1] : nullptr, s);
+  }
+  launch_reduce(h->parts + h->part_off[r0 - 1], h->nparts - h->part_off[r0 - 1], h->dres, s);
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+extern "C" {
+
+void bsam_default_opts(bsam_solve_opts* o) {
+  if (!o) return;
+  o->npre = 3; o->npost = 3; o->omega = 1.0; o->rel_tol = 1e-9; o->max_vcycles = 60;
+  o->ghost_interp = 0; o->mass_correction = 1; o->precision = BSAM_FP64; o->inner_cycles = 1;
+  o->use_graph = 1;
+}
+
+const char* bsam_last_error(void) { return g_err.c_str(); }
+const char* bsam_version(void) { return "bsamgpu 0.1 (gfx950)"; }
+
+int bsam_ctx_create(int hip_device, bsam_ctx** out) {
+  if (!out) return fail(BSAM_E_ARG, "bsamgpu:arg: null output");
+  if (hip_device == -1) {  // host-only context: tables and exports, no device
+    auto* c = new bsam_ctx;
+    c->device = -1;
+    *out = c;
+    return 0;
+  }
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (hip_device < 0 || hip_device >= n) return fail(BSAM_E_ARG, "bsamgpu:arg: device %d of %d", hip_device, n);
+  HIPCHK(hipSetDevice(hip_device));
+  auto* c = new bsam_ctx;
+  c->device = hip_device;
+  *out = c;
+  return 0;
+}
+
+void bsam_ctx_destroy(bsam_ctx* ctx) { delete ctx; }
+
+static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_solve_opts* o, bsam_hier* h) {
+  if (!d || d->nlev < 1 || d->root_idx < 1 || d->root_idx > d->nlev || !d->n || !d->np || !d->boxes ||
+      !d->bc_code || !d->Q || !d->FTrue || !d->DeW || !d->DnS || !d->CC || !d->InvDiagS || !d->g)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: incomplete hierarchy descriptor");
+  h->ctx = ctx;
+  h->nlev = d->nlev;
+  h->root = d->root_idx;
+  for (int i = 0; i < 4; ++i) h->dom[i] = d->domain[i];
+  if (o) h->opts = *o; else bsam_default_opts(&h->opts);
+  if (ctx->device >= 0) HIPCHK(hipSetDevice(ctx->device));
+  h->H.resize(h->nlev);
+  size_t pb = 0, oq = 0, oc = 0, ox = 0, oy = 0, og = 0;
+  for (int k = 0; k < h->nlev; ++k) {
+    HLevel& L = h->H[k];
+    L.n[0] = d->n[2 * k]; L.n[1] = d->n[2 * k + 1];
+    if (L.n[0] < 2 || L.n[1] < 2) return fail(BSAM_E_ARG, "bsamgpu:arg: level %d grid too small", k + 1);
+    L.h[0] = (d->domain[1] - d->domain[0]) / L.n[0];
+    L.h[1] = (d->domain[3] - d->domain[2]) / L.n[1];
+    L.np = d->np[k];
+    if (L.np < 1) return fail(BSAM_E_ARG, "bsamgpu:arg: level %d has no patches", k + 1);
+    for (int p = 0; p < L.np; ++p, ++pb) {
+      Box b{d->boxes[4 * pb], d->boxes[4 * pb + 1], d->boxes[4 * pb + 2], d->boxes[4 * pb + 3]};
+      if (b.ilo < 1 || b.ihi > L.n[0] || b.jlo < 1 || b.jhi > L.n[1] || b.ilo > b.ihi || b.jlo > b.jhi)
+        return fail(BSAM_E_BOX, "eb_make_level:box: invalid patch box.");
+      PatchInfo P{};
+      P.mx = b.ihi - b.ilo + 1; P.my = b.jhi - b.jlo + 1;
+      if (k > 0 && (P.mx % 2 || P.my % 2))
+        return fail(BSAM_E_ALIGN, "eb_build_ops:align: patch not aligned to parent cells.");
+      if (P.mx < 2 || P.my < 2) return fail(BSAM_E_BOX, "eb_make_level:box: patch narrower than 2 cells.");
+      P.par0 = (b.ilo + b.jlo) % 2;
+      for (int s = 0; s < 4; ++s) {
+        P.bc[s] = d->bc_code[4 * pb + s];
+        if (P.bc[s] < 0 || P.bc[s] > 2) return fail(BSAM_E_ARG, "bsamgpu:arg: bad bc code");
+      }
+      const bool edge[4] = {b.ilo == 1, b.ihi == L.n[0], b.jlo == 1, b.jhi == L.n[1]};
+      for (int s = 0; s < 4; ++s)
+        if ((P.bc[s] != 0) != edge[s])
+          return fail(BSAM_E_ARG, "bsamgpu:arg: bc code of side %d inconsistent with the box", s);
+      P.qoff = L.nq; P.coff = L.ncell; P.xoff = L.nx; P.yoff = L.ny; P.goff = L.ng;
+      P.qcoff = L.nqc;
+      L.nq += (P.mx + 2) * (P.my + 2);
+      L.ncell += P.mx * P.my;
+      L.nx += (P.mx + 1) * P.my;
+      L.ny += P.mx * (P.my + 1);
+      L.ng += 2 * (P.mx + P.my);
+      if (k > 0) L.nqc += (P.mx / 2 + 2) * (P.my / 2 + 2);
+      L.box.push_back(b);
+      L.pi.push_back(P);
+    }
+    L.Q.assign(d->Q + oq, d->Q + oq + L.nq);
+    L.FT.assign(d->FTrue + oc, d->FTrue + oc + L.ncell);
+    L.CC.assign(d->CC + oc, d->CC + oc + L.ncell);
+    L.ID.assign(d->InvDiagS + oc, d->InvDiagS + oc + L.ncell);
+    L.DeW.assign(d->DeW + ox, d->DeW + ox + L.nx);
+    L.DnS.assign(d->DnS + oy, d->DnS + oy + L.ny);
+    L.G.assign(d->g + og, d->g + og + L.ng);
+    oq += L.nq; oc += L.ncell; ox += L.nx; oy += L.ny; og += L.ng;
+    for (double v : L.DeW) {
+      if (!std::isfinite(v)) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: coefficient function returned non-finite values.");
+      if (v <= 0) return fail(BSAM_E_D, "eb_make_level:D: D(x,y) must be strictly positive at faces.");
+    }
+    for (double v : L.DnS) {
+      if (!std::isfinite(v)) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: coefficient function returned non-finite values.");
+      if (v <= 0) return fail(BSAM_E_D, "eb_make_level:D: D(x,y) must be strictly positive at faces.");
+    }
+    for (const auto* a : {&L.CC, &L.FT, &L.ID, &L.Q, &L.G})
+      for (double v : *a)
+        if (!std::isfinite(v)) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: coefficient function returned non-finite values.");
+    L.vis.assign(L.ncell, 1);
+    L.tgt_of_cell.assign(L.ncell, -1);
+    make_tiles(L);
+  }
+  if (h->H[0].np != 1) return fail(BSAM_E_ARG, "bsamgpu:arg: level 1 must be one full-domain patch");
+  const bool qif = h->opts.ghost_interp == 0;
+  build_phys_level(h->H[0]);
+  for (int k = 1; k < h->nlev; ++k) {
+    int rc = build_level_tables(h->H[k], h->H[k - 1], qif);
+    if (rc) return rc;
+  }
+  // singular iff all-Neumann level 1 and C == 0 everywhere (ebCoarsest.m:58-66)
+  bool allneu = true;
+  for (int s = 0; s < 4; ++s) allneu = allneu && h->H[0].pi[0].bc[s] != 1;
+  double cmax = 0;
+  for (auto& L : h->H)
+    for (double v : L.CC) cmax = std::max(cmax, std::fabs(v));
+  h->singular = allneu && cmax == 0;
+  std::vector<double> Ainv, bcRhs;
+  int M = 0;
+  int rc = coarsest_assemble(h->H[0], h->singular, Ainv, bcRhs, M);
+  if (rc) return rc;
+  for (int k = 0; k < h->nlev; ++k)
+    if ((rc = validate_level(h->H[k], k > 0 ? &h->H[k - 1] : nullptr, k + 1 < h->nlev ? &h->H[k + 1] : nullptr)))
+      return rc;
+  h->cst.N = h->H[0].n[0] * h->H[0].n[1];
+  h->cst.M = M;
+  if (ctx->device < 0) return 0;  // host-only: no device state
+  if ((rc = upload(h, &h->cst.Ainv, Ainv))) return rc;
+  if ((rc = upload(h, &h->cst.bcRhs, bcRhs))) return rc;
+  // device levels
+  h->d64.resize(h->nlev);
+  for (int k = 0; k < h->nlev; ++k)
+    if ((rc = make_dev_level<double>(h, h->H[k], h->d64[k], false, nullptr))) return rc;
+  h->part_off.resize(h->nlev + 1);
+  h->nparts = 0;
+  for (int k = 0; k < h->nlev; ++k) {
+    h->part_off[k] = h->nparts;
+    h->nparts += (int)h->H[k].tiles.size();
+  }
+  h->part_off[h->nlev] = h->nparts;
+  if ((rc = h->mem.alloc(&h->parts, std::max(1, h->nparts)))) return rc;
+  if ((rc = h->mem.alloc(&h->dres, 1))) return rc;
+  HIPCHK(hipHostMalloc((void**)&h->hres, sizeof(ResidualOut), hipHostMallocDefault));
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  h->cell_base.resize(h->nlev + 1);
+  h->ncell_all = 0;
+  for (int k = 0; k < h->nlev; ++k) {
+    h->cell_base[k] = h->ncell_all;
+    h->ncell_all += h->H[k].ncell;
+  }
+  h->cell_base[h->nlev] = h->ncell_all;
+  return 0;
+}
+
+static int need_dev(const bsam_hier* h) {
+  if (!h) return fail(BSAM_E_ARG, "bsamgpu:arg: null hierarchy");
+  if (h->ctx->device < 0) return fail(BSAM_E_ARG, "bsamgpu:arg: host-only hierarchy has no device state");
+  return 0;
+}
+
+int bsam_hier_create(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_solve_opts* o, bsam_hier** out) {
+  if (!ctx || !out) return fail(BSAM_E_ARG, "bsamgpu:arg: null context or output");
+  auto h = std::make_unique<bsam_hier>();
+  int rc = hier_create_impl(ctx, d, o, h.get());
+  if (rc) return rc;
+  *out = h.release();
+  return 0;
+}
+
+void bsam_hier_destroy(bsam_hier* h) {
+  if (!h) return;
+  if (h->ctx->device >= 0) (void)hipSetDevice(h->ctx->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  delete h;
+}
+
+static int ensure32(bsam_hier* h) {
+  if (h->have32) return 0;
+  h->d32.resize(h->nlev);
+  for (int k = 0; k < h->nlev; ++k) {
+    int rc = make_dev_level<float>(h, h->H[k], h->d32[k], true, &h->d64[k]);
+    if (rc) return rc;
+    launch_fill_value<float>(h->d32[k].FT, h->H[k].ncell, 0.0f, h->stream);
+  }
+  h->have32 = true;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+static int check_level(const bsam_hier* h, int k, int lo) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (h->ctx->device < 0) return fail(BSAM_E_ARG, "bsamgpu:arg: host-only hierarchy has no device state");
+  if (k < lo || k > h->nlev) return fail(BSAM_E_ARG, "bsamgpu:arg: level %d out of range", k);
+  return 0;
+}
+
+static int sync(bsam_hier* h) {
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int bsam_fill_ghosts(bsam_hier* h, int k) {
+  if (int rc = check_level(h, k, 1)) return rc;
+  Seq<double>{h, h->d64, 0, h->stream}.fill(k);
+  return sync(h);
+}
+
+int bsam_relax_level(bsam_hier* h, int k, int nsweeps) {
+  if (int rc = check_level(h, k, 1)) return rc;
+  Seq<double>{h, h->d64, 0, h->stream}.relax(k, nsweeps);
+  return sync(h);
+}
+
+int bsam_transfer(bsam_hier* h, int k, int what) {
+  Seq<double> s{h, h->d64, 0, h->stream};
+  if (what == BSAM_FILLDOWN) {
+    s.filldown();
+    return sync(h);
+  }
+  if (int rc = check_level(h, k, 2)) return rc;
+  switch (what) {
+    case BSAM_RESTRICT: s.restrict_(k); break;
+    case BSAM_RING: s.ring(k); break;
+    case BSAM_CORRECT: s.correct(k); break;
+    default: return fail(BSAM_E_ARG, "eb_transfer:what: unknown transfer %d", what);
+  }
+  return sync(h);
+}
+
+int bsam_coarse_load(bsam_hier* h, int k) {
+  if (int rc = check_level(h, k, 2)) return rc;
+  Seq<double>{h, h->d64, 0, h->stream}.coarse_load(k);
+  return sync(h);
+}
+
+int bsam_coarsest_solve(bsam_hier* h) {
+  if (int rc_ = need_dev(h)) return rc_;
+  Seq<double>{h, h->d64, 0, h->stream}.coarsest();
+  return sync(h);
+}
+
+int bsam_vcycle(bsam_hier* h) {
+  if (int rc_ = need_dev(h)) return rc_;
+  Seq<double>{h, h->d64, 0, h->stream}.vcycle(h->nlev);
+  return sync(h);
+}
+
+int bsam_reset_force(bsam_hier* h) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (int rc = Seq<double>{h, h->d64, 0, h->stream}.reset_force()) return rc;
+  return sync(h);
+}
+
+int bsam_init_level_prolong(bsam_hier* h, int k) {
+  if (int rc = check_level(h, k, 2)) return rc;
+  Seq<double> s{h, h->d64, 0, h->stream};
+  s.fill(k - 1);
+  launch_correct<double>(h->d64[k - 1], h->d64[k - 2], 1, h->stream);
+  s.fill(k);
+  return sync(h);
+}
+
+static int read_residual(bsam_hier* h, bsam_residual* out) {
+  HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (out) {
+    out->linf = h->hres->linf;
+    out->l2 = std::sqrt(h->hres->l2sq);
+    out->fnorm = std::max(h->hres->fnorm, 2.2250738585072014e-308);
+    out->rel = out->linf / out->fnorm;
+    out->ncell = h->hres->ncell;
+  }
+  return 0;
+}
+
+int bsam_comp_residual(bsam_hier* h, bsam_residual* out) {
+  if (int rc_ = need_dev(h)) return rc_;
+  enqueue_residual(h, nullptr, h->stream);
+  HIPCHK(hipGetLastError());
+  return read_residual(h, out);
+}
+
+// one cycle of the plain FP64 loop: reset F, V-cycle, filldown, residual,
+// residual scalars to pinned host memory
+static int enqueue_cycle64(bsam_hier* h, hipStream_t s) {
+  Seq<double> q{h, h->d64, 0, s};
+  if (int rc = q.reset_force()) return rc;
+  q.vcycle(h->nlev);
+  q.filldown();
+  enqueue_residual(h, nullptr, s);
+  HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+// one step of the mixed loop: FP32 V-cycles on the correction problem with the
+// current FP64 composite residual as right side, FP64 update, filldown,
+// ghost refresh and composite residual (which also writes the next right side)
+static int enqueue_cycle_mixed(bsam_hier* h, hipStream_t s) {
+  Seq<float> q{h, h->d32, 1, s};
+  for (int k = 0; k < h->nlev; ++k) {
+    launch_fill_value<float>(h->d32[k].Q, h->H[k].nq, 0.0f, s);
+    launch_fill_value<float>(h->d32[k].QC, h->H[k].nqc, 0.0f, s);
+  }
+  for (int k = 1; k <= h->nlev; ++k) q.fill(k);
+  for (int it = 0; it < std::max(1, h->opts.inner_cycles); ++it) {
+    if (int rc = q.reset_force()) return rc;
+    q.vcycle(h->nlev);
+    q.filldown();
+  }
+  for (int k = 0; k < h->nlev; ++k) launch_add_correction(h->d64[k], h->d32[k], s);
+  Seq<double> d{h, h->d64, 0, s};
+  d.filldown();
+  for (int k = 1; k <= h->nlev; ++k) d.fill(k);
+  std::vector<float*> rout(h->nlev);
+  for (int k = 0; k < h->nlev; ++k) rout[k] = h->d32[k].FT;
+  enqueue_residual(h, rout.data(), s);
+  HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, s));
+  return 0;
+}
+
+static int run_cycle(bsam_hier* h, bool mixed) {
+  hipStream_t s = h->stream;
+  if (!h->opts.use_graph) {
+    int rc = mixed ? enqueue_cycle_mixed(h, s) : enqueue_cycle64(h, s);
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  const int key = (mixed ? 1 : 0) | (h->opts.inner_cycles << 1) | (h->opts.npre << 8) | (h->opts.npost << 12) |
+                  (h->opts.mass_correction << 16) | (h->opts.omega == 1.0 ? 0 : (1 << 17));
+  if (!h->gexec || h->graph_key != key || h->graph_omega != h->opts.omega) {
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = mixed ? enqueue_cycle_mixed(h, s) : enqueue_cycle64(h, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return fail(BSAM_E_HIP, "bsamgpu:hip: graph capture failed: %s", hipGetErrorString(e));
+    h->graph = g;
+    HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+    h->graph_key = key;
+    h->graph_omega = h->opts.omega;
+  }
+  HIPCHK(hipGraphLaunch(h->gexec, s));
+  return 0;
+}
+
+int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* out) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (o) {
+    if (o->ghost_interp != h->opts.ghost_interp)
+      return fail(BSAM_E_ARG, "bsamgpu:arg: GhostInterp is fixed at hierarchy creation");
+    h->opts = *o;
+  }
+  if (h->opts.npre < 0 || h->opts.npost < 0 || h->opts.max_vcycles < 0)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: negative sweep or cycle count");
+  const bool mixed = h->opts.precision == BSAM_MIXED;
+  if (mixed)
+    if (int rc = ensure32(h)) return rc;
+  HIPCHK(hipSetDevice(h->ctx->device));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, h->stream));
+  // ebSolveCycles.m:11-19: fill every level, baseline residual
+  Seq<double> d{h, h->d64, 0, h->stream};
+  for (int k = 1; k <= h->nlev; ++k) d.fill(k);
+  std::vector<float*> rout(h->nlev, nullptr);
+  if (mixed)
+    for (int k = 0; k < h->nlev; ++k) rout[k] = h->d32[k].FT;
+  enqueue_residual(h, mixed ? rout.data() : nullptr, h->stream);
+  HIPCHK(hipGetLastError());
+  bsam_residual res{};
+  if (int rc = read_residual(h, &res)) return rc;
+  std::vector<double> hist{res.linf};
+  double rel = res.rel;
+  long long ncell = res.ncell;
+  int it = 0;
+  while (it < h->opts.max_vcycles && rel > h->opts.rel_tol) {
+    ++it;
+    if (int rc = run_cycle(h, mixed)) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const ResidualOut r = *h->hres;
+    const double fn = std::max(r.fnorm, 2.2250738585072014e-308);
+    hist.push_back(r.linf);
+    rel = r.linf / fn;
+    ncell = r.ncell;
+    if (!std::isfinite(rel)) break;
+  }
+  HIPCHK(hipEventRecord(e1, h->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (out) {
+    out->amr_levels = h->nlev - h->root + 1;
+    out->nlev = h->nlev;
+    out->vcycles = it;
+    out->final_rel = rel;
+    out->visible_cells = ncell;
+    long long tot = 0;
+    for (int k = h->root; k <= h->nlev; ++k) tot += h->H[k - 1].ncell;
+    out->total_cells = tot;
+    out->avg_factor = hist.size() >= 2
+                          ? std::pow(hist.back() / std::max(hist.front(), 2.2250738585072014e-308), 1.0 / (hist.size() - 1))
+                          : NAN;
+    out->res_hist_len = 0;
+    if (out->res_hist)
+      for (size_t i = 0; i < hist.size() && (int)i < out->res_hist_cap; ++i) {
+        out->res_hist[i] = hist[i];
+        out->res_hist_len = (int)i + 1;
+      }
+    out->gpu_ms = ms;
+  }
+  return 0;
+}
+
+int bsam_composite_mean(bsam_hier* h, double* mean, double* area) {
+  if (int rc_ = need_dev(h)) return rc_;
+  for (int k = h->root; k <= h->nlev; ++k) launch_mass<double>(h->d64[k - 1], h->parts + h->part_off[k - 1], h->stream);
+  launch_reduce(h->parts + h->part_off[h->root - 1], h->nparts - h->part_off[h->root - 1], h->dres, h->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double ar = 0;
+  for (int k = h->root; k <= h->nlev; ++k) {
+    long long nv = 0;
+    for (uint8_t v : h->H[k - 1].vis) nv += v;
+    ar += nv * (h->H[k - 1].h[0] * h->H[k - 1].h[1]);
+  }
+  if (area) *area = ar;
+  if (mean) *mean = h->hres->l2sq / std::max(ar, 2.2250738585072014e-308);
+  return 0;
+}
+
+long long bsam_level_field_size(const bsam_hier* h, int k, int field) {
+  if (!h || k < 1 || k > h->nlev) return -1;
+  const HLevel& L = h->H[k - 1];
+  switch (field) {
+    case BSAM_Q: return L.nq;
+    case BSAM_F: case BSAM_FTRUE: return L.ncell;
+    case BSAM_QC: return L.nqc;
+    default: return -1;
+  }
+}
+
+static double* field_ptr(bsam_hier* h, int k, int field) {
+  LevelDev<double>& D = h->d64[k - 1];
+  switch (field) {
+    case BSAM_Q: return D.Q;
+    case BSAM_F: return D.F;
+    case BSAM_FTRUE: return D.FT;
+    case BSAM_QC: return D.QC;
+    default: return nullptr;
+  }
+}
+
+int bsam_get_level(bsam_hier* h, int k, int field, double* out) {
+  if (int rc = check_level(h, k, 1)) return rc;
+  const long long n = bsam_level_field_size(h, k, field);
+  if (n < 0 || !out) return fail(BSAM_E_ARG, "bsamgpu:arg: bad field");
+  if (n == 0) return 0;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(out, field_ptr(h, k, field), n * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int bsam_set_level(bsam_hier* h, int k, int field, const double* in) {
+  if (int rc = check_level(h, k, 1)) return rc;
+  const long long n = bsam_level_field_size(h, k, field);
+  if (n < 0 || !in) return fail(BSAM_E_ARG, "bsamgpu:arg: bad field");
+  if (n == 0) return 0;
+  for (long long i = 0; i < n; ++i)
+    if (!std::isfinite(in[i])) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: non-finite input values.");
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(field_ptr(h, k, field), in, n * sizeof(double), hipMemcpyHostToDevice));
+  if (field == BSAM_FTRUE && h->have32)
+    launch_convert<float, double>(h->d32[k - 1].FT, h->d64[k - 1].FT, n, h->stream);
+  return sync(h);
+}
+
+long long bsam_export_size(const bsam_hier* h, int k, int what) {
+  if (!h || k < 1 || k > h->nlev) return -1;
+  const HLevel& L = h->H[k - 1];
+  switch (what) {
+    case 0: return (long long)L.klass.size();
+    case 1: return k >= 2 ? (long long)L.nqc : 0;
+    case 2: return 2 * (long long)L.corr.size();
+    default: return -1;
+  }
+}
+
+int bsam_export(const bsam_hier* h, int k, int what, long long* out) {
+  const long long n = bsam_export_size(h, k, what);
+  if (n < 0 || !out) return fail(BSAM_E_ARG, "bsamgpu:arg: bad export request");
+  const HLevel& L = h->H[k - 1];
+  switch (what) {
+    case 0:
+      for (long long i = 0; i < n; ++i) out[i] = L.klass[i];
+      break;
+    case 1:
+      for (long long i = 0; i < n; ++i) {
+        const int q = L.wmap_patch[i];
+        out[i] = q < 0 ? -1 : (long long)(L.wmap[i] - h->H[k - 2].pi[q].qoff) + 1000000LL * q;
+      }
+      break;
+    case 2:
+      for (size_t e = 0; e < L.corr.size(); ++e) {
+        out[2 * e] = L.corr_target[e];
+        out[2 * e + 1] = L.corr[e].face;
+      }
+      break;
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ parabolic
+static int ensure_history(bsam_hier* h) {
+  if (h->Uo) return 0;
+  int rc;
+  if ((rc = h->mem.alloc(&h->Uo, h->ncell_all))) return rc;
+  if ((rc = h->mem.alloc(&h->Uoo, h->ncell_all))) return rc;
+  if ((rc = h->mem.alloc(&h->fbuf, h->ncell_all))) return rc;
+  return 0;
+}
+
+int bsam_parabolic_init(bsam_hier* h) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (int rc = ensure_history(h)) return rc;
+  for (int k = 0; k < h->nlev; ++k)
+    launch_history(h->d64[k], h->Uo + h->cell_base[k], h->Uoo + h->cell_base[k], 0, h->stream);
+  return sync(h);
+}
+
+int bsam_parabolic_advance(bsam_hier* h) {
+  if (!h || !h->Uo) return fail(BSAM_E_ARG, "bsamgpu:arg: history not initialised");
+  for (int k = 0; k < h->nlev; ++k)
+    launch_history(h->d64[k], h->Uo + h->cell_base[k], h->Uoo + h->cell_base[k], 1, h->stream);
+  return sync(h);
+}
+
+int bsam_parabolic_rhs(bsam_hier* h, const double* f, double dt, int first_step) {
+  if (!h || !h->Uo) return fail(BSAM_E_ARG, "bsamgpu:arg: history not initialised");
+  if (!(dt > 0)) return fail(BSAM_E_ARG, "ebp_options:Dt: Dt must be positive");
+  const double* fd = nullptr;
+  if (f) {
+    HIPCHK(hipMemcpyAsync(h->fbuf, f, h->ncell_all * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    fd = h->fbuf;
+  }
+  // sub-root levels keep their FTrue (ebpRhs.m:138-140)
+  for (int k = h->root; k <= h->nlev; ++k) {
+    const long long b = h->cell_base[k - 1];
+    launch_bdf2_rhs(h->d64[k - 1].FT, fd ? fd + b : nullptr, h->Uo + b, h->Uoo + b, dt, first_step,
+                    h->H[k - 1].ncell, h->stream);
+    if (h->have32) launch_convert<float, double>(h->d32[k - 1].FT, h->d64[k - 1].FT, h->H[k - 1].ncell, h->stream);
+  }
+  return sync(h);
+}
+
+int bsam_set_reaction_shift(bsam_hier* h, double shift) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (!std::isfinite(shift)) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: non-finite shift");
+  if (int rc = ensure32(h)) return rc;
+  for (int k = 0; k < h->nlev; ++k) {
+    launch_add_scalar(h->d64[k].CC, h->H[k].ncell, shift, h->d32[k].CC, h->stream);
+    launch_invdiag(h->d64[k], h->d32[k].ID, h->stream);
+    for (double& v : h->H[k].CC) v += shift;
+  }
+  if (int rc = sync(h)) return rc;
+  // refactor the coarsest matrix with the shifted reaction
+  bool allneu = true;
+  for (int s = 0; s < 4; ++s) allneu = allneu && h->H[0].pi[0].bc[s] != 1;
+  double cmax = 0;
+  for (auto& L : h->H)
+    for (double v : L.CC) cmax = std::max(cmax, std::fabs(v));
+  const bool singular = allneu && cmax == 0;
+  std::vector<double> Ainv, bcRhs;
+  int M = 0;
+  if (int rc = coarsest_assemble(h->H[0], singular, Ainv, bcRhs, M)) return rc;
+  if (M != h->cst.M) return fail(BSAM_E_ARG, "bsamgpu:arg: reaction shift changes the coarsest system's order");
+  h->singular = singular;
+  HIPCHK(hipMemcpy(h->cst.Ainv, Ainv.data(), Ainv.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(h->cst.bcRhs, bcRhs.data(), bcRhs.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+/* bsamgpu -- MI355X-native adaptive FAS multigrid V-cycle (C ABI).
+ *
+ * The library owns a device-resident copy of a block-structured patch
+ * hierarchy and runs the solve path of the reference MATLAB code on it:
+ * red-black Gauss-Seidel smoothing, QIF/LIF coarse-fine ghost fill,
+ * restriction, ring gather, FAS coarse loading with flux-balance (reflux)
+ * corrections, bilinear correction, the coarsest direct solve and the
+ * composite residual.  Tagging, clustering, regridding and coefficient
+ * evaluation stay with the caller (MATLAB through the MEX gateway, the plain
+ * C driver, or the Python host layer).
+ *
+ * Conventions
+ *  - Levels are numbered 1..nlev as in the reference (level 1 is the coarsest
+ *    16x16-type grid, level root_idx the root, levels above it adaptive).
+ *  - Patch boxes are 1-based inclusive [ilo ihi jlo jhi] in the level's
+ *    global cell indices (EllipticBSAM/src/ebMakeLevel.m:1-4).
+ *  - Arrays are FP64, column-major per patch (x index fastest), packed back to
+ *    back in patch order and level order 1..nlev:
+ *      Q        (mx+2)*(my+2)  padded solution
+ *      FTrue    mx*my          source
+ *      DeW      (mx+1)*my      D on x-faces
+ *      DnS      mx*(my+1)      D on y-faces
+ *      CC       mx*my          reaction
+ *      InvDiagS mx*my          1/(diagonal incl. the physical-boundary fold)
+ *      g        my+my+mx+mx    boundary data W,E,S,N (zeros on internal faces)
+ *  - The caller owns every host buffer; each call copies what it needs.
+ *  - Every function returns 0 on success or a negative BSAM_E_* code;
+ *    bsam_last_error() returns the thread's last message, prefixed with the
+ *    reference's error identifier (e.g. "eb_build_ops:nesting").
+ *  - Calls on distinct hierarchies are independent (each has its own HIP
+ *    stream); calls on one hierarchy must not overlap.
+ */
+#ifndef BSAMGPU_H
+#define BSAMGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSAM_OK 0
+#define BSAM_E_ARG (-1)
+#define BSAM_E_ALIGN (-2)      /* eb_build_ops:align        ebBuildOps.m:38-42 */
+#define BSAM_E_NESTING (-3)    /* eb_build_ops:nesting      ebBuildOps.m:138-142 */
+#define BSAM_E_RING (-4)       /* eb_build_ops:ring         ebBuildOps.m:233-236 */
+#define BSAM_E_OVERHANG (-5)   /* eb_build_ops:overhang     ebBuildOps.m:240-242 */
+#define BSAM_E_CORR (-6)       /* eb_build_ops:corr         ebBuildOps.m:269-271 */
+#define BSAM_E_COVER (-7)      /* eb_gather_parent_coeffs:cover  ebGatherParentCoeffs.m:327-330 */
+#define BSAM_E_BOX (-8)        /* eb_make_level:box         ebMakeLevel.m:86-89 */
+#define BSAM_E_D (-9)          /* eb_make_level:D           ebMakeLevel.m:112-114 */
+#define BSAM_E_NONFINITE (-10) /* eb_make_level:eval        ebMakeLevel.m:171-173 */
+#define BSAM_E_SINGULAR (-11)  /* coarsest matrix singular */
+#define BSAM_E_HIP (-20)       /* HIP runtime failure */
+#define BSAM_E_NOMEM (-21)
+
+enum bsam_field { BSAM_Q = 0, BSAM_F = 1, BSAM_FTRUE = 2, BSAM_QC = 3 };
+enum bsam_transfer_kind { BSAM_RESTRICT = 0, BSAM_RING = 1, BSAM_CORRECT = 2, BSAM_FILLDOWN = 3 };
+enum bsam_precision { BSAM_FP64 = 0, BSAM_MIXED = 1 };
+
+typedef struct bsam_ctx bsam_ctx;
+typedef struct bsam_hier bsam_hier;
+
+/* The hierarchy value H handed to ebSolveCycles (EllipticBSAM/src/ebSetup.m:10-68,
+ * ebMakeLevel.m:78-143).  Descriptor tables (ghost maps, ring/window gathers,
+ * scatters, reflux targets, covered masks, the coarsest factorisation) are
+ * re-derived from the boxes inside the library. */
+typedef struct {
+  int nlev;
+  int root_idx;
+  double domain[4];
+  const int *n;        /* [2*nlev]   cells per level (nx, ny) */
+  const int *np;       /* [nlev]     patches per level */
+  const int *boxes;    /* [4*sum np] */
+  const int *bc_code;  /* [4*sum np] W E S N: 0 internal, 1 Dirichlet, 2 Neumann */
+  const double *Q;
+  const double *FTrue;
+  const double *DeW;
+  const double *DnS;
+  const double *CC;
+  const double *InvDiagS;
+  const double *g;
+} bsam_hier_desc;
+
+/* Solver options (EllipticBSAM/src/ebOptions.m:44-66) plus device choices. */
+typedef struct {
+  int npre;            /* NPre  (3) */
+  int npost;           /* NPost (3) */
+  double omega;        /* Omega (1.0) */
+  double rel_tol;      /* RelTol */
+  int max_vcycles;     /* MaxVCycles */
+  int ghost_interp;    /* 0 = 'qif', 1 = 'lif' */
+  int mass_correction; /* MassCorrection (1) */
+  int precision;       /* BSAM_FP64 or BSAM_MIXED (FP32 inner V-cycles, FP64 defect correction) */
+  int inner_cycles;    /* FP32 V-cycles per defect-correction step (mixed only, >= 1) */
+  int use_graph;       /* replay each cycle as a captured HIP graph */
+} bsam_solve_opts;
+
+/* One solve stage (EllipticBSAM/src/ebSolveCycles.m:58-70) plus device timing. */
+typedef struct {
+  int amr_levels;
+  int nlev;
+  int vcycles;
+  double final_rel;
+  long long visible_cells;
+  long long total_cells;
+  double avg_factor;
+  double *res_hist;    /* caller buffer for |r|_inf per cycle (may be NULL) */
+  int res_hist_cap;    /* its capacity; res_hist_len <= cap entries are written */
+  int res_hist_len;
+  double gpu_ms;       /* device time of the cycle loop (HIP events) */
+} bsam_stage_stats;
+
+/* EllipticBSAM/src/ebCompResidual.m:328-332 */
+typedef struct {
+  double linf, l2, fnorm, rel;
+  long long ncell;
+} bsam_residual;
+
+void bsam_default_opts(bsam_solve_opts *o);
+const char *bsam_last_error(void);
+const char *bsam_version(void);
+
+int bsam_ctx_create(int hip_device, bsam_ctx **out);
+void bsam_ctx_destroy(bsam_ctx *ctx);
+
+/* Upload H and derive its tables: ebBuildOps.m:17-199, ebGatherParentCoeffs.m:286-337,
+ * ebCoarsest.m:14-77 ('assemble'). */
+int bsam_hier_create(bsam_ctx *ctx, const bsam_hier_desc *desc, const bsam_solve_opts *opts,
+                     bsam_hier **out);
+void bsam_hier_destroy(bsam_hier *h);
+
+/* == H = ebSolveCycles(H)   EllipticBSAM/src/ebSolveCycles.m:1-80 */
+int bsam_solve_cycles(bsam_hier *h, const bsam_solve_opts *opts, bsam_stage_stats *out);
+/* == [res,H] = ebCompResidual(H)   EllipticBSAM/src/ebCompResidual.m:1-85 */
+int bsam_comp_residual(bsam_hier *h, bsam_residual *out);
+/* == H = ebVcycle(H, nlev)   EllipticBSAM/src/ebVcycle.m:1-36 */
+int bsam_vcycle(bsam_hier *h);
+/* == H = ebFillGhosts(H, k)   EllipticBSAM/src/ebFillGhosts.m:1-95 */
+int bsam_fill_ghosts(bsam_hier *h, int k);
+/* == H = ebRelaxLevel(H, k, nsweeps)   EllipticBSAM/src/ebRelaxLevel.m:1-39 */
+int bsam_relax_level(bsam_hier *h, int k, int nsweeps);
+/* == H = ebTransfer(H, k, what)   EllipticBSAM/src/ebTransfer.m:1-84 */
+int bsam_transfer(bsam_hier *h, int k, int what);
+/* == H = ebCoarseLoad(H, k)   EllipticBSAM/src/ebCoarseLoad.m:1-47 */
+int bsam_coarse_load(bsam_hier *h, int k);
+/* == H = ebCoarsest(H, 'solve')   EllipticBSAM/src/ebCoarsest.m:79-91 */
+int bsam_coarsest_solve(bsam_hier *h);
+/* F = FTrue on every level (ebSolveCycles.m:30-34) */
+int bsam_reset_force(bsam_hier *h);
+/* ebAddLevel.m:368-375: fill level k-1 ghosts, prolong its window into level k, fill k */
+int bsam_init_level_prolong(bsam_hier *h, int k);
+/* == [m, area] = ebCompositeMean(H)   EllipticBSAM/src/ebCompositeMean.m:1-24 */
+int bsam_composite_mean(bsam_hier *h, double *mean, double *area);
+
+/* Packed per-level field access (layout as in the descriptor; QC is
+ * (mx/2+2)*(my/2+2) per patch, levels >= 2). */
+long long bsam_level_field_size(const bsam_hier *h, int k, int field);
+int bsam_get_level(bsam_hier *h, int k, int field, double *out);
+int bsam_set_level(bsam_hier *h, int k, int field, const double *in);
+
+/* Table export for parity checks against the reference descriptors.
+ * what = 0: ghost classes per patch [W(my) E(my) S(mx) N(mx) SW SE NW NE]
+ *           (1 physical, 2 same-level, 3 coarse-fine: ebBuildOps.m:47-113 land/cornerClass)
+ * what = 1: per QC cell of level k the parent Q element index (patch-local
+ *           padded linear index + 1e6*parent patch), ebBuildOps.m:115-142
+ * what = 2: reflux entries of level k in order: (target parent cell index
+ *           in the packed level-(k-1) cell layout, face 0..3) pairs,
+ *           ebBuildOps.m:158-166 */
+long long bsam_export_size(const bsam_hier *h, int k, int what);
+int bsam_export(const bsam_hier *h, int k, int what, long long *out);
+
+/* BDF2 right-hand side on levels >= root (ParabolicBSAM/src/ebpRhs.m:36,151):
+ * FTrue = f + (2 Uo - Uoo/2)/dt with Uo/Uoo the device-resident history; with
+ * first_step != 0 the BE bootstrap FTrue = f + Uo/dt.  f is packed like FTrue
+ * (NULL means f = 0). */
+int bsam_parabolic_rhs(bsam_hier *h, const double *f, double dt, int first_step);
+/* Rotate the history after an accepted step: Uoo = Uo, Uo = interior(Q) (ebpSolve.m:56-57) */
+int bsam_parabolic_advance(bsam_hier *h);
+/* Initialise the history from the current Q (Uo = Uoo = interior(Q)) */
+int bsam_parabolic_init(bsam_hier *h);
+/* Reaction shift C_eff = C + shift on every level, refreshing InvDiagS on the
+ * device (ParabolicBSAM/src/ebpSetTime.m:11-55 for time-independent D, C). */
+int bsam_set_reaction_shift(bsam_hier *h, double shift);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BSAMGPU_H */
