@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark: starfish 64 -> 1024 composite solve to 1e-10 on one MI355X.
+
+Workload (the diffuse-domain starfish transmission problem, 5 AMR levels,
+64^2 root refined to 1024^2 spacing near the interface, 7 multigrid levels,
+544,664 stored cells, 408,562 visible): one step = one composite solve from
+the zero initial guess to relative composite residual <= 1e-10 (ebSolveCycles:
+fill, baseline residual, V-cycles + filldown + residual per cycle) with the
+hierarchy resident in HBM.
+
+Reported: solves/s (value), seconds per V-cycle, p50/p99 solve latency, the
+smoother's achieved bandwidth against 8 TB/s (roofline of the dominant kernel,
+the finest-level smoother, timed with HIP events on the library stream), the
+FP32-inner / FP64-defect-correction form with its error against the FP64
+solution, and the CPU oracle timed on this host as the baseline.
+
+  python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+  python bench.py --kernel-only      # only the timed smoother launches (for rocprofv3)
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+import warnings
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bsam-in-matlab_amd"))
+sys.path.insert(0, ROOT)
+warnings.filterwarnings("ignore")
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_starfish():
+    from bsamgpu import configs, solve
+
+    prob, aux, geom = configs.starfish_problem()
+    opts = configs.starfish_options(geom)
+    return solve.build_hierarchy(prob, opts), opts
+
+
+def time_solves(G, steps, warmup, precision):
+    for _ in range(warmup):
+        G.reset()
+        G.solve_cycles(precision=precision)
+    lat = []
+    stages = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        G.reset()
+        st = G.solve_cycles(precision=precision)
+        lat.append((time.perf_counter() - t) * 1e3)
+        stages.append(st)
+    total = time.perf_counter() - t0
+    return total, lat, stages
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    i = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[i]
+
+
+def visible_rel_l2(H, Qa, Qb):
+    num = 0.0
+    den = 0.0
+    for k in range(H.rootIdx, H.nlev + 1):
+        L = H.L(k)
+        da = L.h[0] * L.h[1]
+        for p in range(L.np):
+            vis = ~L.covered[p]
+            a = Qa[k][p][1:-1, 1:-1][vis]
+            b = Qb[k][p][1:-1, 1:-1][vis]
+            num += float(((a - b) ** 2).sum()) * da
+            den += float((b ** 2).sum()) * da
+    return (num / den) ** 0.5
+
+
+def cpu_baseline(H):
+    from oracle import bsam_oracle as O
+
+    Orc = O.OracleHierarchy(H)
+    t = time.perf_counter()
+    st = Orc.solve_cycles()
+    dt = time.perf_counter() - t
+    return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"one full starfish solve by the numpy oracle ({st['vcycles']} V-cycles to rel "
+                      f"{st['finalRel']:.2e}) = {dt:.1f} s, single thread, tables prebuilt",
+            "seconds_per_solve": dt, "seconds_per_vcycle": dt / max(st["vcycles"], 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true")
+    ap.add_argument("--smoother-launches", type=int, default=400)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+
+    from bsamgpu import _lib
+    from bsamgpu.device import GpuHierarchy
+
+    log("building starfish hierarchy on the host")
+    t = time.perf_counter()
+    H, opts = build_starfish()
+    log(f"hierarchy: {H.nlev} levels, patches {[H.L(k).np for k in range(1, H.nlev + 1)]} "
+        f"({time.perf_counter() - t:.1f} s)")
+    G = GpuHierarchy(H)
+    kf = H.nlev
+    sm_bytes = G.smoother_bytes(kf, 8)
+    if args.kernel_only:
+        G.time_smoother(kf, 20)
+        ms = G.time_smoother(kf, args.smoother_launches)
+        log(f"smoother level {kf}: {ms * 1e3:.2f} us/launch, {sm_bytes / ms / 1e6:.1f} GB/s")
+        return
+
+    log("FP64 solves")
+    tot64, lat64, st64 = time_solves(G, args.steps, args.warmup, "fp64")
+    s64 = st64[-1]
+    log(f"fp64: {s64['vcycles']} V-cycles, rel {s64['finalRel']:.3e}, {tot64 / args.steps * 1e3:.2f} ms/solve")
+    Q64 = {k: G.get_field(k, _lib.BSAM_Q) for k in range(1, H.nlev + 1)}
+
+    log("mixed-precision solves")
+    totmx, latmx, stmx = time_solves(G, args.steps, args.warmup, "mixed")
+    smx = stmx[-1]
+    Qmx = {k: G.get_field(k, _lib.BSAM_Q) for k in range(1, H.nlev + 1)}
+    err_mx = visible_rel_l2(H, Qmx, Q64)
+    log(f"mixed: {smx['vcycles']} V-cycles, rel {smx['finalRel']:.3e}, {totmx / args.steps * 1e3:.2f} ms/solve, "
+        f"rel L2 vs fp64 {err_mx:.2e}")
+
+    # dominant kernel: finest-level smoother, HIP events on the library stream
+    G.time_smoother(kf, 20)
+    ms64 = G.time_smoother(kf, args.smoother_launches)
+    G.time_smoother(kf, 20, mixed=True)
+    ms32 = G.time_smoother(kf, args.smoother_launches, mixed=True)
+    gbs64 = sm_bytes / ms64 / 1e6
+    gbs32 = G.smoother_bytes(kf, 4) / ms32 / 1e6
+    per_level = {}
+    for k in range(2, H.nlev + 1):
+        msk = G.time_smoother(k, 100)
+        per_level[str(k)] = round(G.smoother_bytes(k, 8) / msk / 1e6, 1)
+    log(f"smoother fp64 {ms64 * 1e3:.2f} us/launch {gbs64:.0f} GB/s; fp32 {ms32 * 1e3:.2f} us {gbs32:.0f} GB/s")
+
+    base = None
+    if not args.no_cpu_baseline:
+        log("CPU baseline (numpy oracle, one solve)")
+        base = cpu_baseline(H)
+        log(f"cpu: {base['seconds_per_solve']:.1f} s/solve")
+
+    ms_step = tot64 / args.steps * 1e3
+    out = {
+        "metric": "V-cycle s & smoother HBM GB/s at 1e-10 rel. residual; p50/p99 solve latency",
+        "value": round(args.steps / tot64, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: starfish transmission coefficients sampled on the host (no dataset)",
+        "config": {"workload": "starfish 64->1024 composite solve (5 AMR levels, 7 MG levels, "
+                               "544664 stored / 408562 visible cells) from zero to rel 1e-10",
+                   "levels": H.nlev, "patches_per_level": [H.L(k).np for k in range(1, H.nlev + 1)],
+                   "rel_tol": opts.RelTol},
+        "vcycles": s64["vcycles"],
+        "final_rel": s64["finalRel"],
+        "vcycle_s": round(ms_step / 1e3 / s64["vcycles"], 7),
+        "p50_ms": round(pct(lat64, 50), 3),
+        "p99_ms": round(pct(lat64, 99), 3),
+        "smoother_GBps": round(gbs64, 1),
+        "smoother_GBps_per_level": per_level,
+        "mixed": {"value": round(args.steps / totmx, 3), "ms_per_step": round(totmx / args.steps * 1e3, 3),
+                  "vcycles": smx["vcycles"], "final_rel": smx["finalRel"],
+                  "speedup_vs_fp64": round(tot64 / totmx, 3), "rel_l2_vs_fp64": err_mx,
+                  "p50_ms": round(pct(latmx, 50), 3), "p99_ms": round(pct(latmx, 99), 3),
+                  "smoother_fp32_GBps": round(gbs32, 1)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"k_relax<double> level {kf} ({H.L(kf).np} patches, "
+                               f"{sum(H.L(kf).dims(p)[0] * H.L(kf).dims(p)[1] for p in range(H.L(kf).np))} cells)",
+                     "bytes_per_launch": sm_bytes, "us_per_launch": round(ms64 * 1e3, 3)},
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -40,7 +40,8 @@ EXPORTS = [
     "bsam_fill_ghosts", "bsam_relax_level", "bsam_transfer", "bsam_coarse_load", "bsam_coarsest_solve",
     "bsam_reset_force", "bsam_init_level_prolong", "bsam_composite_mean", "bsam_level_field_size",
     "bsam_get_level", "bsam_set_level", "bsam_export_size", "bsam_export", "bsam_parabolic_rhs",
-    "bsam_parabolic_advance", "bsam_parabolic_init", "bsam_set_reaction_shift",
+    "bsam_parabolic_advance", "bsam_parabolic_init", "bsam_set_reaction_shift", "bsam_hier_reset",
+    "bsam_time_smoother", "bsam_smoother_bytes",
 ]
 
 
@@ -110,6 +111,9 @@ def load(path=None):
         "bsam_parabolic_advance": (c_int, [vp]),
         "bsam_parabolic_init": (c_int, [vp]),
         "bsam_set_reaction_shift": (c_int, [vp, c_double]),
+        "bsam_hier_reset": (c_int, [vp]),
+        "bsam_time_smoother": (c_int, [vp, c_int, c_int, c_int, P_double]),
+        "bsam_smoother_bytes": (c_double, [vp, c_int, c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -202,6 +202,18 @@ class GpuHierarchy:
             check(self.lib.bsam_export(self.h, k, what, out.ctypes.data_as(_lib.P_ll)), "bsam_export")
         return out
 
+    def reset(self):
+        check(self.lib.bsam_hier_reset(self.h), "bsam_hier_reset")
+
+    def time_smoother(self, k, nlaunch, mixed=False):
+        ms = ctypes.c_double()
+        check(self.lib.bsam_time_smoother(self.h, k, 1 if mixed else 0, nlaunch, ctypes.byref(ms)),
+              "bsam_time_smoother")
+        return ms.value
+
+    def smoother_bytes(self, k, elem_bytes=8):
+        return self.lib.bsam_smoother_bytes(self.h, k, elem_bytes)
+
     # ------------------------------------------------------------ parabolic
     def parabolic_init(self):
         check(self.lib.bsam_parabolic_init(self.h), "bsam_parabolic_init")

@@ -111,6 +111,7 @@ struct bsam_hier {
   int nparts = 0;
   ResidualOut* dres = nullptr;
   ResidualOut* hres = nullptr;             // pinned
+  std::vector<double*> Q0;                 // uploaded initial guess per level
   double* Uo = nullptr;                    // parabolic history (packed cells, all levels)
   double* Uoo = nullptr;
   double* fbuf = nullptr;
@@ -983,6 +984,9 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
   h->d64.resize(h->nlev);
   for (int k = 0; k < h->nlev; ++k)
     if ((rc = make_dev_level<double>(h, h->H[k], h->d64[k], false, nullptr))) return rc;
+  h->Q0.resize(h->nlev);
+  for (int k = 0; k < h->nlev; ++k)
+    if ((rc = upload(h, &h->Q0[k], h->H[k].Q))) return rc;
   h->part_off.resize(h->nlev + 1);
   h->nparts = 0;
   for (int k = 0; k < h->nlev; ++k) {
@@ -1367,6 +1371,53 @@ int bsam_export(const bsam_hier* h, int k, int what, long long* out) {
       break;
   }
   return 0;
+}
+
+int bsam_hier_reset(bsam_hier* h) {
+  if (int rc_ = need_dev(h)) return rc_;
+  for (int k = 0; k < h->nlev; ++k) {
+    LevelDev<double>& D = h->d64[k];
+    HIPCHK(hipMemcpyAsync(D.Q, h->Q0[k], (size_t)D.nq * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    if (D.ncell) HIPCHK(hipMemcpyAsync(D.F, D.FT, (size_t)D.ncell * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    launch_fill_value<double>(D.QC, D.nqc, 0.0, h->stream);
+  }
+  return sync(h);
+}
+
+double bsam_smoother_bytes(const bsam_hier* h, int k, int elem_bytes) {
+  if (!h || k < 1 || k > h->nlev) return -1;
+  double b = 0;
+  for (const PatchInfo& P : h->H[k - 1].pi) {
+    const double mx = P.mx, my = P.my;
+    b += (mx + 2) * (my + 2) + 3 * mx * my + (mx + 1) * my + mx * (my + 1);
+  }
+  return b * elem_bytes / 2.0;
+}
+
+int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* ms_per_launch) {
+  if (int rc = check_level(h, k, 1)) return rc;
+  if (nlaunch < 1 || !ms_per_launch) return fail(BSAM_E_ARG, "bsamgpu:arg: bad smoother timing request");
+  if (precision == BSAM_MIXED)
+    if (int rc = ensure32(h)) return rc;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, h->stream));
+  for (int i = 0; i < nlaunch; ++i) {
+    const int colour = 1 + (i & 1);
+    if (precision == BSAM_MIXED)
+      launch_relax<float>(h->d32[k - 1], colour, (float)h->opts.omega, 1, h->stream);
+    else
+      launch_relax<double>(h->d64[k - 1], colour, h->opts.omega, 0, h->stream);
+  }
+  HIPCHK(hipEventRecord(e1, h->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per_launch = ms / nlaunch;
+  return sync(h);
 }
 
 // ------------------------------------------------------------------ parabolic

@@ -151,6 +151,20 @@ int bsam_init_level_prolong(bsam_hier *h, int k);
 /* == [m, area] = ebCompositeMean(H)   EllipticBSAM/src/ebCompositeMean.m:1-24 */
 int bsam_composite_mean(bsam_hier *h, double *mean, double *area);
 
+/* Restore the uploaded initial guess on every level (Q), F = FTrue, QC = 0:
+ * restarts a solve from the descriptor's state without host traffic. */
+int bsam_hier_reset(bsam_hier *h);
+
+/* Measurement helpers.  bsam_time_smoother launches `nlaunch` single-colour
+ * smoother passes (alternating red/black, no ghost refresh) over every patch
+ * of level k on the hierarchy's stream and returns the average launch time
+ * from HIP events; precision BSAM_MIXED times the FP32 kernel.
+ * bsam_smoother_bytes returns the algorithmic bytes of one colour pass:
+ * half of  b*sum_patches[(mx+2)(my+2) + 3 mx my + (mx+1) my + mx (my+1)]
+ * (read Q, F, InvDiagS, DeW, DnS, write Q per red+black sweep). */
+int bsam_time_smoother(bsam_hier *h, int k, int precision, int nlaunch, double *ms_per_launch);
+double bsam_smoother_bytes(const bsam_hier *h, int k, int elem_bytes);
+
 /* Packed per-level field access (layout as in the descriptor; QC is
  * (mx/2+2)*(my/2+2) per patch, levels >= 2). */
 long long bsam_level_field_size(const bsam_hier *h, int k, int field);
