@@ -45,17 +45,17 @@ def build_starfish():
     return solve.build_hierarchy(prob, opts), opts
 
 
-def time_solves(G, steps, warmup, precision):
+def time_solves(G, steps, warmup, precision, **kw):
     for _ in range(warmup):
         G.reset()
-        G.solve_cycles(precision=precision)
+        G.solve_cycles(precision=precision, **kw)
     lat = []
     stages = []
     t0 = time.perf_counter()
     for _ in range(steps):
         t = time.perf_counter()
         G.reset()
-        st = G.solve_cycles(precision=precision)
+        st = G.solve_cycles(precision=precision, **kw)
         lat.append((time.perf_counter() - t) * 1e3)
         stages.append(st)
     total = time.perf_counter() - t0
@@ -104,6 +104,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-only", action="store_true")
     ap.add_argument("--smoother-launches", type=int, default=400)
+    ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 solves (profiling)")
+    ap.add_argument("--smoother", default="exact", choices=["exact", "tiled"])
+    ap.add_argument("--coarse-cells", type=int, default=0)
+    ap.add_argument("--sweep", action="store_true", help="tuning sweep of the coarse-cycle cut")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
@@ -128,14 +132,25 @@ def main():
         log(f"smoother level {kf}: {ms * 1e3:.2f} us/launch, {sm_bytes / ms / 1e6:.1f} GB/s")
         return
 
+    if args.sweep:
+        for prec in ("fp64", "mixed"):
+            for sm in ("exact", "tiled"):
+                for cc in (-1, 300, 1100, 4100):
+                    tot, _, st = time_solves(G, args.steps, args.warmup, prec, coarse_cells=cc, smoother=sm)
+                    log(f"sweep {prec} {sm} coarse_cells={cc}: {tot / args.steps * 1e3:.3f} ms/solve "
+                        f"({st[-1]['vcycles']} cycles, rel {st[-1]['finalRel']:.2e})")
+        return
+
     log("FP64 solves")
-    tot64, lat64, st64 = time_solves(G, args.steps, args.warmup, "fp64")
+    tot64, lat64, st64 = time_solves(G, args.steps, args.warmup, "fp64", smoother=args.smoother, coarse_cells=args.coarse_cells)
     s64 = st64[-1]
     log(f"fp64: {s64['vcycles']} V-cycles, rel {s64['finalRel']:.3e}, {tot64 / args.steps * 1e3:.2f} ms/solve")
+    if args.fp64_only:
+        return
     Q64 = {k: G.get_field(k, _lib.BSAM_Q) for k in range(1, H.nlev + 1)}
 
     log("mixed-precision solves")
-    totmx, latmx, stmx = time_solves(G, args.steps, args.warmup, "mixed")
+    totmx, latmx, stmx = time_solves(G, args.steps, args.warmup, "mixed", smoother=args.smoother, coarse_cells=args.coarse_cells)
     smx = stmx[-1]
     Qmx = {k: G.get_field(k, _lib.BSAM_Q) for k in range(1, H.nlev + 1)}
     err_mx = visible_rel_l2(H, Qmx, Q64)

@@ -55,7 +55,8 @@ class HierDesc(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("npre", c_int), ("npost", c_int), ("omega", c_double), ("rel_tol", c_double),
                 ("max_vcycles", c_int), ("ghost_interp", c_int), ("mass_correction", c_int),
-                ("precision", c_int), ("inner_cycles", c_int), ("use_graph", c_int)]
+                ("precision", c_int), ("inner_cycles", c_int), ("use_graph", c_int),
+                ("coarse_cells", c_int), ("smoother", c_int)]
 
 
 class StageStats(ctypes.Structure):

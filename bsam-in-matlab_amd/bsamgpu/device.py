@@ -23,7 +23,8 @@ def context(device=0):
     return _CTX[device]
 
 
-def solve_opts(opts, precision=None, use_graph=True, inner_cycles=1, max_vcycles=None):
+def solve_opts(opts, precision=None, use_graph=True, inner_cycles=1, max_vcycles=None, coarse_cells=0,
+               smoother=None):
     o = _lib.default_opts()
     o.npre = int(opts.NPre)
     o.npost = int(opts.NPost)
@@ -36,6 +37,9 @@ def solve_opts(opts, precision=None, use_graph=True, inner_cycles=1, max_vcycles
     o.precision = _lib.BSAM_MIXED if prec == "mixed" else _lib.BSAM_FP64
     o.inner_cycles = int(inner_cycles)
     o.use_graph = 1 if use_graph else 0
+    o.coarse_cells = int(coarse_cells)
+    sm = smoother if smoother is not None else getattr(opts, "Smoother", "exact")
+    o.smoother = 1 if sm in ("tiled", 1) else 0
     return o
 
 

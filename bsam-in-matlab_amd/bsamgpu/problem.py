@@ -97,6 +97,10 @@ class Options:
     # V-cycles; 'fp64' is the plain path, 'mixed' runs FP32 V-cycles inside an
     # FP64 defect-correction loop.
     Precision: str = "fp64"
+    # smoother form: 'exact' reproduces ebRelaxLevel's sequence bit for bit;
+    # 'tiled' runs every colour pass of a relax call in one LDS-staged launch
+    # (same converged solution, sibling ghosts frozen within a call)
+    Smoother: str = "exact"
 
     def __post_init__(self):
         bc = np.round(np.ravel(self.BaseCells)).astype(int)

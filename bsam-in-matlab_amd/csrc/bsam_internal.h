@@ -24,7 +24,8 @@ constexpr int kTile = 256;  // cells per work tile (4 wave64)
 struct PatchInfo {
   int mx, my, par0, cf;   // cf: patch has an internal face or CF corner
   int bc[4];              // W E S N: 0 internal, 1 Dirichlet, 2 Neumann
-  int qoff, coff, xoff, yoff, qcoff, goff, ccoff, pad;
+  int qoff, coff, xoff, yoff, qcoff, goff, ccoff;
+  int gbase;              // first ghost record: W(my) E(my) S(mx) N(mx) SW SE NW NE
 };
 
 // ghost record kinds (one record per ghost cell of a level)
@@ -50,13 +51,23 @@ struct RingRec {
   int dst, src;  // QC index <- parent Q index
 };
 
-// per coarse footprint cell: parent indices for restriction / FAS load
+// A 256-cell work tile with its patch record inline, so that a kernel reaches
+// every offset it needs with one (scalar) load instead of a tile -> patch
+// chain of dependent loads.
+struct TileRec {
+  PatchInfo pi;
+  int p, start, pad0, pad1;
+};
+
+// per coarse footprint cell: parent indices for restriction / FAS load, with
+// the fine patch's offsets inline (one dependent load fewer)
 struct FootRec {
   int p, ci, cj;       // fine patch and coarse cell (0-based, interior)
   int pq;              // parent Q index (padded) of the coarse cell
   int pc;              // parent cell index (F/CC layout)
   int dew_w, dew_e;    // parent DeW indices of the cell's west/east faces
   int dns_s, dns_n;    // parent DnS indices of the cell's south/north faces
+  int qoff, coff, xoff, yoff, qcoff, mx, my;  // fine patch offsets and size
 };
 
 // flux-balance (reflux) contribution of one coarse CF face
@@ -68,6 +79,11 @@ struct CorrRec {
   double cf;           // coarse interface-face diffusivity
 };
 
+// smoother tile: interior cells [i0,i1] x [j0,j1] (0-based) of patch p
+struct STile {
+  int p, i0, i1, j0, j1;
+};
+
 // device tables and buffers of one level
 template <typename T>
 struct LevelDev {
@@ -75,8 +91,10 @@ struct LevelDev {
   double h[2];
   int np, ncell, nq, nx, ny, nqc, ng;
   int ntile, nctile, nghost, nring_all, nring_cf, nfoot, ncorr, ntgt;
+  int nstile, stile_lds;  // smoother tiles and their largest LDS region (cells)
+  STile* stiles;
   PatchInfo* P;
-  int2* tiles;       // (patch, first cell) per 256-cell tile
+  TileRec* tiles;    // patch record + first cell per 256-cell tile
   int2* ctiles;      // (patch, first coarse cell) per tile of the footprint
   GhostRec* ghost;
   RingRec* ring_all;

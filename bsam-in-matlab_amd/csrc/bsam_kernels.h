@@ -4,16 +4,32 @@
 
 namespace bsam {
 
-template <typename T> void launch_relax(const LevelDev<T>& L, int colour, T omega, int homog, hipStream_t s);
+// one colour pass; inl != 0 evaluates internal-face ghosts from the records
+template <typename T>
+void launch_relax(const LevelDev<T>& L, const LevelDev<T>* parent, int colour, T omega, int homog, int inl,
+                  hipStream_t s);
+// all passes of one relax call, one patch per workgroup staged in LDS
+constexpr long long kTiledLdsBytes = 64 * 1024;
+template <typename T> bool tiled_fits(const LevelDev<T>& L);
+template <typename T>
+void launch_relax_tiled(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, T omega, int homog,
+                        hipStream_t s);
 template <typename T> void launch_fill(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, hipStream_t s);
 template <typename T> void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
 template <typename T> void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s);
+// ebCoarseLoad alone (QC ring as stored) and the fused ring + load + reflux
 template <typename T> void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
+template <typename T> void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
 template <typename T> void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s);
 template <typename T> void launch_coarsest(const CoarsestDev& C, const LevelDev<T>& L1, int homog, hipStream_t s);
+// V-cycle of levels 1..kc in one workgroup (LV_dev: device array of levels)
+template <typename T>
+void launch_coarse_cycle(const LevelDev<T>* LV_dev, int kc, const CoarsestDev& C, int npre, int npost, T omega,
+                         int homog, int mass_corr, hipStream_t s);
+template <typename T> void launch_qc_refresh_all(const LevelDev<T>* LV_dev, const int2* items, int nitems, hipStream_t s);
 template <typename T, typename TO>
-void launch_residual(const LevelDev<T>& L, const LevelDev<T>* fine, int mass_corr, ResidualOut* parts, TO* rout,
-                     hipStream_t s);
+void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, int ntl, int mass_corr,
+                         ResidualOut* parts, TO* const* rout, int reset_f, int k0, hipStream_t s);
 void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s);
 template <typename T> void launch_mass(const LevelDev<T>& L, ResidualOut* parts, hipStream_t s);
 void launch_add_correction(const LevelDev<double>& L, const LevelDev<float>& E, hipStream_t s);

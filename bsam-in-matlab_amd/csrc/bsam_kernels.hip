@@ -3,11 +3,16 @@
 // One launch sweeps every patch of a level: work is described by per-level
 // tables (256-cell tiles, ghost records, ring gathers, footprint cells,
 // reflux targets) whose indices are absolute positions in the level's packed
-// buffers.  All kernels are templated on the storage type T: double for the
-// plain path (built with -ffp-contract=off and evaluated in the reference's
-// operation order, so FP64 results equal the CPU oracle bit for bit) and float
-// for the inner V-cycles of the mixed-precision path.  Reflux sums, the
-// composite residual and the coarsest solve always accumulate in FP64.
+// buffers.  Each operation is written once as a per-item device function; the
+// multi-workgroup kernels map items to threads, and the single-workgroup
+// coarse-cycle kernel runs the bottom levels' whole V-cycle with workgroup
+// barriers between dependent steps.
+//
+// All kernels are templated on the storage type T: double for the plain path
+// (built with -ffp-contract=off and evaluated in the reference's operation
+// order, so FP64 results equal the CPU oracle bit for bit) and float for the
+// inner V-cycles of the mixed-precision path.  Reflux sums, the composite
+// residual and the coarsest solve accumulate in FP64.
 //
 // Formulas follow:
 //   relax            EllipticBSAM/src/ebKernels.m:37-95
@@ -17,7 +22,7 @@
 //   reflux           EllipticBSAM/src/ebFaceCorrections.m:107-133, ebCoarseLoad.m:37-45
 //   correct          EllipticBSAM/src/ebTransfer.m:57-71, ebKernels.m:105-123
 //   coarsest         EllipticBSAM/src/ebCoarsest.m:79-91
-//   residual         EllipticBSAM/src/ebCompResidual.m:302-332
+//   residual         EllipticBSAM/src/ebCompResidual.m:271-332
 //   composite mean   EllipticBSAM/src/ebCompositeMean.m:4-23
 #include "bsam_kernels.h"
 
@@ -32,59 +37,6 @@ namespace bsam {
 #define C83_ (8.0 / 3.0)
 #define C13_ (1.0 / 3.0)
 #define C23_ (2.0 / 3.0)
-
-// ---------------------------------------------------------------- smoother
-template <typename T>
-__global__ void __launch_bounds__(kTile) k_relax(const PatchInfo* __restrict__ P,
-                                                 const int2* __restrict__ tiles, T* __restrict__ Q,
-                                                 const T* __restrict__ F, const T* __restrict__ ID,
-                                                 const T* __restrict__ DeW, const T* __restrict__ DnS,
-                                                 const T* __restrict__ G, T hx, T hy, int colour,
-                                                 T omega, int homog) {
-  const int2 t = tiles[blockIdx.x];
-  const PatchInfo& pi = P[t.x];
-  const int mx = pi.mx, my = pi.my;
-  const int c = t.y + (int)threadIdx.x;
-  if (c >= mx * my) return;
-  const int j = c / mx;
-  const int i = c - j * mx;
-  // (li + lj + par0) mod 2 == colour-1 with 1-based local indices
-  if (((i + j + pi.par0) & 1) != colour - 1) return;
-  const int ldq = mx + 2;
-  const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
-  T qe = Q[qc + 1], qw = Q[qc - 1], qn = Q[qc + ldq], qs = Q[qc - ldq];
-  // partial physical ghosts: Dirichlet (8/3)g + (1/3)u_2, Neumann h g
-  if (i == 0 && pi.bc[0]) {
-    const T g = homog ? T(0) : G[pi.goff + j];
-    qw = pi.bc[0] == 1 ? T(C83_) * g + T(C13_) * Q[qc + 1] : hx * g;
-  }
-  if (i == mx - 1 && pi.bc[1]) {
-    const T g = homog ? T(0) : G[pi.goff + my + j];
-    qe = pi.bc[1] == 1 ? T(C83_) * g + T(C13_) * Q[qc - 1] : hx * g;
-  }
-  if (j == 0 && pi.bc[2]) {
-    const T g = homog ? T(0) : G[pi.goff + 2 * my + i];
-    qs = pi.bc[2] == 1 ? T(C83_) * g + T(C13_) * Q[qc + ldq] : hy * g;
-  }
-  if (j == my - 1 && pi.bc[3]) {
-    const T g = homog ? T(0) : G[pi.goff + 2 * my + mx + i];
-    qn = pi.bc[3] == 1 ? T(C83_) * g + T(C13_) * Q[qc - ldq] : hy * g;
-  }
-  const int cc = pi.coff + c;
-  const int fx = pi.xoff + i + j * (mx + 1);
-  const int fy = pi.yoff + i + j * mx;
-  const T hx2 = hx * hx, hy2 = hy * hy;
-  T tmp = F[cc] + qe * DeW[fx + 1] / hx2;
-  tmp = tmp + qw * DeW[fx] / hx2;
-  tmp = tmp + qn * DnS[fy + mx] / hy2;
-  tmp = tmp + qs * DnS[fy] / hy2;
-  tmp = tmp * ID[cc];
-  if (omega == T(1)) {
-    Q[qc] = tmp;
-  } else {
-    Q[qc] = omega * tmp + (T(1) - omega) * Q[qc];
-  }
-}
 
 // ---------------------------------------------------------------- ghosts
 template <typename T>
@@ -115,142 +67,368 @@ __device__ __forceinline__ T ghost_value(const GhostRec& r, const T* __restrict_
   }
 }
 
+// item t of ebFillGhosts: t < nghost -> one ghost cell; then (when the level
+// has a parent) one QC ring refresh of a patch with coarse-fine faces.
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_fill(const GhostRec* __restrict__ recs, int nrec,
-                                                const RingRec* __restrict__ ring, int nring,
-                                                T* __restrict__ Q, const T* __restrict__ Pq,
-                                                T* __restrict__ QC, int homog) {
-  const int t = blockIdx.x * kTile + threadIdx.x;
-  if (t < nrec) {
-    const GhostRec r = recs[t];
+__device__ __forceinline__ void fill_item(const LevelDev<T>& L, const T* __restrict__ Pq, int t,
+                                          int homog) {
+  if (t < L.nghost) {
+    const GhostRec r = L.ghost[t];
     T v;
     if (r.kind == G_PCORNER) {
       // physical corner from the two adjacent side ghosts (ebFillGhosts.m:85-89)
-      const T a = ghost_value<T>(recs[r.s[0]], Q, Pq, homog);
-      const T b = ghost_value<T>(recs[r.s[1]], Q, Pq, homog);
-      v = (a + b) - Q[r.s[2]];
+      const T a = ghost_value<T>(L.ghost[r.s[0]], L.Q, Pq, homog);
+      const T b = ghost_value<T>(L.ghost[r.s[1]], L.Q, Pq, homog);
+      v = (a + b) - L.Q[r.s[2]];
     } else {
-      v = ghost_value<T>(r, Q, Pq, homog);
+      v = ghost_value<T>(r, L.Q, Pq, homog);
     }
-    Q[r.dst] = v;
-  } else if (t < nrec + nring) {
-    const RingRec rr = ring[t - nrec];
-    QC[rr.dst] = Pq[rr.src];
+    L.Q[r.dst] = v;
+  } else if (Pq && t < L.nghost + L.nring_cf) {
+    const RingRec rr = L.ring_cf[t - L.nghost];
+    L.QC[rr.dst] = Pq[rr.src];
   }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_ring(const RingRec* __restrict__ ring, int nring,
-                                                const T* __restrict__ Pq, T* __restrict__ QC) {
-  const int t = blockIdx.x * kTile + threadIdx.x;
-  if (t < nring) {
-    const RingRec rr = ring[t];
-    QC[rr.dst] = Pq[rr.src];
+__device__ __forceinline__ int fill_items(const LevelDev<T>& L, const T* Pq) {
+  return L.nghost + (Pq ? L.nring_cf : 0);
+}
+
+// ---------------------------------------------------------------- smoother
+// One cell of a colour pass (ebKernels.m:37-95).  With inl != 0 the
+// internal-face ghost values are evaluated from the ghost records instead of
+// being read from the materialised ghost frame: inside one relax call they
+// equal what ebFillGhosts would have written after the previous pass, since a
+// colour-c cell's ghost depends only on other-colour cells, on the cell
+// itself (read before it is written) and on the parent level.
+template <typename T>
+__device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __restrict__ Pq, int tile,
+                                           int lane, int colour, T omega, int homog, int inl) {
+  const TileRec& tr = L.tiles[tile];
+  const PatchInfo& pi = tr.pi;
+  const int mx = pi.mx, my = pi.my;
+  const int c = tr.start + lane;
+  if (c >= mx * my) return;
+  const int j = c / mx;
+  const int i = c - j * mx;
+  // (li + lj + par0) mod 2 == colour-1 with 1-based local indices
+  if (((i + j + pi.par0) & 1) != colour - 1) return;
+  T* __restrict__ Q = L.Q;
+  const int ldq = mx + 2;
+  const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
+  T qe = Q[qc + 1], qw = Q[qc - 1], qn = Q[qc + ldq], qs = Q[qc - ldq];
+  const T hx = (T)L.h[0], hy = (T)L.h[1];
+  // partial physical ghosts: Dirichlet (8/3)g + (1/3)u_2, Neumann h g
+  if (i == 0) {
+    if (pi.bc[0]) {
+      const T g = homog ? T(0) : L.G[pi.goff + j];
+      qw = pi.bc[0] == 1 ? T(C83_) * g + T(C13_) * Q[qc + 1] : hx * g;
+    } else if (inl) {
+      qw = ghost_value<T>(L.ghost[pi.gbase + j], Q, Pq, homog);
+    }
+  }
+  if (i == mx - 1) {
+    if (pi.bc[1]) {
+      const T g = homog ? T(0) : L.G[pi.goff + my + j];
+      qe = pi.bc[1] == 1 ? T(C83_) * g + T(C13_) * Q[qc - 1] : hx * g;
+    } else if (inl) {
+      qe = ghost_value<T>(L.ghost[pi.gbase + my + j], Q, Pq, homog);
+    }
+  }
+  if (j == 0) {
+    if (pi.bc[2]) {
+      const T g = homog ? T(0) : L.G[pi.goff + 2 * my + i];
+      qs = pi.bc[2] == 1 ? T(C83_) * g + T(C13_) * Q[qc + ldq] : hy * g;
+    } else if (inl) {
+      qs = ghost_value<T>(L.ghost[pi.gbase + 2 * my + i], Q, Pq, homog);
+    }
+  }
+  if (j == my - 1) {
+    if (pi.bc[3]) {
+      const T g = homog ? T(0) : L.G[pi.goff + 2 * my + mx + i];
+      qn = pi.bc[3] == 1 ? T(C83_) * g + T(C13_) * Q[qc - ldq] : hy * g;
+    } else if (inl) {
+      qn = ghost_value<T>(L.ghost[pi.gbase + 2 * my + mx + i], Q, Pq, homog);
+    }
+  }
+  const int cc = pi.coff + c;
+  const int fx = pi.xoff + i + j * (mx + 1);
+  const int fy = pi.yoff + i + j * mx;
+  const T hx2 = hx * hx, hy2 = hy * hy;
+  T tmp = L.F[cc] + qe * L.DeW[fx + 1] / hx2;
+  tmp = tmp + qw * L.DeW[fx] / hx2;
+  tmp = tmp + qn * L.DnS[fy + mx] / hy2;
+  tmp = tmp + qs * L.DnS[fy] / hy2;
+  tmp = tmp * L.ID[cc];
+  if (omega == T(1)) {
+    Q[qc] = tmp;
+  } else {
+    Q[qc] = omega * tmp + (T(1) - omega) * Q[qc];
+  }
+}
+
+// ---------------------------------------------------------------- tiled smoother
+// All 2*nsweeps colour passes of one relax call on one tile, with the tile
+// and a 2-cell margin (clipped to the padded patch) staged in LDS: one HBM
+// round trip per relax call instead of one per colour.  Inside the tile the
+// passes are exact red-black Gauss-Seidel: physical ghosts are the partial
+// ghosts of ebKernels.m:51-70 and coarse-fine ghosts are re-evaluated from the
+// staged cells and the parent after every pass.  Same-level (sibling) ghosts
+// and cells of neighbouring tiles keep the values they had when the call
+// started (a block Gauss-Seidel coupling between tiles); the converged
+// composite solution is unchanged because at the fixed point those values are
+// stationary.
+template <typename T>
+__device__ __forceinline__ T lds_or_global(const T* S, const T* __restrict__ Q, int s, int qoff, int ldq, int a0,
+                                           int a1, int b0, int b1, int lw) {
+  const int d = s - qoff;
+  const int pj = d / ldq;
+  const int pi_ = d - pj * ldq;
+  if (pi_ >= a0 && pi_ <= a1 && pj >= b0 && pj <= b1) return S[(pi_ - a0) + (pj - b0) * lw];
+  return Q[s];
+}
+
+template <typename T>
+__device__ __forceinline__ T ghost_value_tile(const GhostRec& r, const T* S, const T* __restrict__ Q,
+                                              const T* __restrict__ Pq, int qoff, int ldq, int a0, int a1, int b0,
+                                              int b1, int lw, T frozen) {
+  auto q = [&](int s) { return lds_or_global<T>(S, Q, s, qoff, ldq, a0, a1, b0, b1, lw); };
+  switch (r.kind) {
+    case G_QIF:
+      return (T(A1_) * q(r.s[0]) + T(A2_) * q(r.s[1])) + T(A3_) * Pq[r.s[2]];
+    case G_QIF_END:
+      return (T(A1_) * q(r.s[0]) + T(A2_) * q(r.s[1])) +
+             T(A3_) * ((T(B1_) * Pq[r.s[2]] + T(B2_) * Pq[r.s[3]]) + T(B3_) * Pq[r.s[4]]);
+    case G_LIF:
+      return (T(C23_) * Pq[r.s[2]] + T(C23_) * q(r.s[0])) - T(C13_) * q(r.s[1]);
+    default:  // same-level copy: frozen for the call
+      return frozen;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_relax_tiled(LevelDev<T> L, const T* __restrict__ Pq, int nsweeps,
+                                                     T omega, int homog) {
+  extern __shared__ unsigned char smem_raw[];
+  T* S = reinterpret_cast<T*>(smem_raw);
+  const STile tl = L.stiles[blockIdx.x];
+  const PatchInfo& pi = L.P[tl.p];
+  const int mx = pi.mx, my = pi.my, ldq = mx + 2;
+  const int a0 = max(tl.i0 - 1, 0), a1 = min(tl.i1 + 3, mx + 1);  // padded patch coordinates
+  const int b0 = max(tl.j0 - 1, 0), b1 = min(tl.j1 + 3, my + 1);
+  const int lw = a1 - a0 + 1, lh = b1 - b0 + 1;
+  T* __restrict__ Q = L.Q;
+  for (int x = threadIdx.x; x < lw * lh; x += blockDim.x) {
+    const int lj = x / lw;
+    S[x] = Q[pi.qoff + (a0 + x - lj * lw) + (b0 + lj) * ldq];
+  }
+  __syncthreads();
+  const int tw = tl.i1 - tl.i0 + 1, th = tl.j1 - tl.j0 + 1;
+  const T hx = (T)L.h[0], hy = (T)L.h[1];
+  const T hx2 = hx * hx, hy2 = hy * hy;
+  for (int it = 0; it < nsweeps; ++it)
+    for (int colour = 1; colour <= 2; ++colour) {
+      const bool inl = !(it == 0 && colour == 1);
+      for (int x = threadIdx.x; x < tw * th; x += blockDim.x) {
+        const int jj = x / tw;
+        const int i = tl.i0 + x - jj * tw, j = tl.j0 + jj;
+        if (((i + j + pi.par0) & 1) != colour - 1) continue;
+        const int s = (i + 1 - a0) + (j + 1 - b0) * lw;
+        T qe = S[s + 1], qw = S[s - 1], qn = S[s + lw], qs = S[s - lw];
+        if (i == 0) {
+          if (pi.bc[0]) {
+            const T g = homog ? T(0) : L.G[pi.goff + j];
+            qw = pi.bc[0] == 1 ? T(C83_) * g + T(C13_) * S[s + 1] : hx * g;
+          } else if (inl) {
+            qw = ghost_value_tile<T>(L.ghost[pi.gbase + j], S, Q, Pq, pi.qoff, ldq, a0, a1, b0, b1, lw, qw);
+          }
+        }
+        if (i == mx - 1) {
+          if (pi.bc[1]) {
+            const T g = homog ? T(0) : L.G[pi.goff + my + j];
+            qe = pi.bc[1] == 1 ? T(C83_) * g + T(C13_) * S[s - 1] : hx * g;
+          } else if (inl) {
+            qe = ghost_value_tile<T>(L.ghost[pi.gbase + my + j], S, Q, Pq, pi.qoff, ldq, a0, a1, b0, b1, lw, qe);
+          }
+        }
+        if (j == 0) {
+          if (pi.bc[2]) {
+            const T g = homog ? T(0) : L.G[pi.goff + 2 * my + i];
+            qs = pi.bc[2] == 1 ? T(C83_) * g + T(C13_) * S[s + lw] : hy * g;
+          } else if (inl) {
+            qs = ghost_value_tile<T>(L.ghost[pi.gbase + 2 * my + i], S, Q, Pq, pi.qoff, ldq, a0, a1, b0, b1, lw, qs);
+          }
+        }
+        if (j == my - 1) {
+          if (pi.bc[3]) {
+            const T g = homog ? T(0) : L.G[pi.goff + 2 * my + mx + i];
+            qn = pi.bc[3] == 1 ? T(C83_) * g + T(C13_) * S[s - lw] : hy * g;
+          } else if (inl) {
+            qn = ghost_value_tile<T>(L.ghost[pi.gbase + 2 * my + mx + i], S, Q, Pq, pi.qoff, ldq, a0, a1, b0, b1,
+                                     lw, qn);
+          }
+        }
+        const int cc = pi.coff + i + j * mx;
+        const int fx = pi.xoff + i + j * (mx + 1);
+        const int fy = pi.yoff + i + j * mx;
+        T tmp = L.F[cc] + qe * L.DeW[fx + 1] / hx2;
+        tmp = tmp + qw * L.DeW[fx] / hx2;
+        tmp = tmp + qn * L.DnS[fy + mx] / hy2;
+        tmp = tmp + qs * L.DnS[fy] / hy2;
+        tmp = tmp * L.ID[cc];
+        S[s] = omega == T(1) ? tmp : omega * tmp + (T(1) - omega) * S[s];
+      }
+      __syncthreads();
+    }
+  for (int x = threadIdx.x; x < tw * th; x += blockDim.x) {
+    const int jj = x / tw;
+    const int i = tl.i0 + x - jj * tw, j = tl.j0 + jj;
+    Q[pi.qoff + (i + 1) + (j + 1) * ldq] = S[(i + 1 - a0) + (j + 1 - b0) * lw];
   }
 }
 
 // ---------------------------------------------------------------- transfers
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_restrict(const PatchInfo* __restrict__ P,
-                                                    const FootRec* __restrict__ foot, int nfoot,
-                                                    const T* __restrict__ Q, T* __restrict__ QC,
-                                                    T* __restrict__ Pq, int qc_only) {
-  const int t = blockIdx.x * kTile + threadIdx.x;
-  if (t >= nfoot) return;
-  const FootRec fr = foot[t];
-  const PatchInfo& pi = P[fr.p];
+__device__ __forceinline__ void ring_item(const LevelDev<T>& L, const T* __restrict__ Pq, int t) {
+  const RingRec rr = L.ring_all[t];
+  L.QC[rr.dst] = Pq[rr.src];
+}
+
+template <typename T>
+__device__ __forceinline__ void restrict_item(const LevelDev<T>& L, T* __restrict__ Pq, int t,
+                                              int qc_only) {
+  const FootRec fr = L.foot[t];
+  const FootRec& pi = fr;  // patch offsets carried inline
   const int ldq = pi.mx + 2;
+  const T* __restrict__ Q = L.Q;
   const int q00 = pi.qoff + (2 * fr.ci + 1) + (2 * fr.cj + 1) * ldq;
   const T R = T(0.25) * (((Q[q00] + Q[q00 + 1]) + Q[q00 + ldq]) + Q[q00 + ldq + 1]);
-  QC[pi.qcoff + (fr.ci + 1) + (fr.cj + 1) * (pi.mx / 2 + 2)] = R;
+  L.QC[pi.qcoff + (fr.ci + 1) + (fr.cj + 1) * (pi.mx / 2 + 2)] = R;
   if (!qc_only) Pq[fr.pq] = R;
 }
 
 // L(u) at one cell of a padded array (ebKernels.m:13-25)
 template <typename T>
-__device__ __forceinline__ T op_at(const T* __restrict__ Q, int qc, int ldq, T dw, T de, T ds, T dn,
-                                   T cc, T hx2, T hy2) {
-  const T u = Q[qc];
-  const T fw = dw * (u - Q[qc - 1]);
-  const T fe = de * (Q[qc + 1] - u);
-  const T fs = ds * (u - Q[qc - ldq]);
-  const T fn = dn * (Q[qc + ldq] - u);
+__device__ __forceinline__ T op5(T u, T w, T e, T s, T n, T dw, T de, T ds, T dn, T cc, T hx2, T hy2) {
+  const T fw = dw * (u - w);
+  const T fe = de * (e - u);
+  const T fs = ds * (u - s);
+  const T fn = dn * (n - u);
   return (-(fe - fw) / hx2 - (fn - fs) / hy2) + cc * u;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_coarse_load(
-    const PatchInfo* __restrict__ P, const FootRec* __restrict__ foot, int nfoot,
-    const T* __restrict__ Q, const T* __restrict__ F, const T* __restrict__ DeW,
-    const T* __restrict__ DnS, const T* __restrict__ CC, const T* __restrict__ QC, T hx, T hy,
-    const T* __restrict__ pDeW, const T* __restrict__ pDnS, const T* __restrict__ pCC,
-    T* __restrict__ pF, T hcx, T hcy) {
-  const int t = blockIdx.x * kTile + threadIdx.x;
-  if (t >= nfoot) return;
-  const FootRec fr = foot[t];
-  const PatchInfo& pi = P[fr.p];
+__device__ __forceinline__ T op_at(const T* __restrict__ Q, int qc, int ldq, T dw, T de, T ds, T dn,
+                                   T cc, T hx2, T hy2) {
+  return op5<T>(Q[qc], Q[qc - 1], Q[qc + 1], Q[qc - ldq], Q[qc + ldq], dw, de, ds, dn, cc, hx2, hy2);
+}
+
+// restricted fine residual R(F - L u) over one coarse footprint cell
+template <typename T>
+__device__ __forceinline__ T restricted_residual(const LevelDev<T>& L, const FootRec& pi, int ci, int cj) {
   const int mx = pi.mx;
   const int ldq = mx + 2;
+  const T hx = (T)L.h[0], hy = (T)L.h[1];
   const T hx2 = hx * hx, hy2 = hy * hy;
   T rf[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
-    const int i = 2 * fr.ci + (s & 1);
-    const int j = 2 * fr.cj + (s >> 1);
+    const int i = 2 * ci + (s & 1);
+    const int j = 2 * cj + (s >> 1);
     const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
     const int cc = pi.coff + i + j * mx;
     const int fx = pi.xoff + i + j * (mx + 1);
     const int fy = pi.yoff + i + j * mx;
-    rf[s] = F[cc] - op_at<T>(Q, qc, ldq, DeW[fx], DeW[fx + 1], DnS[fy], DnS[fy + mx], CC[cc], hx2, hy2);
+    rf[s] = L.F[cc] - op_at<T>(L.Q, qc, ldq, L.DeW[fx], L.DeW[fx + 1], L.DnS[fy], L.DnS[fy + mx], L.CC[cc],
+                              hx2, hy2);
   }
-  const T R = T(0.25) * (((rf[0] + rf[1]) + rf[2]) + rf[3]);
-  const int ldc = mx / 2 + 2;
-  const int qcc = pi.qcoff + (fr.ci + 1) + (fr.cj + 1) * ldc;
-  const T Lc = op_at<T>(QC, qcc, ldc, pDeW[fr.dew_w], pDeW[fr.dew_e], pDnS[fr.dns_s], pDnS[fr.dns_n],
-                        pCC[fr.pc], hcx * hcx, hcy * hcy);
-  pF[fr.pc] = R + Lc;
+  return T(0.25) * (((rf[0] + rf[1]) + rf[2]) + rf[3]);
 }
 
-// one flux-balance contribution (ebFaceCorrections.m:111-133), in FP64
+// one flux-balance contribution (ebFaceCorrections.m:111-133), in FP64;
+// `ring` is the QC ring value just outside the coarse face
 template <typename T>
-__device__ __forceinline__ double corr_value(const CorrRec& e, const T* __restrict__ Q,
+__device__ __forceinline__ double corr_value(const CorrRec& e, const T* __restrict__ Q, double ring,
                                              const T* __restrict__ QC, const T* __restrict__ DeW,
                                              const T* __restrict__ DnS, double hcx2, double hcy2) {
   const T* D = e.face < 2 ? DeW : DnS;
   const double h2 = e.face < 2 ? hcx2 : hcy2;
   const double t1 = (double)D[e.d1] * ((double)Q[e.in1] - (double)Q[e.g1]);
   const double t2 = (double)D[e.d2] * ((double)Q[e.in2] - (double)Q[e.g2]);
-  const double t3 = e.cf * ((double)QC[e.ring] - (double)QC[e.qcin]);
+  const double t3 = e.cf * (ring - (double)QC[e.qcin]);
   return ((t1 + t2) + t3) / h2;
 }
 
+// ebCoarseLoad for one footprint cell, QC ring read from QC (standalone form)
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_reflux(const int* __restrict__ tgt_cell,
-                                                  const int* __restrict__ tgt_ptr,
-                                                  const int* __restrict__ tgt_ent, int ntgt,
-                                                  const CorrRec* __restrict__ corr,
-                                                  const T* __restrict__ Q, const T* __restrict__ QC,
-                                                  const T* __restrict__ DeW, const T* __restrict__ DnS,
-                                                  double hcx2, double hcy2, T* __restrict__ pF) {
-  const int t = blockIdx.x * kTile + threadIdx.x;
-  if (t >= ntgt) return;
-  const int cell = tgt_cell[t];
-  double acc = (double)pF[cell];
-  for (int e = tgt_ptr[t]; e < tgt_ptr[t + 1]; ++e)
-    acc = acc + corr_value<T>(corr[tgt_ent[e]], Q, QC, DeW, DnS, hcx2, hcy2);
-  pF[cell] = (T)acc;
+__device__ __forceinline__ void coarse_load_item(const LevelDev<T>& L, const LevelDev<T>& P, int t) {
+  const FootRec fr = L.foot[t];
+  const FootRec& pi = fr;  // patch offsets carried inline
+  const T R = restricted_residual<T>(L, pi, fr.ci, fr.cj);
+  const int ldc = pi.mx / 2 + 2;
+  const T hcx = (T)P.h[0], hcy = (T)P.h[1];
+  const int qcc = pi.qcoff + (fr.ci + 1) + (fr.cj + 1) * ldc;
+  const T Lc = op_at<T>(L.QC, qcc, ldc, P.DeW[fr.dew_w], P.DeW[fr.dew_e], P.DnS[fr.dns_s], P.DnS[fr.dns_n],
+                        P.CC[fr.pc], hcx * hcx, hcy * hcy);
+  P.F[fr.pc] = R + Lc;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_correct(const PatchInfo* __restrict__ P,
-                                                   const int2* __restrict__ tiles,
-                                                   T* __restrict__ Q, const T* __restrict__ QC,
-                                                   const int* __restrict__ wmap,
-                                                   const T* __restrict__ Pq, int init) {
-  const int2 t = tiles[blockIdx.x];
-  const PatchInfo& pi = P[t.x];
+__device__ __forceinline__ void reflux_item(const LevelDev<T>& L, const LevelDev<T>& P, int t, int ring_from_parent) {
+  const int cell = P.tgt_cell[t];
+  double acc = (double)P.F[cell];
+  const double hcx2 = P.h[0] * P.h[0], hcy2 = P.h[1] * P.h[1];
+  for (int e = P.tgt_ptr[t]; e < P.tgt_ptr[t + 1]; ++e) {
+    const CorrRec cr = L.corr[P.tgt_ent[e]];
+    const double ring = ring_from_parent ? (double)P.Q[L.wmap[cr.ring]] : (double)L.QC[cr.ring];
+    acc = acc + corr_value<T>(cr, L.Q, ring, L.QC, L.DeW, L.DnS, hcx2, hcy2);
+  }
+  P.F[cell] = (T)acc;
+}
+
+// ring gather + FAS load + reflux of level k into level k-1 in one sweep
+// (ebTransfer 'ring' followed by ebCoarseLoad): ring values are read from the
+// parent through the window map, which is exactly what the gather writes.
+template <typename T>
+__device__ __forceinline__ int load_items(const LevelDev<T>& L, const LevelDev<T>& P, int mass_corr) {
+  return L.nfoot + L.nring_all + (mass_corr ? P.ntgt : 0);
+}
+
+template <typename T>
+__device__ __forceinline__ void load_item(const LevelDev<T>& L, const LevelDev<T>& P, int t, int mass_corr) {
+  if (t < L.nfoot) {
+    const FootRec fr = L.foot[t];
+    const FootRec& pi = fr;  // patch offsets carried inline
+    const T R = restricted_residual<T>(L, pi, fr.ci, fr.cj);
+    const int ldc = pi.mx / 2 + 2;
+    const int w = pi.qcoff + (fr.ci + 1) + (fr.cj + 1) * ldc;
+    const int cmx = pi.mx / 2, cmy = pi.my / 2;
+    const T* __restrict__ QC = L.QC;
+    const T* __restrict__ Pq = P.Q;
+    const T u = QC[w];
+    const T vw = fr.ci == 0 ? Pq[L.wmap[w - 1]] : QC[w - 1];
+    const T ve = fr.ci == cmx - 1 ? Pq[L.wmap[w + 1]] : QC[w + 1];
+    const T vs = fr.cj == 0 ? Pq[L.wmap[w - ldc]] : QC[w - ldc];
+    const T vn = fr.cj == cmy - 1 ? Pq[L.wmap[w + ldc]] : QC[w + ldc];
+    const T hcx = (T)P.h[0], hcy = (T)P.h[1];
+    const T Lc = op5<T>(u, vw, ve, vs, vn, P.DeW[fr.dew_w], P.DeW[fr.dew_e], P.DnS[fr.dns_s], P.DnS[fr.dns_n],
+                        P.CC[fr.pc], hcx * hcx, hcy * hcy);
+    P.F[fr.pc] = R + Lc;
+  } else if (t < L.nfoot + L.nring_all) {
+    ring_item<T>(L, P.Q, t - L.nfoot);
+  } else {
+    reflux_item<T>(L, P, t - L.nfoot - L.nring_all, 1);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void correct_item(const LevelDev<T>& L, const T* __restrict__ Pq, int tile,
+                                             int lane, int init) {
+  const TileRec& tr = L.tiles[tile];
+  const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
-  const int c = t.y + (int)threadIdx.x;
+  const int c = tr.start + lane;
   if (c >= mx * my) return;
   const int j = c / mx;
   const int i = c - j * mx;
@@ -258,6 +436,8 @@ __global__ void __launch_bounds__(kTile) k_correct(const PatchInfo* __restrict__
   const int w0 = pi.qcoff + ((i >> 1) + 1) + ((j >> 1) + 1) * ldc;
   const int dx = (i & 1) ? 1 : -1;
   const int dy = (j & 1) ? ldc : -ldc;
+  const int* __restrict__ wmap = L.wmap;
+  const T* __restrict__ QC = L.QC;
   T e_c, e_x, e_y, e_d;
   if (init) {
     e_c = Pq[wmap[w0]];
@@ -272,25 +452,146 @@ __global__ void __launch_bounds__(kTile) k_correct(const PatchInfo* __restrict__
   }
   const T p = (((T(9) * e_c + T(3) * e_x) + T(3) * e_y) + e_d) / T(16);
   const int qc = pi.qoff + (i + 1) + (j + 1) * (mx + 2);
-  Q[qc] = init ? p : Q[qc] + p;
+  L.Q[qc] = init ? p : L.Q[qc] + p;
+}
+
+// ---------------------------------------------------------------- kernels
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_relax(LevelDev<T> L, const T* __restrict__ Pq, int colour,
+                                                 T omega, int homog, int inl) {
+  relax_item<T>(L, Pq, blockIdx.x, threadIdx.x, colour, omega, homog, inl);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_fill(LevelDev<T> L, const T* __restrict__ Pq, int homog) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t < fill_items<T>(L, Pq)) fill_item<T>(L, Pq, t, homog);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_ring(LevelDev<T> L, const T* __restrict__ Pq) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t < L.nring_all) ring_item<T>(L, Pq, t);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_restrict(LevelDev<T> L, T* __restrict__ Pq, int qc_only) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t < L.nfoot) restrict_item<T>(L, Pq, t, qc_only);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_coarse_load(LevelDev<T> L, LevelDev<T> P) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t < L.nfoot) coarse_load_item<T>(L, P, t);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_reflux(LevelDev<T> L, LevelDev<T> P) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t < P.ntgt) reflux_item<T>(L, P, t, 0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_load(LevelDev<T> L, LevelDev<T> P, int mass_corr) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t < load_items<T>(L, P, mass_corr)) load_item<T>(L, P, t, mass_corr);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_correct(LevelDev<T> L, const T* __restrict__ Pq, int init) {
+  correct_item<T>(L, Pq, blockIdx.x, threadIdx.x, init);
 }
 
 // ---------------------------------------------------------------- coarsest
+// u = Ainv (F + bcRhs), rows accumulated in ascending column order; the
+// caller has staged rhs in LDS
 template <typename T>
-__global__ void __launch_bounds__(64) k_coarsest(const double* __restrict__ Ainv,
-                                                 const double* __restrict__ bcRhs, int N, int M,
-                                                 int n1, const T* __restrict__ F, T* __restrict__ Q,
-                                                 int homog) {
-  extern __shared__ double rhs[];
-  for (int j = threadIdx.x; j < M; j += blockDim.x)
-    rhs[j] = j < N ? (homog ? (double)F[j] : (double)F[j] + bcRhs[j]) : 0.0;
-  __syncthreads();
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
+__device__ __forceinline__ void coarsest_row(const CoarsestDev& C, const double* rhs, int i, int n1,
+                                             T* __restrict__ Q) {
   double acc = 0.0;
-  for (int j = 0; j < M; ++j) acc = acc + Ainv[i + (size_t)j * M] * rhs[j];
+  for (int j = 0; j < C.M; ++j) acc = acc + C.Ainv[i + (size_t)j * C.M] * rhs[j];
   const int ci = i % n1, cj = i / n1;
   Q[(ci + 1) + (cj + 1) * (n1 + 2)] = (T)acc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_coarsest(CoarsestDev C, LevelDev<T> L1, int homog) {
+  extern __shared__ double rhs[];
+  for (int j = threadIdx.x; j < C.M; j += blockDim.x)
+    rhs[j] = j < C.N ? (homog ? (double)L1.F[j] : (double)L1.F[j] + C.bcRhs[j]) : 0.0;
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < C.N) coarsest_row<T>(C, rhs, i, L1.n[0], L1.Q);
+}
+
+// ---------------------------------------------------------------- coarse cycle
+// The V-cycle of levels 1..kc inside one workgroup (ebVcycle.m:15-35 for
+// k <= kc): every step is a workgroup-strided loop over the step's items,
+// separated by barriers.  The descent's smoother reads the materialised
+// ghosts in its first pass; after the correction the fill is folded into the
+// first post-smoothing pass (inline ghosts), as in the multi-kernel form.
+template <typename T>
+__device__ void blk_relax(const LevelDev<T>& L, const T* Pq, int nsweeps, T omega, int homog, int first_inl) {
+  const int ncell = L.ntile * kTile;
+  for (int it = 0; it < nsweeps; ++it)
+    for (int colour = 1; colour <= 2; ++colour) {
+      const int inl = (it == 0 && colour == 1) ? first_inl : 1;
+      for (int x = threadIdx.x; x < ncell; x += blockDim.x)
+        relax_item<T>(L, Pq, x / kTile, x % kTile, colour, omega, homog, inl);
+      __syncthreads();
+    }
+  if (nsweeps > 0) {
+    const int n = fill_items<T>(L, Pq);
+    for (int x = threadIdx.x; x < n; x += blockDim.x) fill_item<T>(L, Pq, x, homog);
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__device__ void blk_fill(const LevelDev<T>& L, const T* Pq, int homog) {
+  const int n = fill_items<T>(L, Pq);
+  for (int x = threadIdx.x; x < n; x += blockDim.x) fill_item<T>(L, Pq, x, homog);
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __restrict__ LV, int kc,
+                                                       CoarsestDev C, int npre, int npost, T omega,
+                                                       int homog, int mass_corr) {
+  extern __shared__ double rhs[];
+  for (int k = kc; k >= 2; --k) {
+    const LevelDev<T> L = LV[k - 1];
+    const LevelDev<T> P = LV[k - 2];
+    blk_relax<T>(L, P.Q, npre, omega, homog, 0);
+    for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
+    __syncthreads();
+    blk_fill<T>(P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
+    const int n = load_items<T>(L, P, mass_corr);
+    for (int x = threadIdx.x; x < n; x += blockDim.x) load_item<T>(L, P, x, mass_corr);
+    __syncthreads();
+  }
+  {
+    const LevelDev<T> L1 = LV[0];
+    for (int j = threadIdx.x; j < C.M; j += blockDim.x)
+      rhs[j] = j < C.N ? (homog ? (double)L1.F[j] : (double)L1.F[j] + C.bcRhs[j]) : 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < C.N; i += blockDim.x) coarsest_row<T>(C, rhs, i, L1.n[0], L1.Q);
+    __syncthreads();
+    blk_fill<T>(L1, (const T*)nullptr, homog);
+  }
+  for (int k = 2; k <= kc; ++k) {
+    const LevelDev<T> L = LV[k - 1];
+    const T* Pq = LV[k - 2].Q;
+    const int ncell = L.ntile * kTile;
+    for (int x = threadIdx.x; x < ncell; x += blockDim.x) correct_item<T>(L, Pq, x / kTile, x % kTile, 0);
+    __syncthreads();
+    if (npost > 0) {
+      blk_relax<T>(L, Pq, npost, omega, homog, 1);
+    } else {
+      blk_fill<T>(L, Pq, homog);
+    }
+  }
 }
 
 // ---------------------------------------------------------------- reductions
@@ -334,49 +635,76 @@ __device__ __forceinline__ void block_reduce4(double a, double b, double c, long
   }
 }
 
-// composite residual of one level: r = FT - L(Q) + reflux chunks; norms over
-// visible cells; optional FP32/FP64 copy of r (defect-correction right side)
+// QC refresh of every level k >= kmin in one launch (ebCompResidual.m:271-287):
+// ring gather and QC interior = R(u) (no parent scatter).  Items of the
+// combined table: (level, item) with item < nring_all + nfoot.
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_qc_refresh_all(const LevelDev<T>* __restrict__ LV,
+                                                          const int2* __restrict__ items, int nitems) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t >= nitems) return;
+  const int2 it = items[t];
+  const LevelDev<T>& L = LV[it.x];
+  if (it.y < L.nring_all) {
+    ring_item<T>(L, LV[it.x - 1].Q, it.y);
+  } else {
+    restrict_item<T>(L, (T*)nullptr, it.y - L.nring_all, 1);
+  }
+}
+
+// composite residual of every level in [k0, nlev] (ebCompResidual.m:302-332):
+// r = FTrue - L(Q) + reflux chunks, norms over visible cells; optionally
+// writes r (the defect-correction right side) and resets F = FTrue on the
+// tile's cells.  The tile table is (level, tile) pairs.
 template <typename T, typename TO>
-__global__ void __launch_bounds__(kTile) k_residual(
-    const PatchInfo* __restrict__ P, const int2* __restrict__ tiles, const T* __restrict__ Q,
-    const T* __restrict__ FT, const T* __restrict__ DeW, const T* __restrict__ DnS,
-    const T* __restrict__ CC, T hx, T hy, const uint8_t* __restrict__ vis,
-    const int* __restrict__ tgt_of_cell, const int* __restrict__ tgt_ptr,
-    const int* __restrict__ tgt_ent, const CorrRec* __restrict__ fcorr, const T* __restrict__ fQ,
-    const T* __restrict__ fQC, const T* __restrict__ fDeW, const T* __restrict__ fDnS,
-    double hx2c, double hy2c, double area, ResidualOut* __restrict__ part, TO* __restrict__ rout) {
-  const int2 t = tiles[blockIdx.x];
-  const PatchInfo& pi = P[t.x];
+__global__ void __launch_bounds__(kTile) k_residual_all(const LevelDev<T>* __restrict__ LV, int nlev,
+                                                        const int2* __restrict__ tl, int mass_corr,
+                                                        ResidualOut* __restrict__ part,
+                                                        TO* const* __restrict__ rout, int reset_f,
+                                                        int k0) {
+  const int2 lt = tl[blockIdx.x];
+  const LevelDev<T>& L = LV[lt.x];
+  const TileRec& tr = L.tiles[lt.y];
+  const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
-  const int c = t.y + (int)threadIdx.x;
+  const int c = tr.start + (int)threadIdx.x;
   double a = 0.0, b = 0.0, f = 0.0;
   long long n = 0;
   if (c < mx * my) {
-    const int j = c / mx;
-    const int i = c - j * mx;
-    const int ldq = mx + 2;
-    const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
     const int cc = pi.coff + c;
-    const int fx = pi.xoff + i + j * (mx + 1);
-    const int fy = pi.yoff + i + j * mx;
-    T r = FT[cc] - op_at<T>(Q, qc, ldq, DeW[fx], DeW[fx + 1], DnS[fy], DnS[fy + mx], CC[cc], hx * hx, hy * hy);
-    double rd = (double)r;
-    if (tgt_of_cell) {
-      const int row = tgt_of_cell[cc];
-      if (row >= 0) {
-        for (int e = tgt_ptr[row]; e < tgt_ptr[row + 1]; ++e)
-          rd = rd + corr_value<T>(fcorr[tgt_ent[e]], fQ, fQC, fDeW, fDnS, hx2c, hy2c);
+    if (reset_f) L.F[cc] = L.FT[cc];
+    if (lt.x + 1 >= k0) {
+      const int j = c / mx;
+      const int i = c - j * mx;
+      const int ldq = mx + 2;
+      const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
+      const int fx = pi.xoff + i + j * (mx + 1);
+      const int fy = pi.yoff + i + j * mx;
+      const T hx = (T)L.h[0], hy = (T)L.h[1];
+      T r = L.FT[cc] - op_at<T>(L.Q, qc, ldq, L.DeW[fx], L.DeW[fx + 1], L.DnS[fy], L.DnS[fy + mx], L.CC[cc],
+                                hx * hx, hy * hy);
+      double rd = (double)r;
+      if (mass_corr && lt.x + 1 < nlev) {
+        const int row = L.tgt_of_cell[cc];
+        if (row >= 0) {
+          const LevelDev<T>& Fn = LV[lt.x + 1];
+          const double hcx2 = L.h[0] * L.h[0], hcy2 = L.h[1] * L.h[1];
+          for (int e = L.tgt_ptr[row]; e < L.tgt_ptr[row + 1]; ++e) {
+            const CorrRec cr = Fn.corr[L.tgt_ent[e]];
+            rd = rd + corr_value<T>(cr, Fn.Q, (double)Fn.QC[cr.ring], Fn.QC, Fn.DeW, Fn.DnS, hcx2, hcy2);
+          }
+        }
+      }
+      if (rout && rout[lt.x]) rout[lt.x][cc] = (TO)rd;
+      if (L.vis[cc]) {
+        a = fabs(rd);
+        b = rd * rd * (L.h[0] * L.h[1]);
+        f = fabs((double)L.FT[cc]);
+        n = 1;
       }
     }
-    if (rout) rout[cc] = (TO)rd;
-    if (vis[cc]) {
-      a = fabs(rd);
-      b = rd * rd;
-      f = fabs((double)FT[cc]);
-      n = 1;
-    }
   }
-  block_reduce4(a, b * area, f, n, &part[blockIdx.x]);
+  block_reduce4(a, b, f, n, &part[blockIdx.x]);
 }
 
 __global__ void __launch_bounds__(kTile) k_reduce_parts(const ResidualOut* __restrict__ parts,
@@ -395,41 +723,34 @@ __global__ void __launch_bounds__(kTile) k_reduce_parts(const ResidualOut* __res
 
 // composite mean: sum of visible u * da (ebCompositeMean.m:4-23)
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_mass(const PatchInfo* __restrict__ P,
-                                                const int2* __restrict__ tiles,
-                                                const T* __restrict__ Q,
-                                                const uint8_t* __restrict__ vis, double da,
-                                                ResidualOut* __restrict__ part) {
-  const int2 t = tiles[blockIdx.x];
-  const PatchInfo& pi = P[t.x];
+__global__ void __launch_bounds__(kTile) k_mass(LevelDev<T> L, ResidualOut* __restrict__ part) {
+  const TileRec& tr = L.tiles[blockIdx.x];
+  const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
-  const int c = t.y + (int)threadIdx.x;
+  const int c = tr.start + (int)threadIdx.x;
   double s = 0.0;
   long long n = 0;
-  if (c < mx * my && vis[pi.coff + c]) {
+  if (c < mx * my && L.vis[pi.coff + c]) {
     const int j = c / mx;
     const int i = c - j * mx;
-    s = (double)Q[pi.qoff + (i + 1) + (j + 1) * (mx + 2)];
+    s = (double)L.Q[pi.qoff + (i + 1) + (j + 1) * (mx + 2)];
     n = 1;
   }
-  block_reduce4(0.0, s * da, 0.0, n, &part[blockIdx.x]);
+  block_reduce4(0.0, s * (L.h[0] * L.h[1]), 0.0, n, &part[blockIdx.x]);
 }
 
 // ---------------------------------------------------------------- elementwise
 // Q64 interior += Q32 interior (defect-correction update)
-__global__ void __launch_bounds__(kTile) k_add_correction(const PatchInfo* __restrict__ P,
-                                                          const int2* __restrict__ tiles,
-                                                          double* __restrict__ Q,
-                                                          const float* __restrict__ E) {
-  const int2 t = tiles[blockIdx.x];
-  const PatchInfo& pi = P[t.x];
+__global__ void __launch_bounds__(kTile) k_add_correction(LevelDev<double> L, const float* __restrict__ E) {
+  const TileRec& tr = L.tiles[blockIdx.x];
+  const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
-  const int c = t.y + (int)threadIdx.x;
+  const int c = tr.start + (int)threadIdx.x;
   if (c >= mx * my) return;
   const int j = c / mx;
   const int i = c - j * mx;
   const int qc = pi.qoff + (i + 1) + (j + 1) * (mx + 2);
-  Q[qc] = Q[qc] + (double)E[qc];
+  L.Q[qc] = L.Q[qc] + (double)E[qc];
 }
 
 template <typename T>
@@ -457,19 +778,16 @@ __global__ void __launch_bounds__(kTile) k_bdf2_rhs(double* __restrict__ FT, con
 }
 
 // history rotation: Uoo = Uo; Uo = interior(Q)
-__global__ void __launch_bounds__(kTile) k_history(const PatchInfo* __restrict__ P,
-                                                   const int2* __restrict__ tiles,
-                                                   const double* __restrict__ Q,
-                                                   double* __restrict__ Uo, double* __restrict__ Uoo,
-                                                   int rotate) {
-  const int2 t = tiles[blockIdx.x];
-  const PatchInfo& pi = P[t.x];
+__global__ void __launch_bounds__(kTile) k_history(LevelDev<double> L, double* __restrict__ Uo,
+                                                   double* __restrict__ Uoo, int rotate) {
+  const TileRec& tr = L.tiles[blockIdx.x];
+  const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
-  const int c = t.y + (int)threadIdx.x;
+  const int c = tr.start + (int)threadIdx.x;
   if (c >= mx * my) return;
   const int j = c / mx;
   const int i = c - j * mx;
-  const double u = Q[pi.qoff + (i + 1) + (j + 1) * (mx + 2)];
+  const double u = L.Q[pi.qoff + (i + 1) + (j + 1) * (mx + 2)];
   const int cc = pi.coff + c;
   Uoo[cc] = rotate ? Uo[cc] : u;
   Uo[cc] = u;
@@ -477,23 +795,20 @@ __global__ void __launch_bounds__(kTile) k_history(const PatchInfo* __restrict__
 
 // InvDiagS refresh for a reaction shift: 1/(1/ID_phys + fold) with
 // ID_phys = 1/(sum D/h^2 + C) (ebKernels.m:28-34, ebBcFold.m:12-42)
-__global__ void __launch_bounds__(kTile) k_invdiag(const PatchInfo* __restrict__ P,
-                                                   const int2* __restrict__ tiles,
-                                                   const double* __restrict__ DeW,
-                                                   const double* __restrict__ DnS,
-                                                   const double* __restrict__ CC, double hx, double hy,
-                                                   double* __restrict__ ID, float* __restrict__ ID32) {
-  const int2 t = tiles[blockIdx.x];
-  const PatchInfo& pi = P[t.x];
+__global__ void __launch_bounds__(kTile) k_invdiag(LevelDev<double> L, float* __restrict__ ID32) {
+  const TileRec& tr = L.tiles[blockIdx.x];
+  const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
-  const int c = t.y + (int)threadIdx.x;
+  const int c = tr.start + (int)threadIdx.x;
   if (c >= mx * my) return;
   const int j = c / mx;
   const int i = c - j * mx;
   const int fx = pi.xoff + i + j * (mx + 1);
   const int fy = pi.yoff + i + j * mx;
-  const double hx2 = hx * hx, hy2 = hy * hy;
-  const double den = (DeW[fx + 1] + DeW[fx]) / hx2 + (DnS[fy + mx] + DnS[fy]) / hy2 + CC[pi.coff + c];
+  const double* DeW = L.DeW;
+  const double* DnS = L.DnS;
+  const double hx2 = L.h[0] * L.h[0], hy2 = L.h[1] * L.h[1];
+  const double den = (DeW[fx + 1] + DeW[fx]) / hx2 + (DnS[fy + mx] + DnS[fy]) / hy2 + L.CC[pi.coff + c];
   const double idp = 1.0 / den;
   double add = 0.0;
   if (i == 0) {
@@ -513,7 +828,7 @@ __global__ void __launch_bounds__(kTile) k_invdiag(const PatchInfo* __restrict__
     else if (pi.bc[3] == 2) add = add - DnS[fy + mx] / hy2;
   }
   const double v = 1.0 / (1.0 / idp + add);
-  ID[pi.coff + c] = v;
+  L.ID[pi.coff + c] = v;
   if (ID32) ID32[pi.coff + c] = (float)v;
 }
 
@@ -531,80 +846,91 @@ __global__ void __launch_bounds__(kTile) k_add_scalar(double* __restrict__ a, lo
 static inline unsigned nblk(long long n) { return (unsigned)((n + kTile - 1) / kTile); }
 
 template <typename T>
-void launch_relax(const LevelDev<T>& L, int colour, T omega, int homog, hipStream_t s) {
+void launch_relax(const LevelDev<T>& L, const LevelDev<T>* parent, int colour, T omega, int homog, int inl,
+                  hipStream_t s) {
   if (!L.ntile) return;
-  hipLaunchKernelGGL(k_relax<T>, dim3(L.ntile), dim3(kTile), 0, s, L.P, L.tiles, L.Q, L.F, L.ID,
-                     L.DeW, L.DnS, L.G, (T)L.h[0], (T)L.h[1], colour, omega, homog);
+  hipLaunchKernelGGL(k_relax<T>, dim3(L.ntile), dim3(kTile), 0, s, L,
+                     parent ? (const T*)parent->Q : (const T*)nullptr, colour, omega, homog, inl);
+}
+
+template <typename T>
+bool tiled_fits(const LevelDev<T>& L) {
+  return L.nstile > 0 && (long long)L.stile_lds * (long long)sizeof(T) <= kTiledLdsBytes;
+}
+
+template <typename T>
+void launch_relax_tiled(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, T omega, int homog,
+                        hipStream_t s) {
+  if (!L.nstile || nsweeps <= 0) return;
+  hipLaunchKernelGGL(k_relax_tiled<T>, dim3(L.nstile), dim3(256), (size_t)L.stile_lds * sizeof(T), s, L,
+                     parent ? (const T*)parent->Q : (const T*)nullptr, nsweeps, omega, homog);
 }
 
 template <typename T>
 void launch_fill(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, hipStream_t s) {
-  const int nring = parent ? L.nring_cf : 0;
-  const long long n = (long long)L.nghost + nring;
+  const long long n = (long long)L.nghost + (parent ? L.nring_cf : 0);
   if (!n) return;
-  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(kTile), 0, s, L.ghost, L.nghost, L.ring_cf,
-                     nring, L.Q, parent ? (const T*)parent->Q : (const T*)nullptr, L.QC, homog);
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(kTile), 0, s, L,
+                     parent ? (const T*)parent->Q : (const T*)nullptr, homog);
 }
 
 template <typename T>
 void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s) {
   if (!L.nring_all) return;
-  hipLaunchKernelGGL(k_ring<T>, dim3(nblk(L.nring_all)), dim3(kTile), 0, s, L.ring_all, L.nring_all,
-                     (const T*)parent.Q, L.QC);
+  hipLaunchKernelGGL(k_ring<T>, dim3(nblk(L.nring_all)), dim3(kTile), 0, s, L, (const T*)parent.Q);
 }
 
 template <typename T>
 void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s) {
   if (!L.nfoot) return;
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(L.nfoot)), dim3(kTile), 0, s, L.P, L.foot, L.nfoot,
-                     (const T*)L.Q, L.QC, parent.Q, qc_only);
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(L.nfoot)), dim3(kTile), 0, s, L, parent.Q, qc_only);
 }
 
 template <typename T>
-void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr,
-                        hipStream_t s) {
-  if (L.nfoot) {
-    hipLaunchKernelGGL(k_coarse_load<T>, dim3(nblk(L.nfoot)), dim3(kTile), 0, s, L.P, L.foot, L.nfoot,
-                       (const T*)L.Q, (const T*)L.F, (const T*)L.DeW, (const T*)L.DnS,
-                       (const T*)L.CC, (const T*)L.QC, (T)L.h[0], (T)L.h[1], (const T*)parent.DeW,
-                       (const T*)parent.DnS, (const T*)parent.CC, parent.F, (T)parent.h[0],
-                       (T)parent.h[1]);
-  }
-  if (mass_corr && parent.ntgt) {
-    hipLaunchKernelGGL(k_reflux<T>, dim3(nblk(parent.ntgt)), dim3(kTile), 0, s, parent.tgt_cell,
-                       parent.tgt_ptr, parent.tgt_ent, parent.ntgt, L.corr, (const T*)L.Q,
-                       (const T*)L.QC, (const T*)L.DeW, (const T*)L.DnS, parent.h[0] * parent.h[0],
-                       parent.h[1] * parent.h[1], parent.F);
-  }
+void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s) {
+  if (L.nfoot) hipLaunchKernelGGL(k_coarse_load<T>, dim3(nblk(L.nfoot)), dim3(kTile), 0, s, L, parent);
+  if (mass_corr && parent.ntgt)
+    hipLaunchKernelGGL(k_reflux<T>, dim3(nblk(parent.ntgt)), dim3(kTile), 0, s, L, parent);
+}
+
+template <typename T>
+void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s) {
+  const long long n = (long long)L.nfoot + L.nring_all + (mass_corr ? parent.ntgt : 0);
+  if (!n) return;
+  hipLaunchKernelGGL(k_load<T>, dim3(nblk(n)), dim3(kTile), 0, s, L, parent, mass_corr);
 }
 
 template <typename T>
 void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s) {
   if (!L.ntile) return;
-  hipLaunchKernelGGL(k_correct<T>, dim3(L.ntile), dim3(kTile), 0, s, L.P, L.tiles, L.Q,
-                     (const T*)L.QC, (const int*)L.wmap, (const T*)parent.Q, init);
+  hipLaunchKernelGGL(k_correct<T>, dim3(L.ntile), dim3(kTile), 0, s, L, (const T*)parent.Q, init);
 }
 
 template <typename T>
 void launch_coarsest(const CoarsestDev& C, const LevelDev<T>& L1, int homog, hipStream_t s) {
   const unsigned nb = (unsigned)((C.N + 63) / 64);
-  hipLaunchKernelGGL(k_coarsest<T>, dim3(nb), dim3(64), C.M * sizeof(double), s, C.Ainv, C.bcRhs,
-                     C.N, C.M, L1.n[0], (const T*)L1.F, L1.Q, homog);
+  hipLaunchKernelGGL(k_coarsest<T>, dim3(nb), dim3(64), C.M * sizeof(double), s, C, L1, homog);
+}
+
+template <typename T>
+void launch_coarse_cycle(const LevelDev<T>* LV_dev, int kc, const CoarsestDev& C, int npre, int npost, T omega,
+                         int homog, int mass_corr, hipStream_t s) {
+  hipLaunchKernelGGL(k_coarse_cycle<T>, dim3(1), dim3(1024), C.M * sizeof(double), s, LV_dev, kc, C, npre, npost,
+                     omega, homog, mass_corr);
+}
+
+template <typename T>
+void launch_qc_refresh_all(const LevelDev<T>* LV_dev, const int2* items, int nitems, hipStream_t s) {
+  if (!nitems) return;
+  hipLaunchKernelGGL(k_qc_refresh_all<T>, dim3(nblk(nitems)), dim3(kTile), 0, s, LV_dev, items, nitems);
 }
 
 template <typename T, typename TO>
-void launch_residual(const LevelDev<T>& L, const LevelDev<T>* fine, int mass_corr,
-                     ResidualOut* parts, TO* rout, hipStream_t s) {
-  if (!L.ntile) return;
-  const bool corr = mass_corr && fine && L.ntgt;
-  hipLaunchKernelGGL((k_residual<T, TO>), dim3(L.ntile), dim3(kTile), 0, s, L.P, L.tiles, (const T*)L.Q,
-                     (const T*)L.FT, (const T*)L.DeW, (const T*)L.DnS, (const T*)L.CC, (T)L.h[0],
-                     (T)L.h[1], (const uint8_t*)L.vis, corr ? (const int*)L.tgt_of_cell : nullptr,
-                     (const int*)L.tgt_ptr, (const int*)L.tgt_ent,
-                     corr ? (const CorrRec*)fine->corr : nullptr, corr ? (const T*)fine->Q : nullptr,
-                     corr ? (const T*)fine->QC : nullptr, corr ? (const T*)fine->DeW : nullptr,
-                     corr ? (const T*)fine->DnS : nullptr, L.h[0] * L.h[0], L.h[1] * L.h[1],
-                     L.h[0] * L.h[1], parts, rout);
+void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, int ntl, int mass_corr,
+                         ResidualOut* parts, TO* const* rout, int reset_f, int k0, hipStream_t s) {
+  if (!ntl) return;
+  hipLaunchKernelGGL((k_residual_all<T, TO>), dim3(ntl), dim3(kTile), 0, s, LV_dev, nlev, tl, mass_corr, parts,
+                     rout, reset_f, k0);
 }
 
 void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s) {
@@ -614,14 +940,12 @@ void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipSt
 template <typename T>
 void launch_mass(const LevelDev<T>& L, ResidualOut* parts, hipStream_t s) {
   if (!L.ntile) return;
-  hipLaunchKernelGGL(k_mass<T>, dim3(L.ntile), dim3(kTile), 0, s, L.P, L.tiles, (const T*)L.Q,
-                     (const uint8_t*)L.vis, L.h[0] * L.h[1], parts);
+  hipLaunchKernelGGL(k_mass<T>, dim3(L.ntile), dim3(kTile), 0, s, L, parts);
 }
 
 void launch_add_correction(const LevelDev<double>& L, const LevelDev<float>& E, hipStream_t s) {
   if (!L.ntile) return;
-  hipLaunchKernelGGL(k_add_correction, dim3(L.ntile), dim3(kTile), 0, s, L.P, L.tiles, L.Q,
-                     (const float*)E.Q);
+  hipLaunchKernelGGL(k_add_correction, dim3(L.ntile), dim3(kTile), 0, s, L, (const float*)E.Q);
 }
 
 template <typename T>
@@ -644,14 +968,12 @@ void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double
 
 void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rotate, hipStream_t s) {
   if (!L.ntile) return;
-  hipLaunchKernelGGL(k_history, dim3(L.ntile), dim3(kTile), 0, s, L.P, L.tiles, (const double*)L.Q,
-                     Uo, Uoo, rotate);
+  hipLaunchKernelGGL(k_history, dim3(L.ntile), dim3(kTile), 0, s, L, Uo, Uoo, rotate);
 }
 
 void launch_invdiag(const LevelDev<double>& L, float* ID32, hipStream_t s) {
   if (!L.ntile) return;
-  hipLaunchKernelGGL(k_invdiag, dim3(L.ntile), dim3(kTile), 0, s, L.P, L.tiles, (const double*)L.DeW,
-                     (const double*)L.DnS, (const double*)L.CC, L.h[0], L.h[1], L.ID, ID32);
+  hipLaunchKernelGGL(k_invdiag, dim3(L.ntile), dim3(kTile), 0, s, L, ID32);
 }
 
 void launch_add_scalar(double* a, long long n, double sft, float* a32, hipStream_t s) {
@@ -659,20 +981,28 @@ void launch_add_scalar(double* a, long long n, double sft, float* a32, hipStream
   hipLaunchKernelGGL(k_add_scalar, dim3(nblk(n)), dim3(kTile), 0, s, a, n, sft, a32);
 }
 
-#define BSAM_INST(T)                                                                         \
-  template void launch_relax<T>(const LevelDev<T>&, int, T, int, hipStream_t);               \
-  template void launch_fill<T>(const LevelDev<T>&, const LevelDev<T>*, int, hipStream_t);    \
-  template void launch_ring<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);         \
-  template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);     \
-  template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t); \
-  template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t); \
-  template void launch_coarsest<T>(const CoarsestDev&, const LevelDev<T>&, int, hipStream_t); \
-  template void launch_mass<T>(const LevelDev<T>&, ResidualOut*, hipStream_t);              \
+#define BSAM_INST(T)                                                                                    \
+  template void launch_relax<T>(const LevelDev<T>&, const LevelDev<T>*, int, T, int, int, hipStream_t); \
+  template bool tiled_fits<T>(const LevelDev<T>&);                                                      \
+  template void launch_relax_tiled<T>(const LevelDev<T>&, const LevelDev<T>*, int, T, int, hipStream_t); \
+  template void launch_fill<T>(const LevelDev<T>&, const LevelDev<T>*, int, hipStream_t);               \
+  template void launch_ring<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                    \
+  template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);           \
+  template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
+  template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
+  template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);            \
+  template void launch_coarsest<T>(const CoarsestDev&, const LevelDev<T>&, int, hipStream_t);           \
+  template void launch_coarse_cycle<T>(const LevelDev<T>*, int, const CoarsestDev&, int, int, T, int, int, \
+                                       hipStream_t);                                                    \
+  template void launch_qc_refresh_all<T>(const LevelDev<T>*, const int2*, int, hipStream_t);            \
+  template void launch_mass<T>(const LevelDev<T>&, ResidualOut*, hipStream_t);                          \
   template void launch_fill_value<T>(T*, long long, T, hipStream_t);
 BSAM_INST(double)
 BSAM_INST(float)
-template void launch_residual<double, float>(const LevelDev<double>&, const LevelDev<double>*, int,
-                                             ResidualOut*, float*, hipStream_t);
+template void launch_residual_all<double, float>(const LevelDev<double>*, int, const int2*, int, int, ResidualOut*,
+                                                 float* const*, int, int, hipStream_t);
+template void launch_residual_all<float, float>(const LevelDev<float>*, int, const int2*, int, int, ResidualOut*,
+                                                float* const*, int, int, hipStream_t);
 template void launch_convert<float, double>(float*, const double*, long long, hipStream_t);
 template void launch_convert<double, float>(double*, const float*, long long, hipStream_t);
 

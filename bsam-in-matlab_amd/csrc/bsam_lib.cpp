@@ -58,7 +58,9 @@ struct HLevel {
   std::vector<PatchInfo> pi;
   int ncell = 0, nq = 0, nx = 0, ny = 0, nqc = 0, ng = 0;
   std::vector<double> Q, FT, DeW, DnS, CC, ID, G;
-  std::vector<int2> tiles;
+  std::vector<TileRec> tiles;
+  std::vector<STile> stiles;
+  int stile_lds = 0;
   std::vector<GhostRec> ghost;
   std::vector<RingRec> ring_all, ring_cf;
   std::vector<FootRec> foot;
@@ -109,6 +111,16 @@ struct bsam_hier {
   ResidualOut* parts = nullptr;            // residual partials, all levels
   std::vector<int> part_off;               // per level
   int nparts = 0;
+  LevelDev<double>* lv64 = nullptr;        // device copies of the level descriptors
+  LevelDev<float>* lv32 = nullptr;
+  int2* res_tiles = nullptr;               // (level, tile) over all levels
+  int nres_tiles = 0;
+  int2* qc_items = nullptr;                // (level, item): ring + QC interior, k >= max(2, root+1)
+  int nqc_items = 0;
+  float** rout32 = nullptr;                // per level FP32 FTrue (defect right side)
+  float* q32all = nullptr;                 // FP32 Q of all levels (contiguous)
+  float* qc32all = nullptr;                // FP32 QC of all levels (contiguous)
+  long long nq32all = 0, nqc32all = 0;
   ResidualOut* dres = nullptr;
   ResidualOut* hres = nullptr;             // pinned
   std::vector<double*> Q0;                 // uploaded initial guess per level
@@ -137,9 +149,19 @@ inline int cidx(const PatchInfo& p, int i, int j) { return p.coff + i + j * p.mx
 
 void make_tiles(HLevel& L) {
   L.tiles.clear();
+  L.stiles.clear();
+  L.stile_lds = 0;
   for (int p = 0; p < L.np; ++p) {
     const int nc = L.pi[p].mx * L.pi[p].my;
-    for (int s = 0; s < nc; s += kTile) L.tiles.push_back(make_int2(p, s));
+    for (int s = 0; s < nc; s += kTile) {
+      TileRec tr{};
+      tr.p = p;
+      tr.start = s;  // tr.pi is filled once the tables are built (finalize_tiles)
+      L.tiles.push_back(tr);
+    }
+    // smoother tile = the whole patch; its LDS region is the padded patch
+    L.stiles.push_back(STile{p, 0, L.pi[p].mx - 1, 0, L.pi[p].my - 1});
+    L.stile_lds = std::max(L.stile_lds, (L.pi[p].mx + 2) * (L.pi[p].my + 2));
   }
 }
 
@@ -246,6 +268,7 @@ void build_phys_level(HLevel& L) {
   L.klass.clear();
   L.klass_off.clear();
   for (int p = 0; p < L.np; ++p) {
+    L.pi[p].gbase = (int)L.ghost.size();
     const PatchInfo& P = L.pi[p];
     L.klass_off.push_back((int)L.klass.size());
     int first[4];
@@ -325,6 +348,7 @@ int build_level_tables(HLevel& F, HLevel& C, bool qif) {
     }
     auto win = [&](int wi, int wj) { return F.wmap[P.qcoff + wi + wj * ldc]; };
     // ---- ghost classes and records (land / cornerClass / same-level)
+    F.pi[p].gbase = (int)F.ghost.size();
     F.klass_off.push_back((int)F.klass.size());
     int first[4];
     bool anycf = false;
@@ -431,6 +455,8 @@ int build_level_tables(HLevel& F, HLevel& C, bool qif) {
       for (int ci = 0; ci < cmx; ++ci) {
         FootRec fr{};
         fr.p = p; fr.ci = ci; fr.cj = cj;
+        fr.qoff = P.qoff; fr.coff = P.coff; fr.xoff = P.xoff; fr.yoff = P.yoff; fr.qcoff = P.qcoff;
+        fr.mx = P.mx; fr.my = P.my;
         fr.pq = win(ci + 1, cj + 1);
         const int q = OC.at(Il + ci, Jl + cj) - 1;
         const Box& qb = C.box[q];
@@ -643,9 +669,19 @@ int validate_level(const HLevel& L, const HLevel* C, const HLevel* Fn) {
     return fail(BSAM_E_ARG, "bsamgpu:internal: table %s index %lld outside [0,%lld)", what, v, n);
   };
   auto in = [](long long v, long long n) { return v >= 0 && v < n; };
-  for (const int2& t : L.tiles) {
-    if (!in(t.x, L.np)) return bad("tile patch", t.x, L.np);
-    if (!in(t.y, (long long)L.pi[t.x].mx * L.pi[t.x].my)) return bad("tile cell", t.y, (long long)L.pi[t.x].mx * L.pi[t.x].my);
+  for (const TileRec& t : L.tiles) {
+    if (!in(t.p, L.np)) return bad("tile patch", t.p, L.np);
+    if (!in(t.start, (long long)L.pi[t.p].mx * L.pi[t.p].my))
+      return bad("tile cell", t.start, (long long)L.pi[t.p].mx * L.pi[t.p].my);
+    if (std::memcmp(&t.pi, &L.pi[t.p], sizeof(PatchInfo)) != 0)
+      return fail(BSAM_E_ARG, "bsamgpu:internal: stale patch record in a tile");
+  }
+  for (const STile& t : L.stiles) {
+    if (!in(t.p, L.np)) return bad("smoother tile patch", t.p, L.np);
+    const PatchInfo& P = L.pi[t.p];
+    if (t.i0 < 0 || t.i1 >= P.mx || t.j0 < 0 || t.j1 >= P.my || t.i0 > t.i1 || t.j0 > t.j1 ||
+        (P.mx + 2) * (P.my + 2) > L.stile_lds)
+      return fail(BSAM_E_ARG, "bsamgpu:internal: smoother tile outside its patch");
   }
   for (const PatchInfo& P : L.pi) {
     if (P.qoff + (P.mx + 2) * (P.my + 2) > L.nq || P.coff + P.mx * P.my > L.ncell ||
@@ -738,15 +774,17 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
   D.ntile = (int)L.tiles.size(); D.nghost = (int)L.ghost.size();
   D.nring_all = (int)L.ring_all.size(); D.nring_cf = (int)L.ring_cf.size();
   D.nfoot = (int)L.foot.size(); D.ncorr = (int)L.corr.size(); D.ntgt = (int)L.tgt_cell.size();
+  D.nstile = (int)L.stiles.size(); D.stile_lds = L.stile_lds;
   int rc;
   if (tables_from64) {  // share the index tables of the FP64 level
     D.P = d64->P; D.tiles = d64->tiles; D.ghost = d64->ghost; D.ring_all = d64->ring_all;
     D.ring_cf = d64->ring_cf; D.foot = d64->foot; D.corr = d64->corr; D.tgt_cell = d64->tgt_cell;
     D.tgt_ptr = d64->tgt_ptr; D.tgt_ent = d64->tgt_ent; D.tgt_of_cell = d64->tgt_of_cell;
-    D.wmap = d64->wmap; D.vis = d64->vis;
+    D.wmap = d64->wmap; D.vis = d64->vis; D.stiles = d64->stiles;
   } else {
     if ((rc = upload(h, &D.P, L.pi))) return rc;
     if ((rc = upload(h, &D.tiles, L.tiles))) return rc;
+    if ((rc = upload(h, &D.stiles, L.stiles))) return rc;
     if ((rc = upload(h, &D.ghost, L.ghost))) return rc;
     if ((rc = upload(h, &D.ring_all, L.ring_all))) return rc;
     if ((rc = upload(h, &D.ring_cf, L.ring_cf))) return rc;
@@ -774,20 +812,51 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
 }
 
 // ------------------------------------------------------------------ sequences
+// Coarse-cycle cut: levels 1..kc (each at most `cells` cells) run their whole
+// V-cycle inside one workgroup; kc < 2 disables it.
+int coarse_cut(const bsam_hier* h, int cells) {
+  if (cells < 0) return 1;
+  if (cells == 0) cells = 1100;
+  int kc = 1;
+  while (kc < h->nlev && h->H[kc].ncell <= cells) ++kc;
+  return kc;
+}
+
+// The reference's call sequence as level-wide launches.  Three fusions keep
+// the arithmetic and the final state identical to the reference order:
+//  * inside one relax call, passes after the first evaluate internal-face
+//    ghosts from the ghost records (what the fill after the previous pass
+//    would have written) and the level is filled once after the last pass;
+//  * ring gather + FAS load + reflux run as one launch (disjoint items; the
+//    load reads ring values from the parent, which is what the gather wrote);
+//  * after the correction the fill is folded into the first post-smoothing
+//    pass (nothing reads the level's ghosts in between).
 template <typename T>
 struct Seq {
   bsam_hier* h;
   std::vector<LevelDev<T>>& L;
+  const LevelDev<T>* LVd;  // device array of the same descriptors
   int homog;
   hipStream_t s;
   const LevelDev<T>& lev(int k) { return L[k - 1]; }
-  void fill(int k) { launch_fill<T>(lev(k), k > 1 ? &L[k - 2] : nullptr, homog, s); }
-  void relax(int k, int nsweeps) {
+  const LevelDev<T>* par(int k) { return k > 1 ? &L[k - 2] : nullptr; }
+  void fill(int k) { launch_fill<T>(lev(k), par(k), homog, s); }
+  // ebRelaxLevel.m:28-38; first_inline: the first pass may also evaluate
+  // ghosts from the records (only when the preceding fill was skipped)
+  void relax(int k, int nsweeps, int first_inline = 0) {
+    if (nsweeps > 0 && h->opts.smoother == 1 && tiled_fits<T>(lev(k))) {
+      // tiled smoother: pass 1 reads the materialised ghost frame
+      if (first_inline) fill(k);
+      launch_relax_tiled<T>(lev(k), par(k), nsweeps, (T)h->opts.omega, homog, s);
+      fill(k);
+      return;
+    }
     for (int it = 0; it < nsweeps; ++it)
       for (int colour = 1; colour <= 2; ++colour) {
-        launch_relax<T>(lev(k), colour, (T)h->opts.omega, homog, s);
-        fill(k);
+        const int inl = (it == 0 && colour == 1) ? first_inline : 1;
+        launch_relax<T>(lev(k), par(k), colour, (T)h->opts.omega, homog, inl, s);
       }
+    if (nsweeps > 0) fill(k);
   }
   void restrict_(int k) { launch_restrict<T>(lev(k), lev(k - 1), 0, s); }
   void ring(int k) { launch_ring<T>(lev(k), lev(k - 1), s); }
@@ -797,7 +866,7 @@ struct Seq {
     launch_coarsest<T>(h->cst, lev(1), homog, s);
     fill(1);
   }
-  void vcycle(int k) {  // ebVcycle.m:15-35
+  void vcycle_ref(int k) {  // ebVcycle.m:15-35, one launch per reference step
     if (k == 1) {
       coarsest();
       return;
@@ -807,10 +876,32 @@ struct Seq {
     fill(k - 1);
     ring(k);
     coarse_load(k);
-    vcycle(k - 1);
+    vcycle_ref(k - 1);
     correct(k);
     fill(k);
     relax(k, h->opts.npost);
+  }
+  void vcycle(int k, int kc) {  // ebVcycle.m:15-35, fused
+    if (k <= kc && kc >= 2) {
+      launch_coarse_cycle<T>(LVd, k, h->cst, h->opts.npre, h->opts.npost, (T)h->opts.omega, homog,
+                             h->opts.mass_correction, s);
+      return;
+    }
+    if (k == 1) {
+      coarsest();
+      return;
+    }
+    relax(k, h->opts.npre);
+    restrict_(k);
+    fill(k - 1);
+    launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s);
+    vcycle(k - 1, kc);
+    correct(k);
+    if (h->opts.npost > 0) {
+      relax(k, h->opts.npost, 1);
+    } else {
+      fill(k);
+    }
   }
   void filldown() {  // ebTransfer.m:73-79
     for (int kk = h->nlev; kk >= 2; --kk) {
@@ -818,33 +909,24 @@ struct Seq {
       fill(kk - 1);
     }
   }
-  int reset_force() {
-    for (auto& l : L)
-      if (l.ncell) {
-        hipError_t e = hipMemcpyAsync(l.F, l.FT, (size_t)l.ncell * sizeof(T), hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return fail(BSAM_E_HIP, "bsamgpu:hip: F reset failed: %s", hipGetErrorString(e));
-      }
-    return 0;
-  }
 };
 
-// composite residual (ebCompResidual.m:23-84) on the FP64 hierarchy; writes
-// per-cell r into rout (FP32 inner right side) when given.
+// F = FTrue on every level in one launch (ebSolveCycles.m:30-34)
+template <typename T>
+void enqueue_reset_force(bsam_hier* h, const LevelDev<T>* LVd, hipStream_t s) {
+  launch_residual_all<T, float>(LVd, h->nlev, h->res_tiles, h->nres_tiles, 0, h->parts, nullptr, 1,
+                                h->nlev + 1, s);
+}
+
+// composite residual (ebCompResidual.m:23-84) on the FP64 hierarchy: one
+// launch refreshes every QC (ring + interior), one evaluates every level's
+// residual with its reflux chunks, one reduces; rout (device array of per-level
+// FP32 pointers) receives r when given.
 void enqueue_residual(bsam_hier* h, float* const* rout, hipStream_t s) {
-  auto& L = h->d64;
-  const int r0 = h->root;
-  if (h->opts.mass_correction) {
-    for (int k = std::max(2, r0 + 1); k <= h->nlev; ++k) {
-      launch_ring<double>(L[k - 1], L[k - 2], s);
-      launch_restrict<double>(L[k - 1], L[k - 2], 1, s);  // QC interior only
-    }
-  }
-  for (int k = r0; k <= h->nlev; ++k) {
-    const LevelDev<double>* fine = k < h->nlev ? &L[k] : nullptr;
-    launch_residual<double, float>(L[k - 1], fine, h->opts.mass_correction, h->parts + h->part_off[k - 1],
-                                   rout ? rout[k - 1] : nullptr, s);
-  }
-  launch_reduce(h->parts + h->part_off[r0 - 1], h->nparts - h->part_off[r0 - 1], h->dres, s);
+  if (h->opts.mass_correction) launch_qc_refresh_all<double>(h->lv64, h->qc_items, h->nqc_items, s);
+  launch_residual_all<double, float>(h->lv64, h->nlev, h->res_tiles, h->nres_tiles, h->opts.mass_correction,
+                                     h->parts, rout, 0, h->root, s);
+  launch_reduce(h->parts, h->nres_tiles, h->dres, s);
 }
 
 }  // namespace
@@ -856,7 +938,7 @@ void bsam_default_opts(bsam_solve_opts* o) {
   if (!o) return;
   o->npre = 3; o->npost = 3; o->omega = 1.0; o->rel_tol = 1e-9; o->max_vcycles = 60;
   o->ghost_interp = 0; o->mass_correction = 1; o->precision = BSAM_FP64; o->inner_cycles = 1;
-  o->use_graph = 1;
+  o->use_graph = 1; o->coarse_cells = 0; o->smoother = 0;
 }
 
 const char* bsam_last_error(void) { return g_err.c_str(); }
@@ -961,6 +1043,8 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
     int rc = build_level_tables(h->H[k], h->H[k - 1], qif);
     if (rc) return rc;
   }
+  for (auto& L : h->H)  // patch records are final now: copy them into the tiles
+    for (TileRec& t : L.tiles) t.pi = L.pi[t.p];
   // singular iff all-Neumann level 1 and C == 0 everywhere (ebCoarsest.m:58-66)
   bool allneu = true;
   for (int s = 0; s < 4; ++s) allneu = allneu && h->H[0].pi[0].bc[s] != 1;
@@ -989,11 +1073,22 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
     if ((rc = upload(h, &h->Q0[k], h->H[k].Q))) return rc;
   h->part_off.resize(h->nlev + 1);
   h->nparts = 0;
+  std::vector<int2> res_tiles, qc_items;
   for (int k = 0; k < h->nlev; ++k) {
     h->part_off[k] = h->nparts;
     h->nparts += (int)h->H[k].tiles.size();
+    for (int t = 0; t < (int)h->H[k].tiles.size(); ++t) res_tiles.push_back(make_int2(k, t));
+    if (k + 1 >= std::max(2, h->root + 1)) {
+      const int n = (int)h->H[k].ring_all.size() + (int)h->H[k].foot.size();
+      for (int t = 0; t < n; ++t) qc_items.push_back(make_int2(k, t));
+    }
   }
   h->part_off[h->nlev] = h->nparts;
+  h->nres_tiles = (int)res_tiles.size();
+  h->nqc_items = (int)qc_items.size();
+  if ((rc = upload(h, &h->res_tiles, res_tiles))) return rc;
+  if ((rc = upload(h, &h->qc_items, qc_items))) return rc;
+  if ((rc = upload(h, &h->lv64, h->d64))) return rc;
   if ((rc = h->mem.alloc(&h->parts, std::max(1, h->nparts)))) return rc;
   if ((rc = h->mem.alloc(&h->dres, 1))) return rc;
   HIPCHK(hipHostMalloc((void**)&h->hres, sizeof(ResidualOut), hipHostMallocDefault));
@@ -1030,14 +1125,38 @@ void bsam_hier_destroy(bsam_hier* h) {
   delete h;
 }
 
+// FP32 copy of the hierarchy for the inner V-cycles (tables shared with the
+// FP64 levels).  Q and QC of all levels live in one buffer each so that the
+// per-step zeroing is a single launch.
 static int ensure32(bsam_hier* h) {
   if (h->have32) return 0;
   h->d32.resize(h->nlev);
+  long long nq = 0, nqc = 0;
   for (int k = 0; k < h->nlev; ++k) {
     int rc = make_dev_level<float>(h, h->H[k], h->d32[k], true, &h->d64[k]);
     if (rc) return rc;
     launch_fill_value<float>(h->d32[k].FT, h->H[k].ncell, 0.0f, h->stream);
+    nq += h->H[k].nq;
+    nqc += h->H[k].nqc;
   }
+  int rc;
+  if ((rc = h->mem.alloc(&h->q32all, std::max(1LL, nq)))) return rc;
+  if ((rc = h->mem.alloc(&h->qc32all, std::max(1LL, nqc)))) return rc;
+  h->nq32all = nq;
+  h->nqc32all = nqc;
+  long long oq = 0, oqc = 0;
+  std::vector<float*> rout(h->nlev);
+  for (int k = 0; k < h->nlev; ++k) {
+    h->d32[k].Q = h->q32all + oq;
+    h->d32[k].QC = h->qc32all + oqc;
+    oq += h->H[k].nq;
+    oqc += h->H[k].nqc;
+    rout[k] = h->d32[k].FT;
+  }
+  launch_fill_value<float>(h->q32all, nq, 0.0f, h->stream);
+  launch_fill_value<float>(h->qc32all, nqc, 0.0f, h->stream);
+  if ((rc = upload(h, &h->lv32, h->d32))) return rc;
+  if ((rc = upload(h, &h->rout32, rout))) return rc;
   h->have32 = true;
   HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
@@ -1056,20 +1175,24 @@ static int sync(bsam_hier* h) {
   return 0;
 }
 
+static Seq<double> seq64(bsam_hier* h, hipStream_t s) { return Seq<double>{h, h->d64, h->lv64, 0, s}; }
+static Seq<float> seq32(bsam_hier* h, hipStream_t s) { return Seq<float>{h, h->d32, h->lv32, 1, s}; }
+
 int bsam_fill_ghosts(bsam_hier* h, int k) {
   if (int rc = check_level(h, k, 1)) return rc;
-  Seq<double>{h, h->d64, 0, h->stream}.fill(k);
+  seq64(h, h->stream).fill(k);
   return sync(h);
 }
 
 int bsam_relax_level(bsam_hier* h, int k, int nsweeps) {
   if (int rc = check_level(h, k, 1)) return rc;
-  Seq<double>{h, h->d64, 0, h->stream}.relax(k, nsweeps);
+  seq64(h, h->stream).relax(k, nsweeps);
   return sync(h);
 }
 
 int bsam_transfer(bsam_hier* h, int k, int what) {
-  Seq<double> s{h, h->d64, 0, h->stream};
+  if (int rc_ = need_dev(h)) return rc_;
+  Seq<double> s = seq64(h, h->stream);
   if (what == BSAM_FILLDOWN) {
     s.filldown();
     return sync(h);
@@ -1086,31 +1209,31 @@ int bsam_transfer(bsam_hier* h, int k, int what) {
 
 int bsam_coarse_load(bsam_hier* h, int k) {
   if (int rc = check_level(h, k, 2)) return rc;
-  Seq<double>{h, h->d64, 0, h->stream}.coarse_load(k);
+  seq64(h, h->stream).coarse_load(k);
   return sync(h);
 }
 
 int bsam_coarsest_solve(bsam_hier* h) {
   if (int rc_ = need_dev(h)) return rc_;
-  Seq<double>{h, h->d64, 0, h->stream}.coarsest();
+  seq64(h, h->stream).coarsest();
   return sync(h);
 }
 
 int bsam_vcycle(bsam_hier* h) {
   if (int rc_ = need_dev(h)) return rc_;
-  Seq<double>{h, h->d64, 0, h->stream}.vcycle(h->nlev);
+  seq64(h, h->stream).vcycle(h->nlev, coarse_cut(h, h->opts.coarse_cells));
   return sync(h);
 }
 
 int bsam_reset_force(bsam_hier* h) {
   if (int rc_ = need_dev(h)) return rc_;
-  if (int rc = Seq<double>{h, h->d64, 0, h->stream}.reset_force()) return rc;
+  enqueue_reset_force<double>(h, h->lv64, h->stream);
   return sync(h);
 }
 
 int bsam_init_level_prolong(bsam_hier* h, int k) {
   if (int rc = check_level(h, k, 2)) return rc;
-  Seq<double> s{h, h->d64, 0, h->stream};
+  Seq<double> s = seq64(h, h->stream);
   s.fill(k - 1);
   launch_correct<double>(h->d64[k - 1], h->d64[k - 2], 1, h->stream);
   s.fill(k);
@@ -1140,37 +1263,38 @@ int bsam_comp_residual(bsam_hier* h, bsam_residual* out) {
 // one cycle of the plain FP64 loop: reset F, V-cycle, filldown, residual,
 // residual scalars to pinned host memory
 static int enqueue_cycle64(bsam_hier* h, hipStream_t s) {
-  Seq<double> q{h, h->d64, 0, s};
-  if (int rc = q.reset_force()) return rc;
-  q.vcycle(h->nlev);
+  Seq<double> q = seq64(h, s);
+  enqueue_reset_force<double>(h, h->lv64, s);
+  q.vcycle(h->nlev, coarse_cut(h, h->opts.coarse_cells));
   q.filldown();
   enqueue_residual(h, nullptr, s);
   HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, s));
   return 0;
 }
 
-// one step of the mixed loop: FP32 V-cycles on the correction problem with the
-// current FP64 composite residual as right side, FP64 update, filldown,
-// ghost refresh and composite residual (which also writes the next right side)
+// one step of the mixed loop: FP32 V-cycles on the correction problem
+// (homogeneous boundary data, zero initial guess, the current FP64 composite
+// residual as right side), FP64 update, filldown, ghost refresh and composite
+// residual (which also writes the next right side).  With a zero initial
+// guess and homogeneous data every ghost is zero, so no initial fill is needed;
+// covered cells of the correction are recomputed by the FP64 filldown, so the
+// FP32 filldown only runs between inner cycles.
 static int enqueue_cycle_mixed(bsam_hier* h, hipStream_t s) {
-  Seq<float> q{h, h->d32, 1, s};
-  for (int k = 0; k < h->nlev; ++k) {
-    launch_fill_value<float>(h->d32[k].Q, h->H[k].nq, 0.0f, s);
-    launch_fill_value<float>(h->d32[k].QC, h->H[k].nqc, 0.0f, s);
-  }
-  for (int k = 1; k <= h->nlev; ++k) q.fill(k);
-  for (int it = 0; it < std::max(1, h->opts.inner_cycles); ++it) {
-    if (int rc = q.reset_force()) return rc;
-    q.vcycle(h->nlev);
-    q.filldown();
+  Seq<float> q = seq32(h, s);
+  const int kc = coarse_cut(h, h->opts.coarse_cells);
+  launch_fill_value<float>(h->q32all, h->nq32all, 0.0f, s);
+  launch_fill_value<float>(h->qc32all, h->nqc32all, 0.0f, s);
+  const int inner = std::max(1, h->opts.inner_cycles);
+  for (int it = 0; it < inner; ++it) {
+    enqueue_reset_force<float>(h, h->lv32, s);
+    q.vcycle(h->nlev, kc);
+    if (it + 1 < inner) q.filldown();
   }
   for (int k = 0; k < h->nlev; ++k) launch_add_correction(h->d64[k], h->d32[k], s);
-  Seq<double> d{h, h->d64, 0, s};
+  Seq<double> d = seq64(h, s);
   d.filldown();
   for (int k = 1; k <= h->nlev; ++k) d.fill(k);
-  std::vector<float*> rout(h->nlev);
-  for (int k = 0; k < h->nlev; ++k) rout[k] = h->d32[k].FT;
-  enqueue_residual(h, rout.data(), s);
+  enqueue_residual(h, h->rout32, s);
   HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, s));
   return 0;
 }
@@ -1184,7 +1308,8 @@ static int run_cycle(bsam_hier* h, bool mixed) {
     return 0;
   }
   const int key = (mixed ? 1 : 0) | (h->opts.inner_cycles << 1) | (h->opts.npre << 8) | (h->opts.npost << 12) |
-                  (h->opts.mass_correction << 16) | (h->opts.omega == 1.0 ? 0 : (1 << 17));
+                  (h->opts.mass_correction << 16) | (h->opts.omega == 1.0 ? 0 : (1 << 17)) |
+                  (coarse_cut(h, h->opts.coarse_cells) << 18) | (h->opts.smoother << 26);
   if (!h->gexec || h->graph_key != key || h->graph_omega != h->opts.omega) {
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
     if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
@@ -1221,12 +1346,9 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, h->stream));
   // ebSolveCycles.m:11-19: fill every level, baseline residual
-  Seq<double> d{h, h->d64, 0, h->stream};
+  Seq<double> d = seq64(h, h->stream);
   for (int k = 1; k <= h->nlev; ++k) d.fill(k);
-  std::vector<float*> rout(h->nlev, nullptr);
-  if (mixed)
-    for (int k = 0; k < h->nlev; ++k) rout[k] = h->d32[k].FT;
-  enqueue_residual(h, mixed ? rout.data() : nullptr, h->stream);
+  enqueue_residual(h, mixed ? h->rout32 : nullptr, h->stream);
   HIPCHK(hipGetLastError());
   bsam_residual res{};
   if (int rc = read_residual(h, &res)) return rc;
@@ -1405,10 +1527,13 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
   HIPCHK(hipEventRecord(e0, h->stream));
   for (int i = 0; i < nlaunch; ++i) {
     const int colour = 1 + (i & 1);
+    // inline ghosts, as in every pass after the first of a relax call
     if (precision == BSAM_MIXED)
-      launch_relax<float>(h->d32[k - 1], colour, (float)h->opts.omega, 1, h->stream);
+      launch_relax<float>(h->d32[k - 1], k > 1 ? &h->d32[k - 2] : nullptr, colour, (float)h->opts.omega, 1, 1,
+                          h->stream);
     else
-      launch_relax<double>(h->d64[k - 1], colour, h->opts.omega, 0, h->stream);
+      launch_relax<double>(h->d64[k - 1], k > 1 ? &h->d64[k - 2] : nullptr, colour, h->opts.omega, 0, 1,
+                           h->stream);
   }
   HIPCHK(hipEventRecord(e1, h->stream));
   HIPCHK(hipEventSynchronize(e1));

@@ -92,6 +92,13 @@ typedef struct {
   int precision;       /* BSAM_FP64 or BSAM_MIXED (FP32 inner V-cycles, FP64 defect correction) */
   int inner_cycles;    /* FP32 V-cycles per defect-correction step (mixed only, >= 1) */
   int use_graph;       /* replay each cycle as a captured HIP graph */
+  int coarse_cells;    /* levels 1..kc with at most this many cells each run their
+                          V-cycle inside one workgroup (0: default 1100, -1: off) */
+  int smoother;        /* 0: red-black Gauss-Seidel exactly as ebRelaxLevel (one launch
+                          per colour, bit-identical to the reference sequence);
+                          1: LDS-tiled -- every colour pass of a relax call in one
+                          launch, one patch per workgroup, sibling ghosts frozen
+                          for the call (same converged solution) */
 } bsam_solve_opts;
 
 /* One solve stage (EllipticBSAM/src/ebSolveCycles.m:58-70) plus device timing. */
