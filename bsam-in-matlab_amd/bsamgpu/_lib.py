@@ -39,10 +39,13 @@ EXPORTS = [
     "bsam_hier_create", "bsam_hier_destroy", "bsam_solve_cycles", "bsam_comp_residual", "bsam_vcycle",
     "bsam_fill_ghosts", "bsam_relax_level", "bsam_transfer", "bsam_coarse_load", "bsam_coarsest_solve",
     "bsam_reset_force", "bsam_init_level_prolong", "bsam_composite_mean", "bsam_level_field_size",
-    "bsam_get_level", "bsam_set_level", "bsam_export_size", "bsam_export", "bsam_parabolic_rhs",
-    "bsam_parabolic_advance", "bsam_parabolic_init", "bsam_set_reaction_shift", "bsam_hier_reset",
-    "bsam_time_smoother", "bsam_smoother_bytes",
+    "bsam_get_level", "bsam_set_level", "bsam_export_size", "bsam_export", "bsam_hier_reset",
+    "bsam_time_smoother", "bsam_smoother_bytes", "bsam_update_coeffs", "bsam_parabolic_init",
+    "bsam_parabolic_set_history", "bsam_parabolic_get_history", "bsam_parabolic_set_source",
+    "bsam_parabolic_rhs", "bsam_parabolic_advance", "bsam_parabolic_step",
 ]
+
+BSAM_BE, BSAM_BDF2 = 0, 1
 
 
 class HierDesc(ctypes.Structure):
@@ -108,10 +111,15 @@ def load(path=None):
         "bsam_set_level": (c_int, [vp, c_int, c_int, P_double]),
         "bsam_export_size": (c_ll, [vp, c_int, c_int]),
         "bsam_export": (c_int, [vp, c_int, c_int, P_ll]),
-        "bsam_parabolic_rhs": (c_int, [vp, P_double, c_double, c_int]),
-        "bsam_parabolic_advance": (c_int, [vp]),
+        "bsam_update_coeffs": (c_int, [vp, ctypes.POINTER(HierDesc)]),
         "bsam_parabolic_init": (c_int, [vp]),
-        "bsam_set_reaction_shift": (c_int, [vp, c_double]),
+        "bsam_parabolic_set_history": (c_int, [vp, P_double, P_double]),
+        "bsam_parabolic_get_history": (c_int, [vp, c_int, P_double]),
+        "bsam_parabolic_set_source": (c_int, [vp, P_double]),
+        "bsam_parabolic_rhs": (c_int, [vp, c_double, c_int]),
+        "bsam_parabolic_advance": (c_int, [vp]),
+        "bsam_parabolic_step": (c_int, [vp, c_double, c_int, ctypes.POINTER(SolveOpts),
+                                        ctypes.POINTER(StageStats), P_double]),
         "bsam_hier_reset": (c_int, [vp]),
         "bsam_time_smoother": (c_int, [vp, c_int, c_int, c_int, P_double]),
         "bsam_smoother_bytes": (c_double, [vp, c_int, c_int]),

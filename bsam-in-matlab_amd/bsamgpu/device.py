@@ -86,8 +86,15 @@ class GpuHierarchy:
         self.nlev = H.nlev
         self.rootIdx = H.rootIdx
         ctx = context(device)
+        d, self._pk = self._desc(H)
+        so = solve_opts(self.opts)
+        self.h = ctypes.c_void_p()
+        check(self.lib.bsam_hier_create(ctx, ctypes.byref(d), ctypes.byref(so), ctypes.byref(self.h)),
+              "bsam_hier_create")
+
+    @staticmethod
+    def _desc(H):
         pk = pack_hierarchy(H)
-        self._pk = pk
         d = _lib.HierDesc()
         d.nlev = H.nlev
         d.root_idx = H.rootIdx
@@ -96,10 +103,7 @@ class GpuHierarchy:
         d.n, d.np, d.boxes, d.bc_code = iptr(pk["n"]), iptr(pk["np"]), iptr(pk["boxes"]), iptr(pk["bc"])
         d.Q, d.FTrue, d.DeW, d.DnS = dptr(pk["Q"]), dptr(pk["FTrue"]), dptr(pk["DeW"]), dptr(pk["DnS"])
         d.CC, d.InvDiagS, d.g = dptr(pk["CC"]), dptr(pk["ID"]), dptr(pk["g"])
-        so = solve_opts(self.opts)
-        self.h = ctypes.c_void_p()
-        check(self.lib.bsam_hier_create(ctx, ctypes.byref(d), ctypes.byref(so), ctypes.byref(self.h)),
-              "bsam_hier_create")
+        return d, pk  # pk keeps the arrays behind the pointers alive
 
     def close(self):
         if getattr(self, "h", None):
@@ -122,9 +126,7 @@ class GpuHierarchy:
         st.res_hist = dptr(hist)
         st.res_hist_cap = cap
         check(self.lib.bsam_solve_cycles(self.h, ctypes.byref(o), ctypes.byref(st)), "bsam_solve_cycles")
-        return {"amrLevels": st.amr_levels, "nlev": st.nlev, "vcycles": st.vcycles, "finalRel": st.final_rel,
-                "visibleCells": st.visible_cells, "totalCells": st.total_cells, "avgFactor": st.avg_factor,
-                "resHist": hist[:st.res_hist_len].tolist(), "gpu_ms": st.gpu_ms}
+        return _stage(st, hist)
 
     def comp_residual(self):
         r = _lib.Residual()
@@ -219,15 +221,81 @@ class GpuHierarchy:
         return self.lib.bsam_smoother_bytes(self.h, k, elem_bytes)
 
     # ------------------------------------------------------------ parabolic
+    def update_coeffs(self, H=None):
+        """ebpSetTime's refresh (ebpSetTime.m:19-51) from the host hierarchy's
+        current CC/DeW/DnS/InvDiagS/bc data (same boxes)."""
+        d, pk = self._desc(H or self.H)
+        check(self.lib.bsam_update_coeffs(self.h, ctypes.byref(d)), "bsam_update_coeffs")
+
+    def _pack_cells(self, U):
+        return _packF([U[k - 1][p] for k in range(1, self.nlev + 1) for p in range(self.H.L(k).np)])
+
+    def _unpack_cells(self, buf):
+        out, off = [], 0
+        for k in range(1, self.nlev + 1):
+            lev = []
+            for p in range(self.H.L(k).np):
+                mx, my = self.H.L(k).dims(p)
+                lev.append(buf[off:off + mx * my].reshape((mx, my), order="F").copy())
+                off += mx * my
+            out.append(lev)
+        return out
+
+    def _ncell_all(self):
+        return sum(int(np.prod(self.H.L(k).dims(p))) for k in range(1, self.nlev + 1) for p in range(self.H.L(k).np))
+
     def parabolic_init(self):
+        """Uo = interior(Q), no u^{n-1} yet (ebpInit.m:32-35)."""
         check(self.lib.bsam_parabolic_init(self.h), "bsam_parabolic_init")
 
-    def parabolic_rhs(self, f_packed, dt, first):
-        ptr = dptr(f_packed) if f_packed is not None else None
-        check(self.lib.bsam_parabolic_rhs(self.h, ptr, float(dt), int(first)), "bsam_parabolic_rhs")
+    def set_history(self, Uo=None, Uoo=None):
+        """Upload u^n and/or u^{n-1}: per level, per patch (mx, my) arrays."""
+        a = self._pack_cells(Uo) if Uo is not None else None
+        b = self._pack_cells(Uoo) if Uoo is not None else None
+        check(self.lib.bsam_parabolic_set_history(self.h, dptr(a) if a is not None else None,
+                                                  dptr(b) if b is not None else None),
+              "bsam_parabolic_set_history")
+
+    def get_history(self, which=0):
+        buf = np.zeros(self._ncell_all())
+        check(self.lib.bsam_parabolic_get_history(self.h, int(which), dptr(buf)), "bsam_parabolic_get_history")
+        return self._unpack_cells(buf)
+
+    def set_source(self, F=None):
+        """Resident source f(t1) (per level, per patch); None means f = 0."""
+        a = self._pack_cells(F) if F is not None else None
+        check(self.lib.bsam_parabolic_set_source(self.h, dptr(a) if a is not None else None),
+              "bsam_parabolic_set_source")
+
+    def parabolic_rhs(self, dt, scheme):
+        check(self.lib.bsam_parabolic_rhs(self.h, float(dt), _scheme_code(scheme)), "bsam_parabolic_rhs")
 
     def parabolic_advance(self):
         check(self.lib.bsam_parabolic_advance(self.h), "bsam_parabolic_advance")
 
-    def set_reaction_shift(self, shift):
-        check(self.lib.bsam_set_reaction_shift(self.h, float(shift)), "bsam_set_reaction_shift")
+    def parabolic_step(self, dt, scheme, opts=None, **kw):
+        """One accepted step (ebpSolve.m:40-58): rhs, ebSolveCycles, history,
+        compMass.  Returns (stage record, mass)."""
+        o = solve_opts(opts or self.opts, **kw)
+        cap = int(o.max_vcycles) + 1
+        hist = np.zeros(cap)
+        st = _lib.StageStats()
+        st.res_hist = dptr(hist)
+        st.res_hist_cap = cap
+        m = ctypes.c_double()
+        check(self.lib.bsam_parabolic_step(self.h, float(dt), _scheme_code(scheme), ctypes.byref(o),
+                                           ctypes.byref(st), ctypes.byref(m)), "bsam_parabolic_step")
+        return _stage(st, hist), m.value
+
+
+def _scheme_code(scheme):
+    codes = {"be": _lib.BSAM_BE, "bdf2": _lib.BSAM_BDF2}
+    if scheme not in codes:
+        raise _lib.BsamError("ebp_rhs:scheme", f"scheme {scheme} is not available on the device (be, bdf2)")
+    return codes[scheme]
+
+
+def _stage(st, hist):
+    return {"amrLevels": st.amr_levels, "nlev": st.nlev, "vcycles": st.vcycles, "finalRel": st.final_rel,
+            "visibleCells": st.visible_cells, "totalCells": st.total_cells, "avgFactor": st.avg_factor,
+            "resHist": hist[:st.res_hist_len].tolist(), "gpu_ms": st.gpu_ms}

@@ -43,7 +43,7 @@ enum GhostKind : int {
 struct GhostRec {
   int dst, kind;
   int s[5];
-  int pad;
+  int gi;    // boundary-data index of c (G_DIR / G_NEU records only)
   double c;  // g (Dirichlet) or h*g (Neumann)
 };
 
@@ -75,7 +75,8 @@ struct CorrRec {
   int d1, d2;          // face diffusivities (DeW or DnS)
   int in1, g1, in2, g2;// fine interior / ghost Q indices
   int ring, qcin;      // QC ring / QC interior indices
-  int face, pad;
+  int face;
+  int cfidx;           // parent DeW (W/E) or DnS (S/N) index of cf
   double cf;           // coarse interface-face diffusivity
 };
 

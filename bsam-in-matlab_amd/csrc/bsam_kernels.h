@@ -38,7 +38,5 @@ template <typename TD, typename TS> void launch_convert(TD* d, const TS* src, lo
 void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double* Uoo, double dt, int first,
                      long long n, hipStream_t s);
 void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rotate, hipStream_t s);
-void launch_invdiag(const LevelDev<double>& L, float* ID32, hipStream_t s);
-void launch_add_scalar(double* a, long long n, double sft, float* a32, hipStream_t s);
 
 }  // namespace bsam

@@ -793,55 +793,6 @@ __global__ void __launch_bounds__(kTile) k_history(LevelDev<double> L, double* _
   Uo[cc] = u;
 }
 
-// InvDiagS refresh for a reaction shift: 1/(1/ID_phys + fold) with
-// ID_phys = 1/(sum D/h^2 + C) (ebKernels.m:28-34, ebBcFold.m:12-42)
-__global__ void __launch_bounds__(kTile) k_invdiag(LevelDev<double> L, float* __restrict__ ID32) {
-  const TileRec& tr = L.tiles[blockIdx.x];
-  const PatchInfo& pi = tr.pi;
-  const int mx = pi.mx, my = pi.my;
-  const int c = tr.start + (int)threadIdx.x;
-  if (c >= mx * my) return;
-  const int j = c / mx;
-  const int i = c - j * mx;
-  const int fx = pi.xoff + i + j * (mx + 1);
-  const int fy = pi.yoff + i + j * mx;
-  const double* DeW = L.DeW;
-  const double* DnS = L.DnS;
-  const double hx2 = L.h[0] * L.h[0], hy2 = L.h[1] * L.h[1];
-  const double den = (DeW[fx + 1] + DeW[fx]) / hx2 + (DnS[fy + mx] + DnS[fy]) / hy2 + L.CC[pi.coff + c];
-  const double idp = 1.0 / den;
-  double add = 0.0;
-  if (i == 0) {
-    if (pi.bc[0] == 1) add = add + 2 * DeW[fx] / hx2;
-    else if (pi.bc[0] == 2) add = add - DeW[fx] / hx2;
-  }
-  if (i == mx - 1) {
-    if (pi.bc[1] == 1) add = add + 2 * DeW[fx + 1] / hx2;
-    else if (pi.bc[1] == 2) add = add - DeW[fx + 1] / hx2;
-  }
-  if (j == 0) {
-    if (pi.bc[2] == 1) add = add + 2 * DnS[fy] / hy2;
-    else if (pi.bc[2] == 2) add = add - DnS[fy] / hy2;
-  }
-  if (j == my - 1) {
-    if (pi.bc[3] == 1) add = add + 2 * DnS[fy + mx] / hy2;
-    else if (pi.bc[3] == 2) add = add - DnS[fy + mx] / hy2;
-  }
-  const double v = 1.0 / (1.0 / idp + add);
-  L.ID[pi.coff + c] = v;
-  if (ID32) ID32[pi.coff + c] = (float)v;
-}
-
-__global__ void __launch_bounds__(kTile) k_add_scalar(double* __restrict__ a, long long n, double s,
-                                                      float* __restrict__ a32) {
-  const long long t = (long long)blockIdx.x * kTile + threadIdx.x;
-  if (t < n) {
-    const double v = a[t] + s;
-    a[t] = v;
-    if (a32) a32[t] = (float)v;
-  }
-}
-
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(long long n) { return (unsigned)((n + kTile - 1) / kTile); }
 
@@ -969,16 +920,6 @@ void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double
 void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rotate, hipStream_t s) {
   if (!L.ntile) return;
   hipLaunchKernelGGL(k_history, dim3(L.ntile), dim3(kTile), 0, s, L, Uo, Uoo, rotate);
-}
-
-void launch_invdiag(const LevelDev<double>& L, float* ID32, hipStream_t s) {
-  if (!L.ntile) return;
-  hipLaunchKernelGGL(k_invdiag, dim3(L.ntile), dim3(kTile), 0, s, L, ID32);
-}
-
-void launch_add_scalar(double* a, long long n, double sft, float* a32, hipStream_t s) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_add_scalar, dim3(nblk(n)), dim3(kTile), 0, s, a, n, sft, a32);
 }
 
 #define BSAM_INST(T)                                                                                    \

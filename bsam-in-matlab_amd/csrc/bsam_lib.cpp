@@ -126,7 +126,9 @@ struct bsam_hier {
   std::vector<double*> Q0;                 // uploaded initial guess per level
   double* Uo = nullptr;                    // parabolic history (packed cells, all levels)
   double* Uoo = nullptr;
-  double* fbuf = nullptr;
+  bool have_uoo = false;                   // Uoo holds u^{n-1} (after one step or an upload)
+  double* src = nullptr;                   // resident source f(t1), packed cells
+  bool have_src = false;
   std::vector<long long> cell_base;        // per level offset in the history arrays
   long long ncell_all = 0;
   hipGraph_t graph = nullptr;
@@ -179,7 +181,8 @@ GhostRec phys_rec(const HLevel& L, const PatchInfo& P, int side, int t) {
     default: dst = qidx(P, t + 1, my + 1); s0 = qidx(P, t + 1, my); s1 = qidx(P, t + 1, my - 1); h = L.h[1]; break;
   }
   const int goff = side == 0 ? 0 : side == 1 ? my : side == 2 ? 2 * my : 2 * my + mx;
-  const double g = L.G[P.goff + goff + t];
+  r.gi = P.goff + goff + t;
+  const double g = L.G[r.gi];
   r.dst = dst;
   if (P.bc[side] == 1) {
     r.kind = G_DIR; r.s[0] = s0; r.s[1] = s1; r.c = g;
@@ -492,25 +495,29 @@ int build_level_tables(HLevel& F, HLevel& C, bool qif) {
             e.d1 = P.xoff + 0 + a * (mx + 1); e.d2 = P.xoff + 0 + bb * (mx + 1);
             e.in1 = qidx(P, 1, a + 1); e.g1 = qidx(P, 0, a + 1); e.in2 = qidx(P, 1, bb + 1); e.g2 = qidx(P, 0, bb + 1);
             e.ring = P.qcoff + 0 + (j0 + 1) * ldc; e.qcin = P.qcoff + 1 + (j0 + 1) * ldc;
-            e.cf = C.DeW[pdew[0 + (size_t)j0 * (cmx + 1)]];
+            e.cfidx = pdew[0 + (size_t)j0 * (cmx + 1)];
+            e.cf = C.DeW[e.cfidx];
             break;
           case 1:
             e.d1 = P.xoff + mx + a * (mx + 1); e.d2 = P.xoff + mx + bb * (mx + 1);
             e.in1 = qidx(P, mx, a + 1); e.g1 = qidx(P, mx + 1, a + 1); e.in2 = qidx(P, mx, bb + 1); e.g2 = qidx(P, mx + 1, bb + 1);
             e.ring = P.qcoff + (cmx + 1) + (j0 + 1) * ldc; e.qcin = P.qcoff + cmx + (j0 + 1) * ldc;
-            e.cf = C.DeW[pdew[cmx + (size_t)j0 * (cmx + 1)]];
+            e.cfidx = pdew[cmx + (size_t)j0 * (cmx + 1)];
+            e.cf = C.DeW[e.cfidx];
             break;
           case 2:
             e.d1 = P.yoff + a + 0 * mx; e.d2 = P.yoff + bb + 0 * mx;
             e.in1 = qidx(P, a + 1, 1); e.g1 = qidx(P, a + 1, 0); e.in2 = qidx(P, bb + 1, 1); e.g2 = qidx(P, bb + 1, 0);
             e.ring = P.qcoff + (j0 + 1) + 0 * ldc; e.qcin = P.qcoff + (j0 + 1) + 1 * ldc;
-            e.cf = C.DnS[pdns[j0 + (size_t)0 * cmx]];
+            e.cfidx = pdns[j0 + (size_t)0 * cmx];
+            e.cf = C.DnS[e.cfidx];
             break;
           default:
             e.d1 = P.yoff + a + my * mx; e.d2 = P.yoff + bb + my * mx;
             e.in1 = qidx(P, a + 1, my); e.g1 = qidx(P, a + 1, my + 1); e.in2 = qidx(P, bb + 1, my); e.g2 = qidx(P, bb + 1, my + 1);
             e.ring = P.qcoff + (j0 + 1) + (cmy + 1) * ldc; e.qcin = P.qcoff + (j0 + 1) + cmy * ldc;
-            e.cf = C.DnS[pdns[j0 + (size_t)cmy * cmx]];
+            e.cfidx = pdns[j0 + (size_t)cmy * cmx];
+            e.cf = C.DnS[e.cfidx];
             break;
         }
         F.corr.push_back(e);
@@ -1545,13 +1552,139 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
   return sync(h);
 }
 
+// ------------------------------------------------------------------ coefficients
+// ebpSetTime.m:19-51: the caller re-evaluated CC/DeW/DnS/g at the new time
+// (with the implicit shift) and ran the coefficient fill-down; everything the
+// library derived from those values is refreshed here: the physical ghost
+// data of the ghost records, the interface diffusivities of the reflux
+// records (ebGatherParentCoeffs.m:47-61) and the coarsest factorisation.
+int bsam_update_coeffs(bsam_hier* h, const bsam_hier_desc* d) {
+  if (!h) return fail(BSAM_E_ARG, "bsamgpu:arg: null hierarchy");
+  if (!d || d->nlev != h->nlev || d->root_idx != h->root || !d->n || !d->np || !d->boxes || !d->bc_code ||
+      !d->DeW || !d->DnS || !d->CC || !d->InvDiagS || !d->g)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: coefficient update needs the hierarchy's own descriptor");
+  size_t pb = 0, oc = 0, ox = 0, oy = 0, og = 0;
+  for (int k = 0; k < h->nlev; ++k) {
+    const HLevel& L = h->H[k];
+    if (d->n[2 * k] != L.n[0] || d->n[2 * k + 1] != L.n[1] || d->np[k] != L.np)
+      return fail(BSAM_E_ARG, "bsamgpu:arg: coefficient update changes level %d's grid", k + 1);
+    for (int p = 0; p < L.np; ++p, ++pb) {
+      const Box& b = L.box[p];
+      if (d->boxes[4 * pb] != b.ilo || d->boxes[4 * pb + 1] != b.ihi || d->boxes[4 * pb + 2] != b.jlo ||
+          d->boxes[4 * pb + 3] != b.jhi)
+        return fail(BSAM_E_ARG, "bsamgpu:arg: coefficient update changes a patch box of level %d", k + 1);
+      for (int sd = 0; sd < 4; ++sd)
+        if (d->bc_code[4 * pb + sd] != L.pi[p].bc[sd])
+          return fail(BSAM_E_ARG, "bsamgpu:arg: coefficient update changes a boundary type of level %d", k + 1);
+    }
+    for (size_t i = 0; i < (size_t)L.nx; ++i) {
+      const double v = d->DeW[ox + i];
+      if (!std::isfinite(v)) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: coefficient function returned non-finite values.");
+      if (v <= 0) return fail(BSAM_E_D, "eb_make_level:D: D(x,y) must be strictly positive at faces.");
+    }
+    for (size_t i = 0; i < (size_t)L.ny; ++i) {
+      const double v = d->DnS[oy + i];
+      if (!std::isfinite(v)) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: coefficient function returned non-finite values.");
+      if (v <= 0) return fail(BSAM_E_D, "eb_make_level:D: D(x,y) must be strictly positive at faces.");
+    }
+    for (size_t i = 0; i < (size_t)L.ncell; ++i)
+      if (!std::isfinite(d->CC[oc + i]) || !std::isfinite(d->InvDiagS[oc + i]))
+        return fail(BSAM_E_NONFINITE, "eb_make_level:eval: coefficient function returned non-finite values.");
+    for (size_t i = 0; i < (size_t)L.ng; ++i)
+      if (!std::isfinite(d->g[og + i]))
+        return fail(BSAM_E_NONFINITE, "eb_make_level:eval: coefficient function returned non-finite values.");
+    oc += L.ncell; ox += L.nx; oy += L.ny; og += L.ng;
+  }
+  // accepted: replace the host copies and re-derive the dependent records
+  oc = ox = oy = og = 0;
+  for (int k = 0; k < h->nlev; ++k) {
+    HLevel& L = h->H[k];
+    L.CC.assign(d->CC + oc, d->CC + oc + L.ncell);
+    L.ID.assign(d->InvDiagS + oc, d->InvDiagS + oc + L.ncell);
+    L.DeW.assign(d->DeW + ox, d->DeW + ox + L.nx);
+    L.DnS.assign(d->DnS + oy, d->DnS + oy + L.ny);
+    L.G.assign(d->g + og, d->g + og + L.ng);
+    oc += L.ncell; ox += L.nx; oy += L.ny; og += L.ng;
+    for (const PatchInfo& P : L.pi) {  // physical sides: W(my) E(my) S(mx) N(mx) from gbase
+      int first = P.gbase;
+      for (int side = 0; side < 4; ++side) {
+        const int len = side < 2 ? P.my : P.mx;
+        if (P.bc[side] != 0)
+          for (int t = 0; t < len; ++t) L.ghost[first + t] = phys_rec(L, P, side, t);
+        first += len;
+      }
+    }
+  }
+  for (int k = 1; k < h->nlev; ++k) {
+    const HLevel& C = h->H[k - 1];
+    for (CorrRec& e : h->H[k].corr) e.cf = e.face < 2 ? C.DeW[e.cfidx] : C.DnS[e.cfidx];
+  }
+  bool allneu = true;
+  for (int sd = 0; sd < 4; ++sd) allneu = allneu && h->H[0].pi[0].bc[sd] != 1;
+  double cmax = 0;
+  for (auto& L : h->H)
+    for (double v : L.CC) cmax = std::max(cmax, std::fabs(v));
+  const bool singular = allneu && cmax == 0;
+  std::vector<double> Ainv, bcRhs;
+  int M = 0;
+  if (int rc = coarsest_assemble(h->H[0], singular, Ainv, bcRhs, M)) return rc;
+  h->singular = singular;
+  if (h->ctx->device < 0) {
+    h->cst.M = M;
+    return 0;
+  }
+  HIPCHK(hipSetDevice(h->ctx->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  // captured cycles hold the coarsest order and buffers by value
+  if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+  if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+  h->graph_key = -1;
+  if (M != h->cst.M) {
+    if (int rc = upload(h, &h->cst.Ainv, Ainv)) return rc;
+    h->cst.M = M;
+  } else {
+    HIPCHK(hipMemcpy(h->cst.Ainv, Ainv.data(), Ainv.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(h->cst.bcRhs, bcRhs.data(), bcRhs.size() * sizeof(double), hipMemcpyHostToDevice));
+  for (int k = 0; k < h->nlev; ++k) {
+    const HLevel& L = h->H[k];
+    LevelDev<double>& D = h->d64[k];
+    auto put = [&](double* dst, const std::vector<double>& src) -> int {
+      if (!src.empty()) HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
+      return 0;
+    };
+    int rc;
+    if ((rc = put(D.CC, L.CC)) || (rc = put(D.ID, L.ID)) || (rc = put(D.DeW, L.DeW)) || (rc = put(D.DnS, L.DnS)) ||
+        (rc = put(D.G, L.G)))
+      return rc;
+    if (!L.ghost.empty())
+      HIPCHK(hipMemcpy(D.ghost, L.ghost.data(), L.ghost.size() * sizeof(GhostRec), hipMemcpyHostToDevice));
+    if (!L.corr.empty())
+      HIPCHK(hipMemcpy(D.corr, L.corr.data(), L.corr.size() * sizeof(CorrRec), hipMemcpyHostToDevice));
+    if (h->have32) {  // the FP32 levels share the tables; their value arrays are cast copies
+      LevelDev<float>& E = h->d32[k];
+      auto put32 = [&](float* dst, const std::vector<double>& src) -> int {
+        if (src.empty()) return 0;
+        std::vector<float> tmp(src.begin(), src.end());
+        HIPCHK(hipMemcpy(dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice));
+        return 0;
+      };
+      if ((rc = put32(E.CC, L.CC)) || (rc = put32(E.ID, L.ID)) || (rc = put32(E.DeW, L.DeW)) ||
+          (rc = put32(E.DnS, L.DnS)) || (rc = put32(E.G, L.G)))
+        return rc;
+    }
+  }
+  return 0;
+}
+
 // ------------------------------------------------------------------ parabolic
+// Time-stepping state of ebpSolve.m (H.par.Uo, H.par.Uoo) and the source
+// f(t1) live on the device, packed like FTrue over all levels.
 static int ensure_history(bsam_hier* h) {
   if (h->Uo) return 0;
   int rc;
-  if ((rc = h->mem.alloc(&h->Uo, h->ncell_all))) return rc;
-  if ((rc = h->mem.alloc(&h->Uoo, h->ncell_all))) return rc;
-  if ((rc = h->mem.alloc(&h->fbuf, h->ncell_all))) return rc;
+  if ((rc = h->mem.alloc(&h->Uo, std::max(1LL, h->ncell_all)))) return rc;
+  if ((rc = h->mem.alloc(&h->Uoo, std::max(1LL, h->ncell_all)))) return rc;
   return 0;
 }
 
@@ -1560,58 +1693,89 @@ int bsam_parabolic_init(bsam_hier* h) {
   if (int rc = ensure_history(h)) return rc;
   for (int k = 0; k < h->nlev; ++k)
     launch_history(h->d64[k], h->Uo + h->cell_base[k], h->Uoo + h->cell_base[k], 0, h->stream);
+  h->have_uoo = false;
+  return sync(h);
+}
+
+int bsam_parabolic_set_history(bsam_hier* h, const double* Uo, const double* Uoo) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (int rc = ensure_history(h)) return rc;
+  const size_t nb = (size_t)h->ncell_all * sizeof(double);
+  if (Uo) HIPCHK(hipMemcpyAsync(h->Uo, Uo, nb, hipMemcpyHostToDevice, h->stream));
+  if (Uoo) {
+    HIPCHK(hipMemcpyAsync(h->Uoo, Uoo, nb, hipMemcpyHostToDevice, h->stream));
+    h->have_uoo = true;
+  }
+  return sync(h);
+}
+
+int bsam_parabolic_get_history(bsam_hier* h, int which, double* out) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (!h->Uo) return fail(BSAM_E_ARG, "bsamgpu:arg: history not initialised");
+  if ((which != 0 && which != 1) || !out) return fail(BSAM_E_ARG, "bsamgpu:arg: bad history request");
+  HIPCHK(hipMemcpyAsync(out, which ? h->Uoo : h->Uo, (size_t)h->ncell_all * sizeof(double), hipMemcpyDeviceToHost,
+                        h->stream));
+  return sync(h);
+}
+
+int bsam_parabolic_set_source(bsam_hier* h, const double* f) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (!f) {
+    h->have_src = false;
+    return 0;
+  }
+  for (long long i = 0; i < h->ncell_all; ++i)
+    if (!std::isfinite(f[i])) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: coefficient function returned non-finite values.");
+  if (!h->src)
+    if (int rc = h->mem.alloc(&h->src, std::max(1LL, h->ncell_all))) return rc;
+  HIPCHK(hipMemcpyAsync(h->src, f, (size_t)h->ncell_all * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  h->have_src = true;
+  return sync(h);
+}
+
+static int enqueue_rhs(bsam_hier* h, double dt, int scheme) {
+  if (!h->Uo) return fail(BSAM_E_ARG, "bsamgpu:arg: history not initialised");
+  if (!(dt > 0) || !std::isfinite(dt)) return fail(BSAM_E_ARG, "ebp_options:Dt: Dt must be finite and positive");
+  if (scheme != BSAM_BE && scheme != BSAM_BDF2)
+    return fail(BSAM_E_ARG, "ebp_rhs:scheme: scheme %d is not available on the device (be, bdf2)", scheme);
+  if (scheme == BSAM_BDF2 && !h->have_uoo)
+    return fail(BSAM_E_ARG, "ebp_rhs:history: bdf2 needs u^{n-1} (take a be step first)");
+  // sub-root levels keep their FTrue: the FAS load overwrites them (ebpRhs.m:25-27)
+  for (int k = h->root; k <= h->nlev; ++k) {
+    const long long b = h->cell_base[k - 1];
+    launch_bdf2_rhs(h->d64[k - 1].FT, h->have_src ? h->src + b : nullptr, h->Uo + b, h->Uoo + b, dt,
+                    scheme == BSAM_BE, h->H[k - 1].ncell, h->stream);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int bsam_parabolic_rhs(bsam_hier* h, double dt, int scheme) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (int rc = enqueue_rhs(h, dt, scheme)) return rc;
   return sync(h);
 }
 
 int bsam_parabolic_advance(bsam_hier* h) {
-  if (!h || !h->Uo) return fail(BSAM_E_ARG, "bsamgpu:arg: history not initialised");
+  if (int rc_ = need_dev(h)) return rc_;
+  if (!h->Uo) return fail(BSAM_E_ARG, "bsamgpu:arg: history not initialised");
   for (int k = 0; k < h->nlev; ++k)
     launch_history(h->d64[k], h->Uo + h->cell_base[k], h->Uoo + h->cell_base[k], 1, h->stream);
+  h->have_uoo = true;
   return sync(h);
 }
 
-int bsam_parabolic_rhs(bsam_hier* h, const double* f, double dt, int first_step) {
-  if (!h || !h->Uo) return fail(BSAM_E_ARG, "bsamgpu:arg: history not initialised");
-  if (!(dt > 0)) return fail(BSAM_E_ARG, "ebp_options:Dt: Dt must be positive");
-  const double* fd = nullptr;
-  if (f) {
-    HIPCHK(hipMemcpyAsync(h->fbuf, f, h->ncell_all * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    fd = h->fbuf;
-  }
-  // sub-root levels keep their FTrue (ebpRhs.m:138-140)
-  for (int k = h->root; k <= h->nlev; ++k) {
-    const long long b = h->cell_base[k - 1];
-    launch_bdf2_rhs(h->d64[k - 1].FT, fd ? fd + b : nullptr, h->Uo + b, h->Uoo + b, dt, first_step,
-                    h->H[k - 1].ncell, h->stream);
-    if (h->have32) launch_convert<float, double>(h->d32[k - 1].FT, h->d64[k - 1].FT, h->H[k - 1].ncell, h->stream);
-  }
-  return sync(h);
-}
-
-int bsam_set_reaction_shift(bsam_hier* h, double shift) {
+int bsam_parabolic_step(bsam_hier* h, double dt, int scheme, const bsam_solve_opts* o, bsam_stage_stats* out,
+                        double* mass) {
   if (int rc_ = need_dev(h)) return rc_;
-  if (!std::isfinite(shift)) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: non-finite shift");
-  if (int rc = ensure32(h)) return rc;
-  for (int k = 0; k < h->nlev; ++k) {
-    launch_add_scalar(h->d64[k].CC, h->H[k].ncell, shift, h->d32[k].CC, h->stream);
-    launch_invdiag(h->d64[k], h->d32[k].ID, h->stream);
-    for (double& v : h->H[k].CC) v += shift;
+  if (int rc = enqueue_rhs(h, dt, scheme)) return rc;
+  if (int rc = bsam_solve_cycles(h, o, out)) return rc;
+  if (int rc = bsam_parabolic_advance(h)) return rc;
+  if (mass) {
+    double m = 0, a = 0;
+    if (int rc = bsam_composite_mean(h, &m, &a)) return rc;
+    *mass = m * a;  // compMass: mbar * vol (ebpSolve.m:103-106)
   }
-  if (int rc = sync(h)) return rc;
-  // refactor the coarsest matrix with the shifted reaction
-  bool allneu = true;
-  for (int s = 0; s < 4; ++s) allneu = allneu && h->H[0].pi[0].bc[s] != 1;
-  double cmax = 0;
-  for (auto& L : h->H)
-    for (double v : L.CC) cmax = std::max(cmax, std::fabs(v));
-  const bool singular = allneu && cmax == 0;
-  std::vector<double> Ainv, bcRhs;
-  int M = 0;
-  if (int rc = coarsest_assemble(h->H[0], singular, Ainv, bcRhs, M)) return rc;
-  if (M != h->cst.M) return fail(BSAM_E_ARG, "bsamgpu:arg: reaction shift changes the coarsest system's order");
-  h->singular = singular;
-  HIPCHK(hipMemcpy(h->cst.Ainv, Ainv.data(), Ainv.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(h->cst.bcRhs, bcRhs.data(), bcRhs.size() * sizeof(double), hipMemcpyHostToDevice));
   return 0;
 }
 

@@ -189,18 +189,42 @@ int bsam_set_level(bsam_hier *h, int k, int field, const double *in);
 long long bsam_export_size(const bsam_hier *h, int k, int what);
 int bsam_export(const bsam_hier *h, int k, int what, long long *out);
 
-/* BDF2 right-hand side on levels >= root (ParabolicBSAM/src/ebpRhs.m:36,151):
- * FTrue = f + (2 Uo - Uoo/2)/dt with Uo/Uoo the device-resident history; with
- * first_step != 0 the BE bootstrap FTrue = f + Uo/dt.  f is packed like FTrue
- * (NULL means f = 0). */
-int bsam_parabolic_rhs(bsam_hier *h, const double *f, double dt, int first_step);
-/* Rotate the history after an accepted step: Uoo = Uo, Uo = interior(Q) (ebpSolve.m:56-57) */
-int bsam_parabolic_advance(bsam_hier *h);
-/* Initialise the history from the current Q (Uo = Uoo = interior(Q)) */
+/* == the refresh of ebpSetTime (ParabolicBSAM/src/ebpSetTime.m:19-51): replace
+ * CC, DeW, DnS, InvDiagS and g on every level and re-derive what depends on
+ * them (physical ghost data, the reflux interface diffusivities of
+ * ebGatherParentCoeffs.m:47-61, the coarsest factorisation of ebCoarsest.m:14-77).
+ * The caller evaluated the coefficients at the new time with the implicit
+ * shift C + alpha/dt and ran the coefficient fill-down (ebCoeffFilldown.m:1-62).
+ * desc must repeat the hierarchy's n, np, boxes and bc codes; its Q and FTrue
+ * are not read.  Nothing is changed when an argument is rejected. */
+int bsam_update_coeffs(bsam_hier *h, const bsam_hier_desc *desc);
+
+/* Parabolic stepping (ParabolicBSAM/src/ebpSolve.m:23-78).  The history
+ * H.par.Uo = u^n, H.par.Uoo = u^{n-1} and the source f(t1) are device-resident
+ * arrays packed like FTrue over all levels 1..nlev. */
+enum bsam_scheme { BSAM_BE = 0, BSAM_BDF2 = 1 };
+/* Uo = interior(Q), no u^{n-1} yet (ebpInit.m:32-35) */
 int bsam_parabolic_init(bsam_hier *h);
-/* Reaction shift C_eff = C + shift on every level, refreshing InvDiagS on the
- * device (ParabolicBSAM/src/ebpSetTime.m:11-55 for time-independent D, C). */
-int bsam_set_reaction_shift(bsam_hier *h, double shift);
+/* Upload Uo and/or Uoo (NULL keeps the resident array); after a regrid
+ * (ebpRegrid.m:80-91) */
+int bsam_parabolic_set_history(bsam_hier *h, const double *Uo, const double *Uoo);
+/* which = 0: Uo, 1: Uoo */
+int bsam_parabolic_get_history(bsam_hier *h, int which, double *out);
+/* f(x,y,t1) at cell centres, packed like FTrue (sub-root entries unused);
+ * kept resident until replaced; NULL means f = 0 (ebpRhs.m:31-36) */
+int bsam_parabolic_set_source(bsam_hier *h, const double *f);
+/* FTrue on levels >= root (ParabolicBSAM/src/ebpRhs.m:25-48):
+ *   BSAM_BE:   f + Uo/dt
+ *   BSAM_BDF2: f + (2 Uo - Uoo/2)/dt */
+int bsam_parabolic_rhs(bsam_hier *h, double dt, int scheme);
+/* Uoo = Uo, Uo = interior(Q) after an accepted step (ebpSolve.m:55-57) */
+int bsam_parabolic_advance(bsam_hier *h);
+/* One accepted step of ebpSolve.m:40-58 without host traffic beyond the
+ * per-cycle residual: rhs, ebSolveCycles (opts as in bsam_solve_cycles),
+ * history rotation and compMass = composite mean * area (ebpSolve.m:103-106;
+ * mass may be NULL). */
+int bsam_parabolic_step(bsam_hier *h, double dt, int scheme, const bsam_solve_opts *opts,
+                        bsam_stage_stats *out, double *mass);
 
 #ifdef __cplusplus
 }

@@ -1004,3 +1004,267 @@ def eb_solve(prob, opts, host, verbose=0):
             O.solve_cycles(verbose=verbose)
     sync_q()
     return H, O
+
+
+# --------------------------------------------------------------------------
+# Parabolic stepping (ParabolicBSAM/src): every step is one ebSolveCycles of
+# the oracle with C_eff = C + alpha/dt.  Host decisions (ladder, tagging,
+# clustering, level construction, coefficient fill-down) come from the host
+# module passed in, exactly as for eb_solve above.
+def ebp_rhs(f, Uo, Uoo, dt, scheme):
+    """ebpRhs.m:31-36 (be / bdf2 branches)."""
+    if scheme == "be":
+        return f + Uo / dt
+    if scheme == "bdf2":
+        return f + (2 * Uo - 0.5 * Uoo) / dt
+    raise ValueError(f"ebp_rhs:scheme: unknown scheme {scheme}")
+
+
+def conslin(W):
+    """ebpRegrid.m:95-108: per coarse cell, value and centred slopes."""
+    C0 = W[1:-1, 1:-1]
+    sx = (W[2:, 1:-1] - W[:-2, 1:-1]) / 2
+    sy = (W[1:-1, 2:] - W[1:-1, :-2]) / 2
+    F = np.zeros((2 * C0.shape[0], 2 * C0.shape[1]))
+    F[0::2, 0::2] = C0 - sx / 4 - sy / 4
+    F[1::2, 0::2] = C0 + sx / 4 - sy / 4
+    F[0::2, 1::2] = C0 - sx / 4 + sy / 4
+    F[1::2, 1::2] = C0 + sx / 4 + sy / 4
+    return F
+
+
+def conslin_interior(P):
+    """ebpRegrid.m:111-134: one-sided slopes on the edges."""
+    cm, cn = P.shape
+    sx = np.zeros((cm, cn))
+    sy = np.zeros((cm, cn))
+    if cm > 1:
+        sx[1:-1, :] = (P[2:, :] - P[:-2, :]) / 2
+        sx[0, :] = P[1, :] - P[0, :]
+        sx[-1, :] = P[-1, :] - P[-2, :]
+    if cn > 1:
+        sy[:, 1:-1] = (P[:, 2:] - P[:, :-2]) / 2
+        sy[:, 0] = P[:, 1] - P[:, 0]
+        sy[:, -1] = P[:, -1] - P[:, -2]
+    F = np.zeros((2 * cm, 2 * cn))
+    F[0::2, 0::2] = P - sx / 4 - sy / 4
+    F[1::2, 0::2] = P + sx / 4 - sy / 4
+    F[0::2, 1::2] = P - sx / 4 + sy / 4
+    F[1::2, 1::2] = P + sx / 4 + sy / 4
+    return F
+
+
+class ParabolicOracle:
+    """ebpSolve.m / ebpInit.m / ebpSetTime.m / ebpRegrid.m on the oracle.
+
+    ``host`` supplies setup/add_level/tag_cluster/coeff_filldown/ndgrid,
+    ``par`` the problem freezing (eprob) of the host layer."""
+
+    def __init__(self, pp, opts, host, par):
+        self.pp, self.opts, self.host, self.par = pp, opts, host, par
+
+    # ---- helpers on (H, O) pairs
+    @staticmethod
+    def _sync(H, O):
+        for k in range(1, O.nlev + 1):
+            for p in range(O.L(k).np):
+                H.L(k).Q[p] = O.L(k).Q[p].copy()
+
+    def _add(self, H, O, boxes):
+        """ebAddLevel.m:6-33 (the new level's prolonged initial guess included)."""
+        self._sync(H, O)
+        kNew = self.host.add_level(H, boxes)
+        On = OracleHierarchy(H)
+        for k in range(2, O.nlev + 1):
+            On.L(k).QC = [a.copy() for a in O.L(k).QC]
+        On.stats = O.stats
+        On.init_level_prolong(kNew)
+        return kNew, On
+
+    def _u0(self, O, k):
+        lev = O.L(k)
+        dom = self.pp.domain
+        for p in range(lev.np):
+            b = lev.box[p]
+            xc = dom[0] + (np.arange(b[0], b[1] + 1, dtype=float) - 0.5) * lev.h[0]
+            yc = dom[2] + (np.arange(b[2], b[3] + 1, dtype=float) - 0.5) * lev.h[1]
+            X, Y = np.meshgrid(xc, yc, indexing="ij")
+            v = self.pp.u0(X, Y)
+            lev.Q[p][1:-1, 1:-1] = v if np.ndim(v) else float(v) * np.ones(X.shape)
+
+    @staticmethod
+    def interiors(O):
+        return [[O.L(k).Q[p][1:-1, 1:-1].copy() for p in range(O.L(k).np)] for k in range(1, O.nlev + 1)]
+
+    # ---- ebpInit.m:10-35
+    def init(self, aDt1):
+        host, opts = self.host, self.opts
+        H = host.setup(self.par.eprob(self.pp, 0.0, aDt1), opts.eb)
+        O = OracleHierarchy(H)
+        for k in range(1, O.nlev + 1):
+            self._u0(O, k)
+        for k in range(1, O.nlev + 1):
+            O.fill_ghosts(k)
+        self._sync(H, O)
+        while H.nlev - H.rootIdx + 1 < opts.eb.MaxLevels:
+            boxes = host.tag_cluster(H)
+            if not boxes:
+                break
+            kNew, O = self._add(H, O, boxes)
+            self._u0(O, kNew)
+            O.fill_ghosts(kNew)
+            self._sync(H, O)
+        O.transfer(0, "filldown")
+        self._sync(H, O)
+        self.H, self.O = H, O
+        self.Uo, self.Uoo = self.interiors(O), None
+        td = self.pp.tdD or self.pp.tdC or self.pp.tdBC
+        self.stateKey = (0.0 * td, aDt1)
+
+    # ---- ebpSetTime.m:14-55
+    def set_time(self, t, aDt):
+        pp = self.pp
+        key = (t * (pp.tdD or pp.tdC or pp.tdBC), aDt)
+        if key == self.stateKey:
+            return
+        H, O = self.H, self.O
+        dom = pp.domain
+        for k in range(1, H.nlev + 1):
+            lev = H.L(k)
+            for p in range(lev.np):
+                b = lev.box[p]
+                xc = dom[0] + (np.arange(b[0], b[1] + 1, dtype=float) - 0.5) * lev.h[0]
+                yc = dom[2] + (np.arange(b[2], b[3] + 1, dtype=float) - 0.5) * lev.h[1]
+                if pp.tdD:
+                    xf = dom[0] + np.arange(b[0] - 1, b[1] + 1, dtype=float) * lev.h[0]
+                    yf = dom[2] + np.arange(b[2] - 1, b[3] + 1, dtype=float) * lev.h[1]
+                    XF, YC = np.meshgrid(xf, yc, indexing="ij")
+                    lev.DeW[p] = np.broadcast_to(pp.D(XF, YC, t), XF.shape).astype(float)
+                    XC, YF = np.meshgrid(xc, yf, indexing="ij")
+                    lev.DnS[p] = np.broadcast_to(pp.D(XC, YF, t), XC.shape).astype(float)
+                X, Y = np.meshgrid(xc, yc, indexing="ij")
+                lev.CC[p] = np.broadcast_to(pp.C(X, Y, t), X.shape).astype(float) + aDt
+                if pp.tdBC:
+                    fint = lev.faceInternal[p]
+                    if not fint[0]:
+                        lev.bc_g[p][0] = np.ravel(np.broadcast_to(pp.bc[0].g(dom[0] * np.ones(yc.shape), yc, t), yc.shape)).astype(float)
+                    if not fint[1]:
+                        lev.bc_g[p][1] = np.ravel(np.broadcast_to(pp.bc[1].g(dom[1] * np.ones(yc.shape), yc, t), yc.shape)).astype(float)
+                    if not fint[2]:
+                        lev.bc_g[p][2] = np.ravel(np.broadcast_to(pp.bc[2].g(xc, dom[2] * np.ones(xc.shape), t), xc.shape)).astype(float)
+                    if not fint[3]:
+                        lev.bc_g[p][3] = np.ravel(np.broadcast_to(pp.bc[3].g(xc, dom[3] * np.ones(xc.shape), t), xc.shape)).astype(float)
+        self.host.coeff_filldown(H)
+        for k in range(1, H.nlev + 1):
+            src, dst = H.L(k), O.L(k)
+            dst.CC = [a.copy() for a in src.CC]
+            dst.DeW = [a.copy() for a in src.DeW]
+            dst.DnS = [a.copy() for a in src.DnS]
+            dst.InvDiagS = [a.copy() for a in src.InvDiagS]
+            dst.bc_g = [[g.copy() for g in gs] for gs in src.bc_g]
+        O.gather_parent_coeffs()
+        O.coarsest_assemble()
+        self.stateKey = key
+
+    # ---- one step of ebpSolve.m:40-58
+    def step(self, t1, dt, scheme):
+        O = self.O
+        dom = self.pp.domain
+        for k in range(O.rootIdx, O.nlev + 1):
+            lev = O.L(k)
+            for p in range(lev.np):
+                b = lev.box[p]
+                X, Y = np.meshgrid(dom[0] + (np.arange(b[0], b[1] + 1, dtype=float) - 0.5) * lev.h[0],
+                                   dom[2] + (np.arange(b[2], b[3] + 1, dtype=float) - 0.5) * lev.h[1],
+                                   indexing="ij")
+                f = np.broadcast_to(self.pp.f(X, Y, t1), X.shape).astype(float)
+                Uoo = self.Uoo[k - 1][p] if scheme == "bdf2" else None
+                lev.FTrue[p] = ebp_rhs(f, self.Uo[k - 1][p], Uoo, dt, scheme)
+        self.set_time(t1, ALPHA_OF[scheme] / dt)
+        st = O.solve_cycles()
+        self.Uoo = self.Uo
+        self.Uo = self.interiors(O)
+        m, area, _ = O.composite_mean()
+        return st, m * area
+
+    # ---- ebpRegrid.m:15-59
+    def regrid(self, t, aDtNext):
+        host, opts = self.host, self.opts
+        Hold, Oold = self.H, self.O
+        Hn = host.setup(self.par.eprob(self.pp, t, aDtNext), opts.eb)
+        r = Hn.rootIdx
+        On = OracleHierarchy(Hn)
+        On.L(r).Q[0][1:-1, 1:-1] = Oold.L(r).Q[0][1:-1, 1:-1]
+        UooN = [[] for _ in range(r - 1)] + [[self.Uoo[r - 1][0]]]
+        for k in range(r - 1, 0, -1):
+            On.L(k).Q[0][1:-1, 1:-1] = restrict2(On.L(k + 1).Q[0][1:-1, 1:-1])
+        for k in range(1, r + 1):
+            On.fill_ghosts(k)
+        self._sync(Hn, On)
+        while Hn.nlev - Hn.rootIdx + 1 < opts.eb.MaxLevels:
+            boxes = host.tag_cluster(Hn)
+            if not boxes:
+                break
+            kN, On = self._add(Hn, On, boxes)
+            lev = On.L(kN)
+            row = []
+            for p in range(lev.np):
+                Qf = conslin(On.gather_window(kN, p))
+                b = lev.box[p]
+                Pf = np.zeros(((b[1] - b[0] + 1) // 2, (b[3] - b[2] + 1) // 2))
+                for q, s, d in lev.ops[p]["fsc"]:
+                    Pf[s[0]:s[1] + 1, s[2]:s[3] + 1] = UooN[kN - 2][q][d[0]:d[1] + 1, d[2]:d[3] + 1]
+                Uf = conslin_interior(Pf)
+                if kN <= Oold.nlev:
+                    for q in range(Oold.L(kN).np):
+                        qb = Oold.L(kN).box[q]
+                        rr = [max(b[0], qb[0]), min(b[1], qb[1]), max(b[2], qb[2]), min(b[3], qb[3])]
+                        if rr[0] > rr[1] or rr[2] > rr[3]:
+                            continue
+                        di = slice(rr[0] - b[0], rr[1] - b[0] + 1)
+                        dj = slice(rr[2] - b[2], rr[3] - b[2] + 1)
+                        si = slice(rr[0] - qb[0], rr[1] - qb[0] + 1)
+                        sj = slice(rr[2] - qb[2], rr[3] - qb[2] + 1)
+                        Qf[di, dj] = Oold.L(kN).Q[q][1:-1, 1:-1][si, sj]
+                        Uf[di, dj] = self.Uoo[kN - 1][q][si, sj]
+                lev.Q[p][1:-1, 1:-1] = Qf
+                row.append(Uf)
+            UooN.append(row)
+            On.fill_ghosts(kN)
+            self._sync(Hn, On)
+        On.transfer(0, "filldown")
+        self._sync(Hn, On)
+        On.stats = Oold.stats
+        self.H, self.O = Hn, On
+        self.Uo = self.interiors(On)
+        self.Uoo = UooN
+        td = self.pp.tdD or self.pp.tdC or self.pp.tdBC
+        self.stateKey = (t * td, aDtNext)
+
+    # ---- ebpSolve.m:23-78
+    def solve(self, callback=None):
+        opts = self.opts
+        q = opts.TFinal / opts.Dt
+        nsteps = max(1, int(np.floor(q + 0.5)))
+        dt = opts.TFinal / nsteps
+        scheme1 = "be" if opts.Scheme == "bdf2" else opts.Scheme
+        self.init(ALPHA_OF[scheme1] / dt)
+        m, area, _ = self.O.composite_mean()
+        mass = [m * area]
+        stages = []
+        levels = []
+        for n in range(1, nsteps + 1):
+            t1 = n * dt
+            scheme = scheme1 if n == 1 else opts.Scheme
+            st, mn = self.step(t1, dt, scheme)
+            stages.append(st)
+            mass.append(mn)
+            levels.append((self.O.nlev - self.O.rootIdx + 1, st["totalCells"]))
+            if callback is not None:
+                callback(self, n)
+            if opts.RegridEvery > 0 and n % opts.RegridEvery == 0 and n < nsteps and opts.eb.MaxLevels > 1:
+                self.regrid(t1, ALPHA_OF[opts.Scheme] / dt)
+        return {"stages": stages, "mass": mass, "levels": levels, "dt": dt, "nsteps": nsteps}
+
+
+ALPHA_OF = {"be": 1.0, "bdf2": 1.5, "cn": 2.0}
