@@ -1,6 +1,7 @@
 """Host-layer logic (no GPU): options/problem validation, ladder, tagging
 morphology, Berger-Rigoutsos splitting, level construction errors, and the
 C ABI library's exports and host-only table derivation."""
+import copy
 import ctypes
 import os
 import re
@@ -140,3 +141,31 @@ def test_abi_error_codes_match_reference_identifiers():
     with pytest.raises(BsamError) as ei:
         _host_only(H2)
     assert ei.value.ident in ("eb_build_ops:ring", "eb_build_ops:nesting")
+
+
+def test_update_coeffs_validates_structure_and_values():
+    """bsam_update_coeffs (the ebpSetTime refresh) accepts new coefficient
+    values on the same grid and rejects structural changes and non-positive
+    D with the reference's identifiers, leaving the hierarchy unchanged."""
+    H = CASES["lshape"](seed=None)
+    G = _host_only(H)
+    for k in range(1, H.nlev + 1):
+        for p in range(H.L(k).np):
+            H.L(k).CC[p] = H.L(k).CC[p] + 2.0
+    hierarchy.coeff_filldown(H)
+    G.update_coeffs(H)
+    bad = copy.deepcopy(H)
+    bad.L(3).DnS[0][0, 0] = -1.0
+    with pytest.raises(BsamError) as e:
+        G.update_coeffs(bad)
+    assert "eb_make_level:D" in str(e.value)
+    bad = copy.deepcopy(H)
+    bad.L(2).CC[0][1, 1] = float("nan")
+    with pytest.raises(BsamError) as e:
+        G.update_coeffs(bad)
+    assert "eb_make_level:eval" in str(e.value)
+    bad = copy.deepcopy(H)
+    bad.L(2).bc_code[0] = [2, 2, 2, 2]
+    with pytest.raises(BsamError) as e:
+        G.update_coeffs(bad)
+    assert "boundary type" in str(e.value)
