@@ -42,7 +42,8 @@ EXPORTS = [
     "bsam_get_level", "bsam_set_level", "bsam_export_size", "bsam_export", "bsam_hier_reset",
     "bsam_time_smoother", "bsam_smoother_bytes", "bsam_update_coeffs", "bsam_parabolic_init",
     "bsam_parabolic_set_history", "bsam_parabolic_get_history", "bsam_parabolic_set_source",
-    "bsam_parabolic_rhs", "bsam_parabolic_advance", "bsam_parabolic_step",
+    "bsam_parabolic_rhs", "bsam_parabolic_advance", "bsam_parabolic_step", "bsam_batch_create",
+    "bsam_batch_destroy", "bsam_batch_step",
 ]
 
 BSAM_BE, BSAM_BDF2 = 0, 1
@@ -120,6 +121,10 @@ def load(path=None):
         "bsam_parabolic_advance": (c_int, [vp]),
         "bsam_parabolic_step": (c_int, [vp, c_double, c_int, ctypes.POINTER(SolveOpts),
                                         ctypes.POINTER(StageStats), P_double]),
+        "bsam_batch_create": (c_int, [vp, c_int, ctypes.POINTER(vp)]),
+        "bsam_batch_destroy": (None, [vp]),
+        "bsam_batch_step": (c_int, [vp, ctypes.POINTER(vp), P_int, c_int, ctypes.POINTER(SolveOpts),
+                                    ctypes.POINTER(Residual)]),
         "bsam_hier_reset": (c_int, [vp]),
         "bsam_time_smoother": (c_int, [vp, c_int, c_int, c_int, P_double]),
         "bsam_smoother_bytes": (c_double, [vp, c_int, c_int]),

@@ -93,5 +93,61 @@ def starfish_options(geom, base=64, levels=5, epsilon=0.05, **kw):
     return Options(**o)
 
 
+def served_params(nreq=512, seed=5):
+    """Per-request parameters of the served workload: BaseCells ~ U{32,64,128},
+    MaxLevels ~ U{2..5}, D = 1 + sum_k a_k sin(2 pi m_k x) cos(2 pi n_k y) with
+    sum |a_k| <= 0.5, C ~ U(0,2), f = sum_j c_j sin(pi p_j x) sin(pi q_j y) with
+    c_j ~ N(0,1), each side Dirichlet 0 or Neumann 0 with probability 1/2."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nreq):
+        out.append({
+            "id": i,
+            "base": int(rng.choice([32, 64, 128])),
+            "levels": int(rng.integers(2, 6)),
+            "a": rng.uniform(-0.125, 0.125, 4).tolist(),
+            "m": rng.integers(1, 5, 4).tolist(),
+            "n": rng.integers(1, 5, 4).tolist(),
+            "C": float(rng.uniform(0.0, 2.0)),
+            "c": rng.standard_normal(8).tolist(),
+            "p": rng.integers(1, 9, 8).tolist(),
+            "q": rng.integers(1, 9, 8).tolist(),
+            "bc": ["dirichlet" if b else "neumann" for b in rng.integers(0, 2, 4)],
+        })
+    return out
+
+
+def served_arrivals(nreq, rate, seed=55):
+    """Poisson arrival times (s) at `rate` requests per second."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, nreq)).tolist()
+
+
+def served_problem(par):
+    a, m, n = np.array(par["a"]), np.array(par["m"]), np.array(par["n"])
+    c, p, q = np.array(par["c"]), np.array(par["p"]), np.array(par["q"])
+
+    def D(x, y):
+        s = 0.0
+        for k in range(4):
+            s = s + a[k] * np.sin(2 * np.pi * m[k] * x) * np.cos(2 * np.pi * n[k] * y)
+        return 1 + s
+
+    def f(x, y):
+        s = 0.0 * x
+        for j in range(8):
+            s = s + c[j] * np.sin(np.pi * p[j] * x) * np.sin(np.pi * q[j] * y)
+        return s
+
+    bc = {side: (t, 0) for side, t in zip(("west", "east", "south", "north"), par["bc"])}
+    return Problem([0, 1, 0, 1], D, par["C"], f, bc)
+
+
+def served_options(par, **kw):
+    base = dict(BaseCells=par["base"], MaxLevels=par["levels"], TagThreshold=1e-3, RelTol=1e-8, Verbose=0)
+    base.update(kw)
+    return Options(**base)
+
+
 def heat_initial():
     return lambda x, y: 1 + 0.2 * np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y)

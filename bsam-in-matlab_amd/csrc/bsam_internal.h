@@ -123,4 +123,15 @@ struct ResidualOut {
   long long ncell, pad2[3];
 };
 
+// one hierarchy of a batched launch: every workgroup runs one item's whole
+// preamble (phase 0: fill + baseline residual) or cycle (phase 1)
+struct BatchItem {
+  const LevelDev<double>* LV64;  // device array of the FP64 level descriptors
+  const LevelDev<float>* LV32;   // FP32 levels (mixed precision) or null
+  float* const* rout32;          // per-level FP32 defect arrays (mixed) or null
+  ResidualOut* out;              // residual scalars of this item
+  CoarsestDev cst;
+  int nlev, root, phase, pad;
+};
+
 }  // namespace bsam

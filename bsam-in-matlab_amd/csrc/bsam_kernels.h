@@ -38,5 +38,9 @@ template <typename TD, typename TS> void launch_convert(TD* d, const TS* src, lo
 void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double* Uoo, double dt, int first,
                      long long n, hipStream_t s);
 void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rotate, hipStream_t s);
+// one workgroup (1024 threads) per item: the item's preamble or whole cycle;
+// max_m: largest coarsest order among the items (LDS right side)
+void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre, int npost, double omega,
+                        int mass_corr, int mixed, int inner, hipStream_t s);
 
 }  // namespace bsam

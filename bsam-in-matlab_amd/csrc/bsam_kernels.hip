@@ -555,11 +555,11 @@ __device__ void blk_fill(const LevelDev<T>& L, const T* Pq, int homog) {
   __syncthreads();
 }
 
+// ebVcycle.m:15-35 for levels kc..1 inside one workgroup; rhs is LDS scratch
+// of at least C.M doubles for the coarsest solve
 template <typename T>
-__global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __restrict__ LV, int kc,
-                                                       CoarsestDev C, int npre, int npost, T omega,
-                                                       int homog, int mass_corr) {
-  extern __shared__ double rhs[];
+__device__ void blk_vcycle(const LevelDev<T>* __restrict__ LV, int kc, const CoarsestDev& C, int npre,
+                           int npost, T omega, int homog, int mass_corr, double* rhs) {
   for (int k = kc; k >= 2; --k) {
     const LevelDev<T> L = LV[k - 1];
     const LevelDev<T> P = LV[k - 2];
@@ -592,6 +592,14 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __rest
       blk_fill<T>(L, Pq, homog);
     }
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __restrict__ LV, int kc,
+                                                       CoarsestDev C, int npre, int npost, T omega,
+                                                       int homog, int mass_corr) {
+  extern __shared__ double rhs[];
+  blk_vcycle<T>(LV, kc, C, npre, npost, omega, homog, mass_corr, rhs);
 }
 
 // ---------------------------------------------------------------- reductions
@@ -656,6 +664,55 @@ __global__ void __launch_bounds__(kTile) k_qc_refresh_all(const LevelDev<T>* __r
 // r = FTrue - L(Q) + reflux chunks, norms over visible cells; optionally
 // writes r (the defect-correction right side) and resets F = FTrue on the
 // tile's cells.  The tile table is (level, tile) pairs.
+// one cell of the composite residual: r = FTrue - L(Q) + reflux chunks
+// (ebCompResidual.m:302-332).  Accumulates max|r|, sum r^2 da, max|FTrue| and
+// the visible count; optionally writes r to rout[level] and resets F = FTrue.
+template <typename T, typename TO>
+__device__ __forceinline__ void residual_item(const LevelDev<T>* __restrict__ LV, int nlev, int lvl, int tile,
+                                              int lane, int mass_corr, TO* const* __restrict__ rout, int reset_f,
+                                              int k0, double& a, double& b, double& f, long long& n) {
+  const LevelDev<T>& L = LV[lvl];
+  const TileRec& tr = L.tiles[tile];
+  const PatchInfo& pi = tr.pi;
+  const int mx = pi.mx, my = pi.my;
+  const int c = tr.start + lane;
+  if (c >= mx * my) return;
+  const int cc = pi.coff + c;
+  if (reset_f) L.F[cc] = L.FT[cc];
+  if (lvl + 1 < k0) return;
+  const int j = c / mx;
+  const int i = c - j * mx;
+  const int ldq = mx + 2;
+  const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
+  const int fx = pi.xoff + i + j * (mx + 1);
+  const int fy = pi.yoff + i + j * mx;
+  const T hx = (T)L.h[0], hy = (T)L.h[1];
+  T r = L.FT[cc] - op_at<T>(L.Q, qc, ldq, L.DeW[fx], L.DeW[fx + 1], L.DnS[fy], L.DnS[fy + mx], L.CC[cc],
+                            hx * hx, hy * hy);
+  double rd = (double)r;
+  if (mass_corr && lvl + 1 < nlev) {
+    const int row = L.tgt_of_cell[cc];
+    if (row >= 0) {
+      const LevelDev<T>& Fn = LV[lvl + 1];
+      const double hcx2 = L.h[0] * L.h[0], hcy2 = L.h[1] * L.h[1];
+      for (int e = L.tgt_ptr[row]; e < L.tgt_ptr[row + 1]; ++e) {
+        const CorrRec cr = Fn.corr[L.tgt_ent[e]];
+        rd = rd + corr_value<T>(cr, Fn.Q, (double)Fn.QC[cr.ring], Fn.QC, Fn.DeW, Fn.DnS, hcx2, hcy2);
+      }
+    }
+  }
+  if (rout && rout[lvl]) rout[lvl][cc] = (TO)rd;
+  if (L.vis[cc]) {
+    a = fmax(a, fabs(rd));
+    b = b + rd * rd * (L.h[0] * L.h[1]);
+    f = fmax(f, fabs((double)L.FT[cc]));
+    n += 1;
+  }
+}
+
+// composite residual of every level in [k0, nlev]; optionally writes r (the
+// defect-correction right side) and resets F = FTrue on the tile's cells.
+// The tile table is (level, tile) pairs.
 template <typename T, typename TO>
 __global__ void __launch_bounds__(kTile) k_residual_all(const LevelDev<T>* __restrict__ LV, int nlev,
                                                         const int2* __restrict__ tl, int mass_corr,
@@ -663,47 +720,9 @@ __global__ void __launch_bounds__(kTile) k_residual_all(const LevelDev<T>* __res
                                                         TO* const* __restrict__ rout, int reset_f,
                                                         int k0) {
   const int2 lt = tl[blockIdx.x];
-  const LevelDev<T>& L = LV[lt.x];
-  const TileRec& tr = L.tiles[lt.y];
-  const PatchInfo& pi = tr.pi;
-  const int mx = pi.mx, my = pi.my;
-  const int c = tr.start + (int)threadIdx.x;
   double a = 0.0, b = 0.0, f = 0.0;
   long long n = 0;
-  if (c < mx * my) {
-    const int cc = pi.coff + c;
-    if (reset_f) L.F[cc] = L.FT[cc];
-    if (lt.x + 1 >= k0) {
-      const int j = c / mx;
-      const int i = c - j * mx;
-      const int ldq = mx + 2;
-      const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
-      const int fx = pi.xoff + i + j * (mx + 1);
-      const int fy = pi.yoff + i + j * mx;
-      const T hx = (T)L.h[0], hy = (T)L.h[1];
-      T r = L.FT[cc] - op_at<T>(L.Q, qc, ldq, L.DeW[fx], L.DeW[fx + 1], L.DnS[fy], L.DnS[fy + mx], L.CC[cc],
-                                hx * hx, hy * hy);
-      double rd = (double)r;
-      if (mass_corr && lt.x + 1 < nlev) {
-        const int row = L.tgt_of_cell[cc];
-        if (row >= 0) {
-          const LevelDev<T>& Fn = LV[lt.x + 1];
-          const double hcx2 = L.h[0] * L.h[0], hcy2 = L.h[1] * L.h[1];
-          for (int e = L.tgt_ptr[row]; e < L.tgt_ptr[row + 1]; ++e) {
-            const CorrRec cr = Fn.corr[L.tgt_ent[e]];
-            rd = rd + corr_value<T>(cr, Fn.Q, (double)Fn.QC[cr.ring], Fn.QC, Fn.DeW, Fn.DnS, hcx2, hcy2);
-          }
-        }
-      }
-      if (rout && rout[lt.x]) rout[lt.x][cc] = (TO)rd;
-      if (L.vis[cc]) {
-        a = fabs(rd);
-        b = rd * rd * (L.h[0] * L.h[1]);
-        f = fabs((double)L.FT[cc]);
-        n = 1;
-      }
-    }
-  }
+  residual_item<T, TO>(LV, nlev, lt.x, lt.y, (int)threadIdx.x, mass_corr, rout, reset_f, k0, a, b, f, n);
   block_reduce4(a, b, f, n, &part[blockIdx.x]);
 }
 
@@ -737,6 +756,132 @@ __global__ void __launch_bounds__(kTile) k_mass(LevelDev<T> L, ResidualOut* __re
     n = 1;
   }
   block_reduce4(0.0, s * (L.h[0] * L.h[1]), 0.0, n, &part[blockIdx.x]);
+}
+
+// ---------------------------------------------------------------- batched cycles
+// Workgroup-wide forms of the solve-stage steps, used by the batched kernel:
+// one workgroup carries one hierarchy through a whole stage step, so any
+// number of independent hierarchies advance in one launch and each one's
+// arithmetic is exactly that of its solo run.
+__device__ __forceinline__ void block_reduce4_wg(double a, double b, double c, long long n, ResidualOut* out) {
+  __shared__ double sa[16], sb[16], sc[16];
+  __shared__ long long sn[16];
+  a = wave_max(a);
+  b = wave_sum(b);
+  c = wave_max(c);
+  n = wave_sum_ll(n);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sa[w] = a; sb[w] = b; sc[w] = c; sn[w] = n;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ra = sa[0], rb = sb[0], rc = sc[0];
+    long long rn = sn[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+      ra = fmax(ra, sa[k]); rb = rb + sb[k]; rc = fmax(rc, sc[k]); rn += sn[k];
+    }
+    out->linf = ra; out->l2sq = rb; out->fnorm = rc; out->ncell = rn;
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void blk_reset_force(const LevelDev<T>* __restrict__ LV, int nlev) {
+  for (int k = 0; k < nlev; ++k) {
+    const LevelDev<T> L = LV[k];
+    for (int x = threadIdx.x; x < L.ncell; x += blockDim.x) L.F[x] = L.FT[x];
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ void blk_filldown(const LevelDev<T>* __restrict__ LV, int nlev, int homog) {
+  for (int k = nlev; k >= 2; --k) {  // ebTransfer.m:73-79
+    const LevelDev<T> L = LV[k - 1];
+    const LevelDev<T> P = LV[k - 2];
+    for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
+    __syncthreads();
+    blk_fill<T>(P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
+  }
+}
+
+// composite residual (ebCompResidual.m:23-84): QC refresh of the AMR levels,
+// then every visible cell of levels >= root
+__device__ void blk_residual(const LevelDev<double>* __restrict__ LV, int nlev, int root, int mass_corr,
+                             float* const* rout, ResidualOut* out) {
+  if (mass_corr) {
+    for (int k = max(2, root + 1); k <= nlev; ++k) {
+      const LevelDev<double> L = LV[k - 1];
+      const double* Pq = LV[k - 2].Q;
+      const int n = L.nring_all + L.nfoot;
+      for (int x = threadIdx.x; x < n; x += blockDim.x) {
+        if (x < L.nring_all) ring_item<double>(L, Pq, x);
+        else restrict_item<double>(L, (double*)nullptr, x - L.nring_all, 1);
+      }
+    }
+    __syncthreads();
+  }
+  double a = 0.0, b = 0.0, f = 0.0;
+  long long n = 0;
+  for (int k = root; k <= nlev; ++k) {
+    const int ncell = LV[k - 1].ntile * kTile;
+    for (int x = threadIdx.x; x < ncell; x += blockDim.x)
+      residual_item<double, float>(LV, nlev, k - 1, x / kTile, x % kTile, mass_corr, rout, 0, root, a, b, f, n);
+  }
+  block_reduce4_wg(a, b, f, n, out);
+}
+
+__global__ void __launch_bounds__(1024) k_batch_cycle(const BatchItem* __restrict__ items, int npre, int npost,
+                                                      double omega, int mass_corr, int mixed, int inner) {
+  extern __shared__ double rhs[];
+  const BatchItem it = items[blockIdx.x];
+  const LevelDev<double>* LV = it.LV64;
+  const int nlev = it.nlev;
+  if (it.phase == 0) {  // ebSolveCycles.m:11-19: fill every level, baseline residual
+    for (int k = 1; k <= nlev; ++k) blk_fill<double>(LV[k - 1], k > 1 ? (const double*)LV[k - 2].Q : nullptr, 0);
+    blk_residual(LV, nlev, it.root, mass_corr, mixed ? it.rout32 : nullptr, it.out);
+    return;
+  }
+  if (!mixed) {  // ebSolveCycles.m:28-41
+    blk_reset_force<double>(LV, nlev);
+    blk_vcycle<double>(LV, nlev, it.cst, npre, npost, omega, 0, mass_corr, rhs);
+    blk_filldown<double>(LV, nlev, 0);
+    blk_residual(LV, nlev, it.root, mass_corr, nullptr, it.out);
+    return;
+  }
+  // FP32 V-cycles on the correction problem, FP64 update, filldown, ghosts,
+  // residual (which also writes the next FP32 right side)
+  const LevelDev<float>* L32 = it.LV32;
+  for (int k = 0; k < nlev; ++k) {
+    const LevelDev<float> L = L32[k];
+    for (int x = threadIdx.x; x < L.nq; x += blockDim.x) L.Q[x] = 0.0f;
+    for (int x = threadIdx.x; x < L.nqc; x += blockDim.x) L.QC[x] = 0.0f;
+  }
+  __syncthreads();
+  for (int i = 0; i < inner; ++i) {
+    blk_reset_force<float>(L32, nlev);
+    blk_vcycle<float>(L32, nlev, it.cst, npre, npost, (float)omega, 1, mass_corr, rhs);
+    if (i + 1 < inner) blk_filldown<float>(L32, nlev, 1);
+  }
+  for (int k = 0; k < nlev; ++k) {
+    const LevelDev<double> L = LV[k];
+    const float* E = L32[k].Q;
+    const int ncell = L.ntile * kTile;
+    for (int x = threadIdx.x; x < ncell; x += blockDim.x) {
+      const TileRec& tr = L.tiles[x / kTile];
+      const int c = tr.start + x % kTile;
+      if (c >= tr.pi.mx * tr.pi.my) continue;
+      const int j = c / tr.pi.mx;
+      const int ii = c - j * tr.pi.mx;
+      const int qc = tr.pi.qoff + (ii + 1) + (j + 1) * (tr.pi.mx + 2);
+      L.Q[qc] = L.Q[qc] + (double)E[qc];
+    }
+  }
+  __syncthreads();
+  blk_filldown<double>(LV, nlev, 0);
+  for (int k = 1; k <= nlev; ++k) blk_fill<double>(LV[k - 1], k > 1 ? (const double*)LV[k - 2].Q : nullptr, 0);
+  blk_residual(LV, nlev, it.root, mass_corr, it.rout32, it.out);
 }
 
 // ---------------------------------------------------------------- elementwise
@@ -920,6 +1065,13 @@ void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double
 void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rotate, hipStream_t s) {
   if (!L.ntile) return;
   hipLaunchKernelGGL(k_history, dim3(L.ntile), dim3(kTile), 0, s, L, Uo, Uoo, rotate);
+}
+
+void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre, int npost, double omega,
+                        int mass_corr, int mixed, int inner, hipStream_t s) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(k_batch_cycle, dim3(nitems), dim3(1024), (size_t)max_m * sizeof(double), s, items, npre,
+                     npost, omega, mass_corr, mixed, inner);
 }
 
 #define BSAM_INST(T)                                                                                    \

@@ -1247,6 +1247,13 @@ int bsam_init_level_prolong(bsam_hier* h, int k) {
   return sync(h);
 }
 
+// |r|_inf / max(|FTrue|_inf, realmin) (ebCompResidual.m:328-332); NaN when the
+// residual holds non-finite values (the max-reductions skip NaN, the sum does not)
+static double residual_rel(const ResidualOut& r) {
+  if (!std::isfinite(r.l2sq)) return NAN;
+  return r.linf / std::max(r.fnorm, 2.2250738585072014e-308);
+}
+
 static int read_residual(bsam_hier* h, bsam_residual* out) {
   HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -1254,7 +1261,7 @@ static int read_residual(bsam_hier* h, bsam_residual* out) {
     out->linf = h->hres->linf;
     out->l2 = std::sqrt(h->hres->l2sq);
     out->fnorm = std::max(h->hres->fnorm, 2.2250738585072014e-308);
-    out->rel = out->linf / out->fnorm;
+    out->rel = residual_rel(*h->hres);
     out->ncell = h->hres->ncell;
   }
   return 0;
@@ -1368,9 +1375,8 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
     if (int rc = run_cycle(h, mixed)) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
     const ResidualOut r = *h->hres;
-    const double fn = std::max(r.fnorm, 2.2250738585072014e-308);
     hist.push_back(r.linf);
-    rel = r.linf / fn;
+    rel = residual_rel(r);
     ncell = r.ncell;
     if (!std::isfinite(rel)) break;
   }
@@ -1775,6 +1781,95 @@ int bsam_parabolic_step(bsam_hier* h, double dt, int scheme, const bsam_solve_op
     double m = 0, a = 0;
     if (int rc = bsam_composite_mean(h, &m, &a)) return rc;
     *mass = m * a;  // compMass: mbar * vol (ebpSolve.m:103-106)
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ batches
+// Many independent hierarchies advanced by one launch: one workgroup per
+// hierarchy runs its stage preamble or one whole cycle (same arithmetic as
+// the hierarchy's own solve), so requests of any size and level count share
+// the device without changing each other's results.
+struct bsam_batch {
+  bsam_ctx* ctx = nullptr;
+  int cap = 0;
+  hipStream_t stream = nullptr;
+  DevBufs<char> mem;
+  BatchItem* d_items = nullptr;
+  ResidualOut* d_out = nullptr;
+  BatchItem* h_items = nullptr;  // pinned
+  ResidualOut* h_out = nullptr;  // pinned
+  ~bsam_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (h_items) (void)hipHostFree(h_items);
+    if (h_out) (void)hipHostFree(h_out);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+int bsam_batch_create(bsam_ctx* ctx, int capacity, bsam_batch** out) {
+  if (!ctx || !out || capacity < 1) return fail(BSAM_E_ARG, "bsamgpu:arg: bad batch request");
+  if (ctx->device < 0) return fail(BSAM_E_ARG, "bsamgpu:arg: host-only context has no device state");
+  HIPCHK(hipSetDevice(ctx->device));
+  auto b = std::make_unique<bsam_batch>();
+  b->ctx = ctx;
+  b->cap = capacity;
+  int rc;
+  if ((rc = b->mem.alloc(&b->d_items, capacity))) return rc;
+  if ((rc = b->mem.alloc(&b->d_out, capacity))) return rc;
+  HIPCHK(hipHostMalloc((void**)&b->h_items, capacity * sizeof(BatchItem), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&b->h_out, capacity * sizeof(ResidualOut), hipHostMallocDefault));
+  HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  *out = b.release();
+  return 0;
+}
+
+void bsam_batch_destroy(bsam_batch* b) { delete b; }
+
+int bsam_batch_step(bsam_batch* b, bsam_hier* const* hs, const int* phase, int n, const bsam_solve_opts* o,
+                    bsam_residual* out) {
+  if (!b || !hs || !phase || !o || !out || n < 0) return fail(BSAM_E_ARG, "bsamgpu:arg: bad batch step");
+  if (n > b->cap) return fail(BSAM_E_ARG, "bsamgpu:arg: %d items exceed the batch capacity %d", n, b->cap);
+  if (o->npre < 0 || o->npost < 0) return fail(BSAM_E_ARG, "bsamgpu:arg: negative sweep count");
+  if (n == 0) return 0;
+  HIPCHK(hipSetDevice(b->ctx->device));
+  const bool mixed = o->precision == BSAM_MIXED;
+  int max_m = 1;
+  for (int i = 0; i < n; ++i) {
+    bsam_hier* h = hs[i];
+    if (!h || h->ctx->device != b->ctx->device)
+      return fail(BSAM_E_ARG, "bsamgpu:arg: batch item %d is not a hierarchy on the batch's device", i);
+    if (phase[i] != 0 && phase[i] != 1) return fail(BSAM_E_ARG, "bsamgpu:arg: batch phase must be 0 or 1");
+    for (int j = 0; j < i; ++j)
+      if (hs[j] == h) return fail(BSAM_E_ARG, "bsamgpu:arg: hierarchy listed twice in one batch step");
+    if (mixed)
+      if (int rc = ensure32(h)) return rc;
+    BatchItem& it = b->h_items[i];
+    it.LV64 = h->lv64;
+    it.LV32 = mixed ? h->lv32 : nullptr;
+    it.rout32 = mixed ? h->rout32 : nullptr;
+    it.out = b->d_out + i;
+    it.cst = h->cst;
+    it.nlev = h->nlev;
+    it.root = h->root;
+    it.phase = phase[i];
+    it.pad = 0;
+    max_m = std::max(max_m, h->cst.M);
+  }
+  hipStream_t s = b->stream;
+  HIPCHK(hipMemcpyAsync(b->d_items, b->h_items, n * sizeof(BatchItem), hipMemcpyHostToDevice, s));
+  launch_batch_cycle(b->d_items, n, max_m, o->npre, o->npost, o->omega, o->mass_correction, mixed ? 1 : 0,
+                     std::max(1, o->inner_cycles), s);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->h_out, b->d_out, n * sizeof(ResidualOut), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  for (int i = 0; i < n; ++i) {
+    const ResidualOut& r = b->h_out[i];
+    out[i].linf = r.linf;
+    out[i].l2 = std::sqrt(r.l2sq);
+    out[i].fnorm = std::max(r.fnorm, 2.2250738585072014e-308);
+    out[i].rel = residual_rel(r);
+    out[i].ncell = r.ncell;
   }
   return 0;
 }

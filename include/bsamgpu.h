@@ -226,6 +226,23 @@ int bsam_parabolic_advance(bsam_hier *h);
 int bsam_parabolic_step(bsam_hier *h, double dt, int scheme, const bsam_solve_opts *opts,
                         bsam_stage_stats *out, double *mass);
 
+/* Batched stage steps for serving many independent hierarchies
+ * (no counterpart in the reference, which solves one hierarchy at a time).
+ * One launch advances n hierarchies of the batch's device; each runs in its
+ * own workgroup with exactly the arithmetic of its own bsam_solve_cycles, so
+ * results do not depend on what else is in the batch.  phase[i] = 0 runs the
+ * stage preamble of ebSolveCycles.m:11-19 (fill every level, baseline
+ * residual), 1 runs one cycle of ebSolveCycles.m:28-41 (reset F, V-cycle,
+ * filldown, residual; with opts->precision == BSAM_MIXED the FP32-inner
+ * defect-correction step).  out[i] receives item i's composite residual; the
+ * caller applies the stopping rule (rel <= RelTol or the cycle cap).
+ * opts->coarse_cells, smoother and use_graph are not used. */
+typedef struct bsam_batch bsam_batch;
+int bsam_batch_create(bsam_ctx *ctx, int capacity, bsam_batch **out);
+void bsam_batch_destroy(bsam_batch *b);
+int bsam_batch_step(bsam_batch *b, bsam_hier *const *hs, const int *phase, int n,
+                    const bsam_solve_opts *opts, bsam_residual *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,148 @@
+"""Served workload: seeded request fixture, and (GPU) the continuous-batching
+loop against one-at-a-time solves.
+
+Stated tolerance: bit-identical.  A request's stage records (V-cycles,
+residual histories) and its final Q are the same whether it is served alone
+or in a mix, and in FP64 they equal the plain multi-launch ebSolve path.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from bsamgpu import configs, serving
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _fixture():
+    with open(os.path.join(GOLD, "config5_requests.json")) as f:
+        return json.load(f)
+
+
+def test_request_fixture_matches_generator():
+    d = _fixture()
+    assert d["requests"] == configs.served_params(512, seed=5)
+    assert np.allclose(d["arrivals_at_rate_1"], configs.served_arrivals(512, 1.0, seed=55), rtol=0, atol=0)
+    for r in d["requests"]:
+        assert r["base"] in (32, 64, 128) and 2 <= r["levels"] <= 5
+        assert sum(abs(a) for a in r["a"]) <= 0.5 and 0 <= r["C"] <= 2
+        assert all(b in ("dirichlet", "neumann") for b in r["bc"])
+
+
+def test_config2_fixture_matches_generator():
+    with open(os.path.join(GOLD, "config2_params.json")) as f:
+        d = json.load(f)
+    ref = configs.varcoef_params(2)
+    assert all(d[k] == ref[k] for k in ref)
+
+
+def test_request_stopping_rule():
+    """ebSolveCycles.m:24-41: stop at rel <= RelTol, at the cycle cap, or on
+    a non-finite residual."""
+    o = configs.served_options({"base": 32, "levels": 2}, MaxVCycles=3)
+    r = serving.Request(0, None, o)
+    assert not r.take({"linf": 1.0, "rel": 1.0, "ncell": 10})
+    assert not r.take({"linf": 0.5, "rel": 0.5, "ncell": 10})
+    assert not r.take({"linf": 0.1, "rel": 0.1, "ncell": 10})
+    assert r.take({"linf": 0.01, "rel": 0.01, "ncell": 10})  # third cycle: cap
+    assert r.it == 3 and r.hist == [1.0, 0.5, 0.1, 0.01]
+    r = serving.Request(1, None, o)
+    assert r.take({"linf": 1e-12, "rel": 1e-12, "ncell": 10})  # converged before any cycle
+    r = serving.Request(2, None, o)
+    r.take({"linf": 1.0, "rel": 1.0, "ncell": 10})
+    assert r.take({"linf": float("nan"), "rel": float("nan"), "ncell": 10})
+
+
+def _requests(ids, rate=None):
+    d = _fixture()
+    arr = d["arrivals_at_rate_1"]
+    out = []
+    for i in ids:
+        par = d["requests"][i]
+        t = arr[i] / rate if rate else 0.0
+        out.append(serving.Request(i, configs.served_problem(par), configs.served_options(par), t))
+    return out
+
+
+# small and mid-size requests of every kind: Dirichlet/Neumann mixes, 1-3 stages
+PICK = [0, 2, 3, 6, 8, 9, 11, 12, 17, 20, 25, 31]
+
+
+def _same(a, b):
+    assert len(a.stages) == len(b.stages), (a.id, len(a.stages), len(b.stages))
+    for sa, sb in zip(a.stages, b.stages):
+        assert sa["vcycles"] == sb["vcycles"] and sa["resHist"] == sb["resHist"], a.id
+    for qa, qb in zip(a.Q, b.Q):
+        for x, y in zip(qa, qb):
+            assert np.array_equal(x, y), a.id
+
+
+@pytest.mark.gpu
+def test_batched_fp64_equals_plain_solves():
+    reqs = _requests(PICK)
+    srv = serving.Server(slots=64)
+    done = {r.id: r for r in srv.run(reqs, realtime=False)}
+    srv.close()
+    for r in reqs:
+        assert done[r.id].error is None, done[r.id].error
+        stages, Q = serving.serial_solve(r)
+        print(f"request {r.id}: {len(stages)} stage(s), cycles {[s['vcycles'] for s in stages]}", flush=True)
+        assert [s["vcycles"] for s in stages] == [s["vcycles"] for s in done[r.id].stages]
+        assert [s["resHist"] for s in stages] == [s["resHist"] for s in done[r.id].stages]
+        for qa, qb in zip(Q, done[r.id].Q):
+            for x, y in zip(qa, qb):
+                assert np.array_equal(x, y), r.id
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_result_is_the_same_alone_and_in_the_mix(precision):
+    alone = {}
+    for r in _requests(PICK[:6]):
+        srv = serving.Server(slots=4, precision=precision)
+        alone[r.id] = srv.run([r], realtime=False)[0]
+        srv.close()
+    srv = serving.Server(slots=64, precision=precision)
+    mix = srv.run(_requests(PICK), realtime=False)
+    srv.close()
+    for r in mix:
+        assert r.error is None, r.error
+        if r.id in alone:
+            _same(r, alone[r.id])
+        assert r.stages[-1]["finalRel"] <= r.opts.RelTol
+
+
+@pytest.mark.gpu
+def test_mixed_precision_serving_accuracy():
+    """FP32-inner served solves against the FP64 ones: same converged
+    solution to the requests' tolerance (1e-8 relative residual)."""
+    reqs64 = _requests(PICK[:8])
+    srv = serving.Server(slots=16, precision="fp64")
+    r64 = {r.id: r for r in srv.run(reqs64, realtime=False)}
+    srv.close()
+    srv = serving.Server(slots=16, precision="mixed")
+    rmx = {r.id: r for r in srv.run(_requests(PICK[:8]), realtime=False)}
+    srv.close()
+    for i, a in r64.items():
+        b = rmx[i]
+        if len(a.Q) != len(b.Q):
+            continue  # a different hierarchy (tagging saw a different stage solution)
+        num = sum(float(np.sum((x - y)[1:-1, 1:-1] ** 2)) for qa, qb in zip(a.Q, b.Q) for x, y in zip(qa, qb))
+        den = sum(float(np.sum(x[1:-1, 1:-1] ** 2)) for qa in a.Q for x in qa)
+        print(f"request {i}: rel L2 mixed vs fp64 {np.sqrt(num / den):.2e}", flush=True)
+        assert np.sqrt(num / den) < 1e-6
+
+
+@pytest.mark.gpu
+def test_realtime_poisson_arrivals():
+    reqs = _requests(range(24), rate=200.0)
+    srv = serving.Server(slots=32, mem_budget=1 << 30)
+    done = srv.run(reqs, realtime=True)
+    srv.close()
+    assert len(done) == 24 and all(r.error is None for r in done)
+    assert all(r.t_done >= r.arrival and r.t_admit >= r.arrival - 1e-9 for r in done)
+    st = serving.latency_stats(done)
+    print(st, f"ticks {srv.ticks}, mean batch {srv.tick_items / max(srv.ticks, 1):.1f}", flush=True)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
