@@ -22,6 +22,8 @@ import json
 import os
 import statistics
 import sys
+
+import numpy as np
 import time
 import warnings
 
@@ -31,6 +33,20 @@ sys.path.insert(0, ROOT)
 warnings.filterwarnings("ignore")
 
 HBM_PEAK_GBS = 8000.0
+
+
+def pmc_traffic(kernel_tag):
+    """HBM-side bytes per launch of the profiled kernel, as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) recorded in
+    profiles/; None when no such record exists."""
+    path = os.path.join(ROOT, "profiles", f"pmc_traffic_{kernel_tag}.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("TRAFFIC_BYTES_PER_LAUNCH="):
+                return int(line.split("=", 1)[1])
+    return None
 
 
 def log(msg):
@@ -96,6 +112,118 @@ def cpu_baseline(H):
             "seconds_per_solve": dt, "seconds_per_vcycle": dt / max(st["vcycles"], 1)}
 
 
+def bench_large_smoother(n, launches=40):
+    """The same smoother kernel on one uniform n x n level (D = C = 1): large
+    enough to be HBM-bound rather than launch-bound."""
+    from bsamgpu import Options, Problem, hierarchy
+    from bsamgpu.device import GpuHierarchy
+
+    prob = Problem([0, 1, 0, 1], 1, 1, 1, ("dirichlet", 0))
+    H = hierarchy.setup(prob, Options(BaseCells=n, MaxLevels=1, Verbose=0))
+    G = GpuHierarchy(H)
+    k = H.nlev
+    out = {"cells": n * n}
+    for name, mixed, b in (("fp64", False, 8), ("fp32", True, 4)):
+        G.time_smoother(k, 5, mixed=mixed)
+        ms = G.time_smoother(k, launches, mixed=mixed)
+        byt = G.smoother_bytes(k, b)
+        out[name] = {"us_per_launch": round(ms * 1e3, 2), "bytes_per_launch": byt,
+                     "GBps": round(byt / ms / 1e6, 1), "frac": round(byt / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    G.close()
+    return out
+
+
+def bench_parabolic():
+    """Heat BDF2 (BaseCells 64, MaxLevels 4, Dt 0.002, T 0.1, RegridEvery 5,
+    RelTol 1e-12 per step): wall time of ebp_solve with the history resident,
+    V-cycles, composite mass drift; FP32-inner against FP64."""
+    from bsamgpu import parabolic as par
+
+    def run(precision):
+        pp = par.ParabolicProblem([0, 1, 0, 1], 1, 0, 0, ("neumann", 0),
+                                  lambda x, y: 1 + 0.2 * np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y))
+        opts = par.parabolic_options(Scheme="bdf2", Dt=0.002, TFinal=0.1, BaseCells=64, MaxLevels=4,
+                                     RegridEvery=5, StepVerbose=0, RelTol=1e-12, Precision=precision)
+        t = time.perf_counter()
+        sol = par.ebp_solve(pp, opts)
+        wall = time.perf_counter() - t
+        m = np.array(sol["mass"])
+        gpu_ms = sum(st["gpu_ms"] for st in sol["stages"])
+        return sol, {"wall_s": round(wall, 4), "steps": len(sol["cycles"]), "vcycles": int(sum(sol["cycles"])),
+                     "regrids": sol["nregrids"], "mass_drift_rel": float(np.max(np.abs(m - m[0])) / abs(m[0])),
+                     "solve_gpu_ms": round(gpu_ms, 3), "steps_per_s": round(len(sol["cycles"]) / wall, 2)}
+
+    run("fp64")  # warm-up (first-call library and graph setup)
+    s64, r64 = run("fp64")
+    smx, rmx = run("mixed")
+    H64, Hmx = s64["H"], smx["H"]
+    if H64.nlev == Hmx.nlev:
+        num = den = 0.0
+        for k in range(H64.rootIdx, H64.nlev + 1):
+            for p in range(H64.L(k).np):
+                a = Hmx.L(k).Q[p][1:-1, 1:-1]
+                b = H64.L(k).Q[p][1:-1, 1:-1]
+                num += float(((a - b) ** 2).sum())
+                den += float((b ** 2).sum())
+        rmx["rel_l2_vs_fp64"] = (num / den) ** 0.5
+    return {"workload": "heat BDF2 64^2 root, MaxLevels 4, 50 steps, 9 regrids, RelTol 1e-12",
+            "fp64": r64, "mixed": rmx}
+
+
+def bench_serving(nreq, rate):
+    """Served stream: nreq requests of the seeded fixture with Poisson
+    arrivals at `rate` req/s, continuous batching (FP32-inner and FP64)
+    against one-at-a-time plain FP64 ebSolve on the same arrivals."""
+    import json as _json
+
+    from bsamgpu import configs, serving
+
+    with open(os.path.join(ROOT, "tests", "golden", "config5_requests.json")) as f:
+        fx = _json.load(f)
+
+    def reqs():
+        return [serving.Request(i, configs.served_problem(fx["requests"][i]),
+                                configs.served_options(fx["requests"][i]), fx["arrivals_at_rate_1"][i] / rate)
+                for i in range(nreq)]
+
+    out = {"requests": nreq, "rate_per_s": rate, "tolerance": 1e-8}
+    # warm-up of the batched kernels and the host pool
+    srv = serving.Server(slots=256, mem_budget=32 << 30, precision="mixed")
+    srv.run(reqs()[:8], realtime=False)
+    srv.close()
+    for prec in ("mixed", "fp64"):
+        srv = serving.Server(slots=256, mem_budget=32 << 30, precision=prec)
+        t = time.perf_counter()
+        done = srv.run(reqs(), realtime=True)
+        wall = time.perf_counter() - t
+        srv.close()
+        bad = [r.id for r in done if r.error is not None or r.stages[-1]["finalRel"] > r.opts.RelTol]
+        st = serving.latency_stats(done)
+        out[f"batched_{prec}"] = {"p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+                                  "throughput_per_s": round(nreq / max(r.t_done for r in done), 2),
+                                  "wall_s": round(wall, 3), "ticks": srv.ticks,
+                                  "mean_batch": round(srv.tick_items / max(srv.ticks, 1), 1),
+                                  "peak_budget_bytes": srv.peak_bytes, "unconverged": bad}
+    # one at a time, FIFO on the same arrivals
+    service = []
+    t0 = time.perf_counter()
+    for r in reqs():
+        t = time.perf_counter()
+        serving.serial_solve(r, "fp64")
+        service.append(time.perf_counter() - t)
+    total = time.perf_counter() - t0
+    fin = 0.0
+    lat = []
+    for r, sv in zip(reqs(), service):
+        fin = max(fin, r.arrival) + sv
+        lat.append(fin - r.arrival)
+    out["one_at_a_time_fp64"] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                                 "throughput_per_s": round(nreq / total, 2),
+                                 "mean_service_ms": round(1e3 * total / nreq, 2)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +236,15 @@ def main():
     ap.add_argument("--smoother", default="exact", choices=["exact", "tiled"])
     ap.add_argument("--coarse-cells", type=int, default=0)
     ap.add_argument("--sweep", action="store_true", help="tuning sweep of the coarse-cycle cut")
+    ap.add_argument("--no-parabolic", action="store_true")
+    ap.add_argument("--large-smoother", type=int, default=4096, metavar="N",
+                    help="also time the smoother on one uniform N x N level (0: skip)")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=128)
+    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--only", choices=["parabolic", "serving"], help="run one secondary workload and print it")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="launch a plain device copy of MIB MiB 20 times (scale for the PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
@@ -115,15 +252,37 @@ def main():
         sys.exit(2)
     os.environ.setdefault("OMP_NUM_THREADS", "1")
 
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(20):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 20 copies of {args.copy_calibration} MiB (read + write each)")
+        return
+
     from bsamgpu import _lib
     from bsamgpu.device import GpuHierarchy
+
+    if args.only == "parabolic":
+        print(json.dumps(bench_parabolic()), flush=True)
+        return
+    if args.only == "serving":
+        print(json.dumps(bench_serving(args.serve_requests, args.serve_rate)), flush=True)
+        return
 
     log("building starfish hierarchy on the host")
     t = time.perf_counter()
     H, opts = build_starfish()
     log(f"hierarchy: {H.nlev} levels, patches {[H.L(k).np for k in range(1, H.nlev + 1)]} "
         f"({time.perf_counter() - t:.1f} s)")
+    t = time.perf_counter()
     G = GpuHierarchy(H)
+    upload_s = time.perf_counter() - t
+    log(f"device hierarchy (tables + upload): {upload_s:.3f} s")
     kf = H.nlev
     sm_bytes = G.smoother_bytes(kf, 8)
     if args.kernel_only:
@@ -170,6 +329,23 @@ def main():
         per_level[str(k)] = round(G.smoother_bytes(k, 8) / msk / 1e6, 1)
     log(f"smoother fp64 {ms64 * 1e3:.2f} us/launch {gbs64:.0f} GB/s; fp32 {ms32 * 1e3:.2f} us {gbs32:.0f} GB/s")
 
+    large = None
+    if args.large_smoother:
+        log(f"smoother on a uniform {args.large_smoother}^2 level")
+        large = bench_large_smoother(args.large_smoother)
+        log(f"large smoother: {large}")
+
+    parab = None
+    if not args.no_parabolic:
+        log("parabolic heat workload")
+        parab = bench_parabolic()
+        log(f"parabolic: {parab['fp64']}")
+    serv = None
+    if not args.no_serving:
+        log(f"serving {args.serve_requests} requests at {args.serve_rate}/s")
+        serv = bench_serving(args.serve_requests, args.serve_rate)
+        log(f"serving: {serv}")
+
     base = None
     if not args.no_cpu_baseline:
         log("CPU baseline (numpy oracle, one solve)")
@@ -195,23 +371,28 @@ def main():
                    "levels": H.nlev, "patches_per_level": [H.L(k).np for k in range(1, H.nlev + 1)],
                    "rel_tol": opts.RelTol},
         "vcycles": s64["vcycles"],
+        "hierarchy_create_s": round(upload_s, 4),
         "final_rel": s64["finalRel"],
         "vcycle_s": round(ms_step / 1e3 / s64["vcycles"], 7),
         "p50_ms": round(pct(lat64, 50), 3),
         "p99_ms": round(pct(lat64, 99), 3),
         "smoother_GBps": round(gbs64, 1),
         "smoother_GBps_per_level": per_level,
+        "smoother_uniform_level": large,
         "mixed": {"value": round(args.steps / totmx, 3), "ms_per_step": round(totmx / args.steps * 1e3, 3),
                   "vcycles": smx["vcycles"], "final_rel": smx["finalRel"],
                   "speedup_vs_fp64": round(tot64 / totmx, 3), "rel_l2_vs_fp64": err_mx,
                   "p50_ms": round(pct(latmx, 50), 3), "p99_ms": round(pct(latmx, 99), 3),
                   "smoother_fp32_GBps": round(gbs32, 1)},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs64 / HBM_PEAK_GBS, 4),
+                     "traffic": pmc_traffic("k_relax_starfish_level7"),
                      "kernel": f"k_relax<double> level {kf} ({H.L(kf).np} patches, "
                                f"{sum(H.L(kf).dims(p)[0] * H.L(kf).dims(p)[1] for p in range(H.L(kf).np))} cells)",
                      "bytes_per_launch": sm_bytes, "us_per_launch": round(ms64 * 1e3, 3)},
         "cpu_baseline": base,
+        "parabolic": parab,
+        "serving": serv,
     }
     print(json.dumps(out), flush=True)
 
