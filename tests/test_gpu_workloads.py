@@ -1,0 +1,97 @@
+"""Device path on the benchmark workloads at their real sizes.
+
+  README elliptic (config 1) through eb_solve: the published stage log.
+  Variable-D 256^2 (config 2): bit-exact against the oracle, and the finding
+  that ulap at 1e-3 tags nothing there (one stage, one AMR level).
+  Starfish 64 -> 1024 (config 3, 544,664 cells): the whole FP64 solve equals
+  the oracle's bit for bit (residual history, every Q), and the mixed form
+  reaches the same solution within 1e-10 relative L2.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from bsamgpu import _lib, configs, solve
+from bsamgpu.device import GpuHierarchy
+from oracle import bsam_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "published.json")))
+
+
+def test_readme_elliptic_published_log_on_device():
+    sol = solve.eb_solve(configs.plain_problem(), configs.plain_options(RelTol=1e-9))
+    ref = GOLD["readme_elliptic"]
+    stages = sol["stats"]["stages"]
+    assert len(stages) == 3
+    for st, want in zip(stages, ref["stages"]):
+        assert st["amrLevels"] == want["amrLevels"]
+        assert st["vcycles"] == want["vcycles"]
+        assert f"{st['finalRel']:.2e}" == want["finalRel"]
+        assert f"{st['avgFactor']:.3f}" == want["avgFactor"]
+        assert st["visibleCells"] == want["visibleCells"]
+    sol["device"].close()
+
+
+def test_varcoef_256_bit_exact_and_untagged():
+    with open(os.path.join(os.path.dirname(__file__), "golden", "config2_params.json")) as f:
+        par = json.load(f)
+    prob = configs.varcoef_problem(par)
+    opts = configs.varcoef_options()
+    sol = solve.eb_solve(prob, opts)
+    H = sol["H"]
+    assert len(sol["stats"]["stages"]) == 1 and H.nlev == H.rootIdx  # nothing tagged at 1e-3
+    sol["device"].close()
+    from bsamgpu import hierarchy
+
+    H0 = hierarchy.setup(prob, opts)
+    Orc = O.OracleHierarchy(H0)
+    so = Orc.solve_cycles()
+    G = GpuHierarchy(H0)
+    sg = G.solve_cycles()
+    assert sg["resHist"] == so["resHist"] and sg["vcycles"] == so["vcycles"]
+    for k in range(1, H0.nlev + 1):
+        for p, q in enumerate(G.get_field(k, _lib.BSAM_Q)):
+            assert np.array_equal(q, Orc.L(k).Q[p]), (k, p)
+    G.close()
+
+
+@pytest.fixture(scope="module")
+def starfish():
+    prob, aux, g = configs.starfish_problem()
+    return solve.build_hierarchy(prob, configs.starfish_options(g))
+
+
+def test_starfish_full_size_bit_exact(starfish):
+    H = starfish
+    print("starfish: oracle solve (CPU)", flush=True)
+    Orc = O.OracleHierarchy(H)
+    so = Orc.solve_cycles()
+    print(f"oracle: {so['vcycles']} cycles, rel {so['finalRel']:.3e}; device solve", flush=True)
+    G = GpuHierarchy(H)
+    sg = G.solve_cycles()
+    assert sg["vcycles"] == so["vcycles"] == GOLD["starfish_full"]["vcycles"]
+    assert sg["resHist"] == so["resHist"]
+    assert sg["finalRel"] == so["finalRel"]
+    for k in range(1, H.nlev + 1):
+        got = G.get_field(k, _lib.BSAM_Q)
+        for p in range(H.L(k).np):
+            assert np.array_equal(got[p], Orc.L(k).Q[p]), (k, p)
+    # FP32-inner form: same fixed point (north-star bound 1e-10 rel L2, visible cells)
+    G.reset()
+    sm = G.solve_cycles(precision="mixed")
+    num = den = 0.0
+    for k in range(H.rootIdx, H.nlev + 1):
+        got = G.get_field(k, _lib.BSAM_Q)
+        for p in range(H.L(k).np):
+            vis = ~H.L(k).covered[p]
+            a = got[p][1:-1, 1:-1][vis]
+            b = Orc.L(k).Q[p][1:-1, 1:-1][vis]
+            num += float(((a - b) ** 2).sum())
+            den += float((b ** 2).sum())
+    err = (num / den) ** 0.5
+    print(f"mixed: {sm['vcycles']} cycles, rel {sm['finalRel']:.3e}, rel L2 vs FP64 oracle {err:.2e}", flush=True)
+    assert sm["finalRel"] <= 1e-10 and err < 1e-10
+    G.close()
