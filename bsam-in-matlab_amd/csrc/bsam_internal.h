@@ -26,6 +26,7 @@ struct PatchInfo {
   int bc[4];              // W E S N: 0 internal, 1 Dirichlet, 2 Neumann
   int qoff, coff, xoff, yoff, qcoff, goff, ccoff;
   int gbase;              // first ghost record: W(my) E(my) S(mx) N(mx) SW SE NW NE
+  unsigned magic;         // c / mx == umulhi(c, magic) for every cell c (0: divide)
 };
 
 // ghost record kinds (one record per ghost cell of a level)
@@ -90,6 +91,9 @@ template <typename T>
 struct LevelDev {
   int n[2];
   double h[2];
+  double ih2[2];     // 1/h^2 (exact when pow2)
+  int pow2;          // h[0], h[1] are powers of two: q/h^2 == q*(1/h^2) bit for bit
+  int even_mx;       // every patch has an even width (colour-compacted passes)
   int np, ncell, nq, nx, ny, nqc, ng;
   int ntile, nctile, nghost, nring_all, nring_cf, nfoot, ncorr, ntgt;
   int nstile, stile_lds;  // smoother tiles and their largest LDS region (cells)

@@ -22,10 +22,35 @@ template <typename T> void launch_coarse_load(const LevelDev<T>& L, const LevelD
 template <typename T> void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
 template <typename T> void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s);
 template <typename T> void launch_coarsest(const CoarsestDev& C, const LevelDev<T>& L1, int homog, hipStream_t s);
-// V-cycle of levels 1..kc in one workgroup (LV_dev: device array of levels)
+// V-cycle of levels 1..kc in one workgroup (LV_dev: device array of levels,
+// LV_host: the same descriptors on the host).  When the levels' fields and
+// tables fit the workgroup's LDS they are staged there for the whole cycle.
 template <typename T>
-void launch_coarse_cycle(const LevelDev<T>* LV_dev, int kc, const CoarsestDev& C, int npre, int npost, T omega,
-                         int homog, int mass_corr, hipStream_t s);
+void launch_coarse_cycle(const LevelDev<T>* LV_dev, const LevelDev<T>* LV_host, int kc, const CoarsestDev& C,
+                         int npre, int npost, T omega, int homog, int mass_corr, hipStream_t s);
+
+// LDS staging of the coarse cycle: the arrays a V-cycle of levels 1..kc reads
+// or writes, in a fixed order shared by the host (sizing) and the kernel.
+constexpr int kStageMaxLevels = 8;
+constexpr long long kStageLdsLimit = 160LL * 1024 - 8 * 1024;
+__host__ __device__ inline long long stage_align(long long b) { return (b + 15) & ~15LL; }
+#define BSAM_STAGE_FIELDS(X, L, T)                                                                    \
+  X(Q, T, (L).nq) X(F, T, (L).ncell) X(QC, T, (L).nqc) X(ID, T, (L).ncell) X(CC, T, (L).ncell)        \
+  X(DeW, T, (L).nx) X(DnS, T, (L).ny) X(G, T, (L).ng) X(tiles, TileRec, (L).ntile)                   \
+  X(ghost, GhostRec, (L).nghost) X(foot, FootRec, (L).nfoot) X(ring_all, RingRec, (L).nring_all)     \
+  X(ring_cf, RingRec, (L).nring_cf) X(corr, CorrRec, (L).ncorr) X(wmap, int, (L).nqc)
+// fine_ncorr: reflux entries of level k+1 (the size of level k's tgt_ent);
+// negative when level k's targets are not used inside the cycle (top level)
+template <typename T>
+__host__ __device__ inline long long stage_level_bytes(const LevelDev<T>& L, int fine_ncorr) {
+  long long b = 0;
+#define BSAM_STAGE_SIZE(field, type, count) b += stage_align((long long)(count) * (long long)sizeof(type));
+  BSAM_STAGE_FIELDS(BSAM_STAGE_SIZE, L, T)
+#undef BSAM_STAGE_SIZE
+  if (fine_ncorr >= 0)
+    b += stage_align(4LL * L.ntgt) + stage_align(4LL * (L.ntgt + 1)) + stage_align(4LL * fine_ncorr);
+  return b;
+}
 template <typename T> void launch_qc_refresh_all(const LevelDev<T>* LV_dev, const int2* items, int nitems, hipStream_t s);
 template <typename T, typename TO>
 void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, int ntl, int mass_corr,

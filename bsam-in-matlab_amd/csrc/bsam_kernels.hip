@@ -38,6 +38,20 @@ namespace bsam {
 #define C13_ (1.0 / 3.0)
 #define C23_ (2.0 / 3.0)
 
+// q / h^2 of a level.  When h is a power of two so is h^2, and q * (1/h^2) is
+// the same correctly rounded number as q / h^2, at a fraction of the cost.
+template <typename T>
+__device__ __forceinline__ T div_h2(T q, T h2, T ih2, int pow2) {
+  if (pow2) return q * ih2;
+  return q / h2;
+}
+
+// row of cell c of a patch (c / mx), by a multiply-high when the host found
+// a magic number exact for every cell of the patch
+__device__ __forceinline__ int cell_row(int c, int mx, unsigned magic) {
+  return magic ? (int)__umulhi((unsigned)c, magic) : c / mx;
+}
+
 // ---------------------------------------------------------------- ghosts
 template <typename T>
 __device__ __forceinline__ T ghost_value(const GhostRec& r, const T* __restrict__ Q,
@@ -103,17 +117,9 @@ __device__ __forceinline__ int fill_items(const LevelDev<T>& L, const T* Pq) {
 // colour-c cell's ghost depends only on other-colour cells, on the cell
 // itself (read before it is written) and on the parent level.
 template <typename T>
-__device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __restrict__ Pq, int tile,
-                                           int lane, int colour, T omega, int homog, int inl) {
-  const TileRec& tr = L.tiles[tile];
-  const PatchInfo& pi = tr.pi;
+__device__ __forceinline__ void relax_cell(const LevelDev<T>& L, const T* __restrict__ Pq, const PatchInfo& pi,
+                                           int i, int j, T omega, int homog, int inl) {
   const int mx = pi.mx, my = pi.my;
-  const int c = tr.start + lane;
-  if (c >= mx * my) return;
-  const int j = c / mx;
-  const int i = c - j * mx;
-  // (li + lj + par0) mod 2 == colour-1 with 1-based local indices
-  if (((i + j + pi.par0) & 1) != colour - 1) return;
   T* __restrict__ Q = L.Q;
   const int ldq = mx + 2;
   const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
@@ -152,20 +158,54 @@ __device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __rest
       qn = ghost_value<T>(L.ghost[pi.gbase + 2 * my + mx + i], Q, Pq, homog);
     }
   }
-  const int cc = pi.coff + c;
+  const int cc = pi.coff + i + j * mx;
   const int fx = pi.xoff + i + j * (mx + 1);
   const int fy = pi.yoff + i + j * mx;
   const T hx2 = hx * hx, hy2 = hy * hy;
-  T tmp = L.F[cc] + qe * L.DeW[fx + 1] / hx2;
-  tmp = tmp + qw * L.DeW[fx] / hx2;
-  tmp = tmp + qn * L.DnS[fy + mx] / hy2;
-  tmp = tmp + qs * L.DnS[fy] / hy2;
+  const T ihx2 = (T)L.ih2[0], ihy2 = (T)L.ih2[1];
+  const int p2 = L.pow2;
+  T tmp = L.F[cc] + div_h2<T>(qe * L.DeW[fx + 1], hx2, ihx2, p2);
+  tmp = tmp + div_h2<T>(qw * L.DeW[fx], hx2, ihx2, p2);
+  tmp = tmp + div_h2<T>(qn * L.DnS[fy + mx], hy2, ihy2, p2);
+  tmp = tmp + div_h2<T>(qs * L.DnS[fy], hy2, ihy2, p2);
   tmp = tmp * L.ID[cc];
   if (omega == T(1)) {
     Q[qc] = tmp;
   } else {
     Q[qc] = omega * tmp + (T(1) - omega) * Q[qc];
   }
+}
+
+template <typename T>
+__device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __restrict__ Pq, int tile,
+                                           int lane, int colour, T omega, int homog, int inl) {
+  const TileRec& tr = L.tiles[tile];
+  const PatchInfo& pi = tr.pi;
+  const int mx = pi.mx, my = pi.my;
+  const int c = tr.start + lane;
+  if (c >= mx * my) return;
+  const int j = cell_row(c, mx, pi.magic);
+  const int i = c - j * mx;
+  // (li + lj + par0) mod 2 == colour-1 with 1-based local indices
+  if (((i + j + pi.par0) & 1) != colour - 1) return;
+  relax_cell<T>(L, Pq, pi, i, j, omega, homog, inl);
+}
+
+// Item t of a colour pass over colour-c cells only: the t-th cell of colour c
+// in tile order.  A tile of 256 cells of a patch with even mx holds 128 cells
+// of each colour (every row has mx/2 of them and tiles start on even cells),
+// so half the lanes cover a pass.  Returns false for padding items.
+template <typename T>
+__device__ __forceinline__ void relax_item_colour(const LevelDev<T>& L, const T* __restrict__ Pq, int t,
+                                                  int colour, T omega, int homog, int inl) {
+  const TileRec& tr = L.tiles[t >> 7];
+  const PatchInfo& pi = tr.pi;
+  const int mx = pi.mx;
+  const int a = (tr.start >> 1) + (t & 127);  // index among the patch's colour-c cells
+  if (2 * a >= mx * pi.my) return;
+  const int j = cell_row(2 * a, mx, pi.magic);  // a / (mx/2)
+  const int i = 2 * (a - j * (mx >> 1)) + ((colour - 1 + j + pi.par0) & 1);
+  relax_cell<T>(L, Pq, pi, i, j, omega, homog, inl);
 }
 
 // ---------------------------------------------------------------- tiled smoother
@@ -272,10 +312,10 @@ __global__ void __launch_bounds__(256) k_relax_tiled(LevelDev<T> L, const T* __r
         const int cc = pi.coff + i + j * mx;
         const int fx = pi.xoff + i + j * (mx + 1);
         const int fy = pi.yoff + i + j * mx;
-        T tmp = L.F[cc] + qe * L.DeW[fx + 1] / hx2;
-        tmp = tmp + qw * L.DeW[fx] / hx2;
-        tmp = tmp + qn * L.DnS[fy + mx] / hy2;
-        tmp = tmp + qs * L.DnS[fy] / hy2;
+        T tmp = L.F[cc] + div_h2<T>(qe * L.DeW[fx + 1], hx2, (T)L.ih2[0], L.pow2);
+        tmp = tmp + div_h2<T>(qw * L.DeW[fx], hx2, (T)L.ih2[0], L.pow2);
+        tmp = tmp + div_h2<T>(qn * L.DnS[fy + mx], hy2, (T)L.ih2[1], L.pow2);
+        tmp = tmp + div_h2<T>(qs * L.DnS[fy], hy2, (T)L.ih2[1], L.pow2);
         tmp = tmp * L.ID[cc];
         S[s] = omega == T(1) ? tmp : omega * tmp + (T(1) - omega) * S[s];
       }
@@ -309,19 +349,32 @@ __device__ __forceinline__ void restrict_item(const LevelDev<T>& L, T* __restric
 }
 
 // L(u) at one cell of a padded array (ebKernels.m:13-25)
+// the level's h^2 and reciprocals for div_h2
 template <typename T>
-__device__ __forceinline__ T op5(T u, T w, T e, T s, T n, T dw, T de, T ds, T dn, T cc, T hx2, T hy2) {
+struct H2 {
+  T hx2, hy2, ihx2, ihy2;
+  int pow2;
+};
+
+template <typename T>
+__device__ __forceinline__ H2<T> level_h2(const LevelDev<T>& L) {
+  const T hx = (T)L.h[0], hy = (T)L.h[1];
+  return H2<T>{hx * hx, hy * hy, (T)L.ih2[0], (T)L.ih2[1], L.pow2};
+}
+
+template <typename T>
+__device__ __forceinline__ T op5(T u, T w, T e, T s, T n, T dw, T de, T ds, T dn, T cc, const H2<T>& g) {
   const T fw = dw * (u - w);
   const T fe = de * (e - u);
   const T fs = ds * (u - s);
   const T fn = dn * (n - u);
-  return (-(fe - fw) / hx2 - (fn - fs) / hy2) + cc * u;
+  return (div_h2<T>(-(fe - fw), g.hx2, g.ihx2, g.pow2) - div_h2<T>(fn - fs, g.hy2, g.ihy2, g.pow2)) + cc * u;
 }
 
 template <typename T>
 __device__ __forceinline__ T op_at(const T* __restrict__ Q, int qc, int ldq, T dw, T de, T ds, T dn,
-                                   T cc, T hx2, T hy2) {
-  return op5<T>(Q[qc], Q[qc - 1], Q[qc + 1], Q[qc - ldq], Q[qc + ldq], dw, de, ds, dn, cc, hx2, hy2);
+                                   T cc, const H2<T>& g) {
+  return op5<T>(Q[qc], Q[qc - 1], Q[qc + 1], Q[qc - ldq], Q[qc + ldq], dw, de, ds, dn, cc, g);
 }
 
 // restricted fine residual R(F - L u) over one coarse footprint cell
@@ -329,8 +382,7 @@ template <typename T>
 __device__ __forceinline__ T restricted_residual(const LevelDev<T>& L, const FootRec& pi, int ci, int cj) {
   const int mx = pi.mx;
   const int ldq = mx + 2;
-  const T hx = (T)L.h[0], hy = (T)L.h[1];
-  const T hx2 = hx * hx, hy2 = hy * hy;
+  const H2<T> g = level_h2<T>(L);
   T rf[4];
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
@@ -340,24 +392,30 @@ __device__ __forceinline__ T restricted_residual(const LevelDev<T>& L, const Foo
     const int cc = pi.coff + i + j * mx;
     const int fx = pi.xoff + i + j * (mx + 1);
     const int fy = pi.yoff + i + j * mx;
-    rf[s] = L.F[cc] - op_at<T>(L.Q, qc, ldq, L.DeW[fx], L.DeW[fx + 1], L.DnS[fy], L.DnS[fy + mx], L.CC[cc],
-                              hx2, hy2);
+    rf[s] = L.F[cc] - op_at<T>(L.Q, qc, ldq, L.DeW[fx], L.DeW[fx + 1], L.DnS[fy], L.DnS[fy + mx], L.CC[cc], g);
   }
   return T(0.25) * (((rf[0] + rf[1]) + rf[2]) + rf[3]);
 }
 
 // one flux-balance contribution (ebFaceCorrections.m:111-133), in FP64;
 // `ring` is the QC ring value just outside the coarse face
+// hc: the coarse (parent) level's h^2 in FP64
 template <typename T>
 __device__ __forceinline__ double corr_value(const CorrRec& e, const T* __restrict__ Q, double ring,
                                              const T* __restrict__ QC, const T* __restrict__ DeW,
-                                             const T* __restrict__ DnS, double hcx2, double hcy2) {
+                                             const T* __restrict__ DnS, const H2<double>& hc) {
   const T* D = e.face < 2 ? DeW : DnS;
-  const double h2 = e.face < 2 ? hcx2 : hcy2;
   const double t1 = (double)D[e.d1] * ((double)Q[e.in1] - (double)Q[e.g1]);
   const double t2 = (double)D[e.d2] * ((double)Q[e.in2] - (double)Q[e.g2]);
   const double t3 = e.cf * (ring - (double)QC[e.qcin]);
-  return ((t1 + t2) + t3) / h2;
+  return e.face < 2 ? div_h2<double>((t1 + t2) + t3, hc.hx2, hc.ihx2, hc.pow2)
+                    : div_h2<double>((t1 + t2) + t3, hc.hy2, hc.ihy2, hc.pow2);
+}
+
+// FP64 h^2 of a level whatever the field type (reflux sums are FP64)
+template <typename T>
+__device__ __forceinline__ H2<double> level_h2_64(const LevelDev<T>& L) {
+  return H2<double>{L.h[0] * L.h[0], L.h[1] * L.h[1], L.ih2[0], L.ih2[1], L.pow2};
 }
 
 // ebCoarseLoad for one footprint cell, QC ring read from QC (standalone form)
@@ -367,10 +425,9 @@ __device__ __forceinline__ void coarse_load_item(const LevelDev<T>& L, const Lev
   const FootRec& pi = fr;  // patch offsets carried inline
   const T R = restricted_residual<T>(L, pi, fr.ci, fr.cj);
   const int ldc = pi.mx / 2 + 2;
-  const T hcx = (T)P.h[0], hcy = (T)P.h[1];
   const int qcc = pi.qcoff + (fr.ci + 1) + (fr.cj + 1) * ldc;
   const T Lc = op_at<T>(L.QC, qcc, ldc, P.DeW[fr.dew_w], P.DeW[fr.dew_e], P.DnS[fr.dns_s], P.DnS[fr.dns_n],
-                        P.CC[fr.pc], hcx * hcx, hcy * hcy);
+                        P.CC[fr.pc], level_h2<T>(P));
   P.F[fr.pc] = R + Lc;
 }
 
@@ -378,11 +435,11 @@ template <typename T>
 __device__ __forceinline__ void reflux_item(const LevelDev<T>& L, const LevelDev<T>& P, int t, int ring_from_parent) {
   const int cell = P.tgt_cell[t];
   double acc = (double)P.F[cell];
-  const double hcx2 = P.h[0] * P.h[0], hcy2 = P.h[1] * P.h[1];
+  const H2<double> hc = level_h2_64<T>(P);
   for (int e = P.tgt_ptr[t]; e < P.tgt_ptr[t + 1]; ++e) {
     const CorrRec cr = L.corr[P.tgt_ent[e]];
     const double ring = ring_from_parent ? (double)P.Q[L.wmap[cr.ring]] : (double)L.QC[cr.ring];
-    acc = acc + corr_value<T>(cr, L.Q, ring, L.QC, L.DeW, L.DnS, hcx2, hcy2);
+    acc = acc + corr_value<T>(cr, L.Q, ring, L.QC, L.DeW, L.DnS, hc);
   }
   P.F[cell] = (T)acc;
 }
@@ -411,9 +468,8 @@ __device__ __forceinline__ void load_item(const LevelDev<T>& L, const LevelDev<T
     const T ve = fr.ci == cmx - 1 ? Pq[L.wmap[w + 1]] : QC[w + 1];
     const T vs = fr.cj == 0 ? Pq[L.wmap[w - ldc]] : QC[w - ldc];
     const T vn = fr.cj == cmy - 1 ? Pq[L.wmap[w + ldc]] : QC[w + ldc];
-    const T hcx = (T)P.h[0], hcy = (T)P.h[1];
     const T Lc = op5<T>(u, vw, ve, vs, vn, P.DeW[fr.dew_w], P.DeW[fr.dew_e], P.DnS[fr.dns_s], P.DnS[fr.dns_n],
-                        P.CC[fr.pc], hcx * hcx, hcy * hcy);
+                        P.CC[fr.pc], level_h2<T>(P));
     P.F[fr.pc] = R + Lc;
   } else if (t < L.nfoot + L.nring_all) {
     ring_item<T>(L, P.Q, t - L.nfoot);
@@ -430,7 +486,7 @@ __device__ __forceinline__ void correct_item(const LevelDev<T>& L, const T* __re
   const int mx = pi.mx, my = pi.my;
   const int c = tr.start + lane;
   if (c >= mx * my) return;
-  const int j = c / mx;
+  const int j = cell_row(c, mx, pi.magic);
   const int i = c - j * mx;
   const int ldc = mx / 2 + 2;
   const int w0 = pi.qcoff + ((i >> 1) + 1) + ((j >> 1) + 1) * ldc;
@@ -509,8 +565,20 @@ __global__ void __launch_bounds__(kTile) k_correct(LevelDev<T> L, const T* __res
 template <typename T>
 __device__ __forceinline__ void coarsest_row(const CoarsestDev& C, const double* rhs, int i, int n1,
                                              T* __restrict__ Q) {
+  // ascending-j accumulation as in the oracle; the loads of 16 columns are
+  // issued before their products are added so that they overlap
+  const double* __restrict__ a = C.Ainv + i;
+  const size_t M = (size_t)C.M;
   double acc = 0.0;
-  for (int j = 0; j < C.M; ++j) acc = acc + C.Ainv[i + (size_t)j * C.M] * rhs[j];
+  int j = 0;
+  for (; j + 16 <= C.M; j += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = a[(size_t)(j + u) * M];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc = acc + v[u] * rhs[j + u];
+  }
+  for (; j < C.M; ++j) acc = acc + a[(size_t)j * M] * rhs[j];
   const int ci = i % n1, cj = i / n1;
   Q[(ci + 1) + (cj + 1) * (n1 + 2)] = (T)acc;
 }
@@ -537,8 +605,13 @@ __device__ void blk_relax(const LevelDev<T>& L, const T* Pq, int nsweeps, T omeg
   for (int it = 0; it < nsweeps; ++it)
     for (int colour = 1; colour <= 2; ++colour) {
       const int inl = (it == 0 && colour == 1) ? first_inl : 1;
-      for (int x = threadIdx.x; x < ncell; x += blockDim.x)
-        relax_item<T>(L, Pq, x / kTile, x % kTile, colour, omega, homog, inl);
+      if (L.even_mx) {
+        for (int x = threadIdx.x; x < (ncell >> 1); x += blockDim.x)
+          relax_item_colour<T>(L, Pq, x, colour, omega, homog, inl);
+      } else {
+        for (int x = threadIdx.x; x < ncell; x += blockDim.x)
+          relax_item<T>(L, Pq, x / kTile, x % kTile, colour, omega, homog, inl);
+      }
       __syncthreads();
     }
   if (nsweeps > 0) {
@@ -561,8 +634,8 @@ template <typename T>
 __device__ void blk_vcycle(const LevelDev<T>* __restrict__ LV, int kc, const CoarsestDev& C, int npre,
                            int npost, T omega, int homog, int mass_corr, double* rhs) {
   for (int k = kc; k >= 2; --k) {
-    const LevelDev<T> L = LV[k - 1];
-    const LevelDev<T> P = LV[k - 2];
+    const LevelDev<T>& L = LV[k - 1];
+    const LevelDev<T>& P = LV[k - 2];
     blk_relax<T>(L, P.Q, npre, omega, homog, 0);
     for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
     __syncthreads();
@@ -572,7 +645,7 @@ __device__ void blk_vcycle(const LevelDev<T>* __restrict__ LV, int kc, const Coa
     __syncthreads();
   }
   {
-    const LevelDev<T> L1 = LV[0];
+    const LevelDev<T>& L1 = LV[0];
     for (int j = threadIdx.x; j < C.M; j += blockDim.x)
       rhs[j] = j < C.N ? (homog ? (double)L1.F[j] : (double)L1.F[j] + C.bcRhs[j]) : 0.0;
     __syncthreads();
@@ -581,7 +654,7 @@ __device__ void blk_vcycle(const LevelDev<T>* __restrict__ LV, int kc, const Coa
     blk_fill<T>(L1, (const T*)nullptr, homog);
   }
   for (int k = 2; k <= kc; ++k) {
-    const LevelDev<T> L = LV[k - 1];
+    const LevelDev<T>& L = LV[k - 1];
     const T* Pq = LV[k - 2].Q;
     const int ncell = L.ntile * kTile;
     for (int x = threadIdx.x; x < ncell; x += blockDim.x) correct_item<T>(L, Pq, x / kTile, x % kTile, 0);
@@ -594,12 +667,131 @@ __device__ void blk_vcycle(const LevelDev<T>* __restrict__ LV, int kc, const Coa
   }
 }
 
+__device__ __forceinline__ void blk_copy_words(void* dst, const void* src, long long nbytes) {
+  const uint32_t* __restrict__ a = (const uint32_t*)src;
+  uint32_t* __restrict__ b = (uint32_t*)dst;
+  const long long n = nbytes >> 2;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x) b[i] = a[i];
+}
+
+// Copy segments (src, dst, 4-byte words) as one workgroup-wide loop with
+// several loads in flight per thread, so that many small arrays cost about
+// one memory latency rather than one each.
+struct CopySeg {
+  const uint32_t* src;
+  uint32_t* dst;
+  int words;
+  int first;  // prefix sum of words over earlier segments
+};
+constexpr int kMaxCopySegs = 24 * kStageMaxLevels;
+
+__device__ void blk_copy_segments(const CopySeg* seg, int nseg, int total) {
+  constexpr int U = 8;
+  for (int base = threadIdx.x; base < total; base += U * (int)blockDim.x) {
+    uint32_t v[U];
+    int where[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int w = base + u * (int)blockDim.x;
+      where[u] = -1;
+      if (w < total) {
+        int lo = 0, hi = nseg - 1;  // last segment with first <= w
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (seg[mid].first <= w) lo = mid; else hi = mid - 1;
+        }
+        where[u] = lo;
+        v[u] = seg[lo].src[w - seg[lo].first];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int w = base + u * (int)blockDim.x;
+      if (where[u] >= 0) seg[where[u]].dst[w - seg[where[u]].first] = v[u];
+    }
+  }
+}
+
+// Copy the arrays of levels 1..kc into LDS (layout of stage_level_bytes) and
+// publish descriptors pointing at the copies in LS.
+template <typename T>
+__device__ void blk_stage_levels(const LevelDev<T>* __restrict__ LV, int kc, LevelDev<T>* LS, char* base,
+                                 CopySeg* seg, int* nseg_total) {
+  for (int k = 0; k < kc; ++k) blk_copy_words(&LS[k], &LV[k], (long long)sizeof(LevelDev<T>));
+  __syncthreads();
+  if (threadIdx.x == 0) {  // layout: one segment per staged array
+    long long off = 0;
+    int ns = 0, words = 0;
+    for (int k = 0; k < kc; ++k) {
+      const LevelDev<T>& L = LV[k];
+#define BSAM_STAGE_SEG(field, type, count)                             \
+  {                                                                    \
+    const long long nb = (long long)(count) * (long long)sizeof(type); \
+    if (nb > 0) {                                                      \
+      seg[ns] = CopySeg{(const uint32_t*)L.field, (uint32_t*)(base + off), (int)(nb >> 2), words}; \
+      words += (int)(nb >> 2);                                         \
+      LS[k].field = (type*)(base + off);                               \
+      off += stage_align(nb);                                          \
+      ++ns;                                                            \
+    }                                                                  \
+  }
+      BSAM_STAGE_FIELDS(BSAM_STAGE_SEG, L, T)
+      if (k + 1 < kc) {  // level k+1's reflux targets on level k
+        const int fine_ncorr = LV[k + 1].ncorr;
+        BSAM_STAGE_SEG(tgt_cell, int, L.ntgt)
+        BSAM_STAGE_SEG(tgt_ptr, int, L.ntgt + 1)
+        BSAM_STAGE_SEG(tgt_ent, int, fine_ncorr)
+      }
+#undef BSAM_STAGE_SEG
+    }
+    nseg_total[0] = ns;
+    nseg_total[1] = words;
+  }
+  __syncthreads();
+  blk_copy_segments(seg, nseg_total[0], nseg_total[1]);
+  __syncthreads();
+}
+
 template <typename T>
 __global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __restrict__ LV, int kc,
                                                        CoarsestDev C, int npre, int npost, T omega,
-                                                       int homog, int mass_corr) {
-  extern __shared__ double rhs[];
-  blk_vcycle<T>(LV, kc, C, npre, npost, omega, homog, mass_corr, rhs);
+                                                       int homog, int mass_corr, int staged) {
+  extern __shared__ double smem[];
+  double* rhs = smem;
+  if (!staged) {
+    blk_vcycle<T>(LV, kc, C, npre, npost, omega, homog, mass_corr, rhs);
+    return;
+  }
+  // levels 1..kc live in LDS for the whole cycle; the cycle writes Q, F and QC
+  __shared__ LevelDev<T> LS[kStageMaxLevels];
+  __shared__ CopySeg seg[kMaxCopySegs];
+  __shared__ int nseg_total[2];
+  char* base = reinterpret_cast<char*>(smem) + stage_align((long long)C.M * (long long)sizeof(double));
+  blk_stage_levels<T>(LV, kc, LS, base, seg, nseg_total);
+  blk_vcycle<T>(LS, kc, C, npre, npost, omega, homog, mass_corr, rhs);
+  __syncthreads();
+  if (threadIdx.x == 0) {  // write back what the cycle changes: Q, F, QC
+    int ns = 0, words = 0;
+    for (int k = 0; k < kc; ++k) {
+      const LevelDev<T>& G = LV[k];
+      const LevelDev<T>& S = LS[k];
+      const int wq = (int)((long long)G.nq * (long long)sizeof(T) >> 2);
+      const int wf = (int)((long long)G.ncell * (long long)sizeof(T) >> 2);
+      const int wc = (int)((long long)G.nqc * (long long)sizeof(T) >> 2);
+      seg[ns++] = CopySeg{(const uint32_t*)S.Q, (uint32_t*)G.Q, wq, words};
+      words += wq;
+      seg[ns++] = CopySeg{(const uint32_t*)S.F, (uint32_t*)G.F, wf, words};
+      words += wf;
+      if (wc) {
+        seg[ns++] = CopySeg{(const uint32_t*)S.QC, (uint32_t*)G.QC, wc, words};
+        words += wc;
+      }
+    }
+    nseg_total[0] = ns;
+    nseg_total[1] = words;
+  }
+  __syncthreads();
+  blk_copy_segments(seg, nseg_total[0], nseg_total[1]);
 }
 
 // ---------------------------------------------------------------- reductions
@@ -680,24 +872,23 @@ __device__ __forceinline__ void residual_item(const LevelDev<T>* __restrict__ LV
   const int cc = pi.coff + c;
   if (reset_f) L.F[cc] = L.FT[cc];
   if (lvl + 1 < k0) return;
-  const int j = c / mx;
+  const int j = cell_row(c, mx, pi.magic);
   const int i = c - j * mx;
   const int ldq = mx + 2;
   const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
   const int fx = pi.xoff + i + j * (mx + 1);
   const int fy = pi.yoff + i + j * mx;
-  const T hx = (T)L.h[0], hy = (T)L.h[1];
   T r = L.FT[cc] - op_at<T>(L.Q, qc, ldq, L.DeW[fx], L.DeW[fx + 1], L.DnS[fy], L.DnS[fy + mx], L.CC[cc],
-                            hx * hx, hy * hy);
+                            level_h2<T>(L));
   double rd = (double)r;
   if (mass_corr && lvl + 1 < nlev) {
     const int row = L.tgt_of_cell[cc];
     if (row >= 0) {
       const LevelDev<T>& Fn = LV[lvl + 1];
-      const double hcx2 = L.h[0] * L.h[0], hcy2 = L.h[1] * L.h[1];
+      const H2<double> hc = level_h2_64<T>(L);
       for (int e = L.tgt_ptr[row]; e < L.tgt_ptr[row + 1]; ++e) {
         const CorrRec cr = Fn.corr[L.tgt_ent[e]];
-        rd = rd + corr_value<T>(cr, Fn.Q, (double)Fn.QC[cr.ring], Fn.QC, Fn.DeW, Fn.DnS, hcx2, hcy2);
+        rd = rd + corr_value<T>(cr, Fn.Q, (double)Fn.QC[cr.ring], Fn.QC, Fn.DeW, Fn.DnS, hc);
       }
     }
   }
@@ -750,7 +941,7 @@ __global__ void __launch_bounds__(kTile) k_mass(LevelDev<T> L, ResidualOut* __re
   double s = 0.0;
   long long n = 0;
   if (c < mx * my && L.vis[pi.coff + c]) {
-    const int j = c / mx;
+    const int j = cell_row(c, mx, pi.magic);
     const int i = c - j * mx;
     s = (double)L.Q[pi.qoff + (i + 1) + (j + 1) * (mx + 2)];
     n = 1;
@@ -789,8 +980,11 @@ __device__ __forceinline__ void block_reduce4_wg(double a, double b, double c, l
 template <typename T>
 __device__ void blk_reset_force(const LevelDev<T>* __restrict__ LV, int nlev) {
   for (int k = 0; k < nlev; ++k) {
-    const LevelDev<T> L = LV[k];
-    for (int x = threadIdx.x; x < L.ncell; x += blockDim.x) L.F[x] = L.FT[x];
+    const LevelDev<T>& L = LV[k];
+    T* __restrict__ F = L.F;
+    const T* __restrict__ FT = L.FT;
+    const int n = L.ncell;
+    for (int x = threadIdx.x; x < n; x += blockDim.x) F[x] = FT[x];
   }
   __syncthreads();
 }
@@ -798,8 +992,8 @@ __device__ void blk_reset_force(const LevelDev<T>* __restrict__ LV, int nlev) {
 template <typename T>
 __device__ void blk_filldown(const LevelDev<T>* __restrict__ LV, int nlev, int homog) {
   for (int k = nlev; k >= 2; --k) {  // ebTransfer.m:73-79
-    const LevelDev<T> L = LV[k - 1];
-    const LevelDev<T> P = LV[k - 2];
+    const LevelDev<T>& L = LV[k - 1];
+    const LevelDev<T>& P = LV[k - 2];
     for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
     __syncthreads();
     blk_fill<T>(P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
@@ -812,7 +1006,7 @@ __device__ void blk_residual(const LevelDev<double>* __restrict__ LV, int nlev, 
                              float* const* rout, ResidualOut* out) {
   if (mass_corr) {
     for (int k = max(2, root + 1); k <= nlev; ++k) {
-      const LevelDev<double> L = LV[k - 1];
+      const LevelDev<double>& L = LV[k - 1];
       const double* Pq = LV[k - 2].Q;
       const int n = L.nring_all + L.nfoot;
       for (int x = threadIdx.x; x < n; x += blockDim.x) {
@@ -835,7 +1029,7 @@ __device__ void blk_residual(const LevelDev<double>* __restrict__ LV, int nlev, 
 __global__ void __launch_bounds__(1024) k_batch_cycle(const BatchItem* __restrict__ items, int npre, int npost,
                                                       double omega, int mass_corr, int mixed, int inner) {
   extern __shared__ double rhs[];
-  const BatchItem it = items[blockIdx.x];
+  const BatchItem& it = items[blockIdx.x];
   const LevelDev<double>* LV = it.LV64;
   const int nlev = it.nlev;
   if (it.phase == 0) {  // ebSolveCycles.m:11-19: fill every level, baseline residual
@@ -854,7 +1048,7 @@ __global__ void __launch_bounds__(1024) k_batch_cycle(const BatchItem* __restric
   // residual (which also writes the next FP32 right side)
   const LevelDev<float>* L32 = it.LV32;
   for (int k = 0; k < nlev; ++k) {
-    const LevelDev<float> L = L32[k];
+    const LevelDev<float>& L = L32[k];
     for (int x = threadIdx.x; x < L.nq; x += blockDim.x) L.Q[x] = 0.0f;
     for (int x = threadIdx.x; x < L.nqc; x += blockDim.x) L.QC[x] = 0.0f;
   }
@@ -865,14 +1059,14 @@ __global__ void __launch_bounds__(1024) k_batch_cycle(const BatchItem* __restric
     if (i + 1 < inner) blk_filldown<float>(L32, nlev, 1);
   }
   for (int k = 0; k < nlev; ++k) {
-    const LevelDev<double> L = LV[k];
+    const LevelDev<double>& L = LV[k];
     const float* E = L32[k].Q;
     const int ncell = L.ntile * kTile;
     for (int x = threadIdx.x; x < ncell; x += blockDim.x) {
       const TileRec& tr = L.tiles[x / kTile];
       const int c = tr.start + x % kTile;
       if (c >= tr.pi.mx * tr.pi.my) continue;
-      const int j = c / tr.pi.mx;
+      const int j = cell_row(c, tr.pi.mx, tr.pi.magic);
       const int ii = c - j * tr.pi.mx;
       const int qc = tr.pi.qoff + (ii + 1) + (j + 1) * (tr.pi.mx + 2);
       L.Q[qc] = L.Q[qc] + (double)E[qc];
@@ -892,7 +1086,7 @@ __global__ void __launch_bounds__(kTile) k_add_correction(LevelDev<double> L, co
   const int mx = pi.mx, my = pi.my;
   const int c = tr.start + (int)threadIdx.x;
   if (c >= mx * my) return;
-  const int j = c / mx;
+  const int j = cell_row(c, mx, pi.magic);
   const int i = c - j * mx;
   const int qc = pi.qoff + (i + 1) + (j + 1) * (mx + 2);
   L.Q[qc] = L.Q[qc] + (double)E[qc];
@@ -930,7 +1124,7 @@ __global__ void __launch_bounds__(kTile) k_history(LevelDev<double> L, double* _
   const int mx = pi.mx, my = pi.my;
   const int c = tr.start + (int)threadIdx.x;
   if (c >= mx * my) return;
-  const int j = c / mx;
+  const int j = cell_row(c, mx, pi.magic);
   const int i = c - j * mx;
   const double u = L.Q[pi.qoff + (i + 1) + (j + 1) * (mx + 2)];
   const int cc = pi.coff + c;
@@ -1009,10 +1203,21 @@ void launch_coarsest(const CoarsestDev& C, const LevelDev<T>& L1, int homog, hip
 }
 
 template <typename T>
-void launch_coarse_cycle(const LevelDev<T>* LV_dev, int kc, const CoarsestDev& C, int npre, int npost, T omega,
-                         int homog, int mass_corr, hipStream_t s) {
-  hipLaunchKernelGGL(k_coarse_cycle<T>, dim3(1), dim3(1024), C.M * sizeof(double), s, LV_dev, kc, C, npre, npost,
-                     omega, homog, mass_corr);
+void launch_coarse_cycle(const LevelDev<T>* LV_dev, const LevelDev<T>* LV_host, int kc, const CoarsestDev& C,
+                         int npre, int npost, T omega, int homog, int mass_corr, hipStream_t s) {
+  const long long rhs_b = stage_align((long long)C.M * (long long)sizeof(double));
+  long long stage_b = 0;
+  for (int k = 0; k < kc; ++k) stage_b += stage_level_bytes<T>(LV_host[k], k + 1 < kc ? LV_host[k + 1].ncorr : -1);
+  const int staged = (kc <= kStageMaxLevels && rhs_b + stage_b <= kStageLdsLimit) ? 1 : 0;
+  const size_t lds = staged ? (size_t)(rhs_b + stage_b) : (size_t)C.M * sizeof(double);
+  static bool raised = false;  // allow the full 160 KB of gfx950 LDS for dynamic allocations
+  if (!raised) {
+    (void)hipFuncSetAttribute((const void*)k_coarse_cycle<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kStageLdsLimit);
+    raised = true;
+  }
+  hipLaunchKernelGGL(k_coarse_cycle<T>, dim3(1), dim3(1024), lds, s, LV_dev, kc, C, npre, npost, omega, homog,
+                     mass_corr, staged);
 }
 
 template <typename T>
@@ -1085,7 +1290,8 @@ void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre,
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);            \
   template void launch_coarsest<T>(const CoarsestDev&, const LevelDev<T>&, int, hipStream_t);           \
-  template void launch_coarse_cycle<T>(const LevelDev<T>*, int, const CoarsestDev&, int, int, T, int, int, \
+  template void launch_coarse_cycle<T>(const LevelDev<T>*, const LevelDev<T>*, int, const CoarsestDev&, int, int, T, \
+                                       int, int, \
                                        hipStream_t);                                                    \
   template void launch_qc_refresh_all<T>(const LevelDev<T>*, const int2*, int, hipStream_t);            \
   template void launch_mass<T>(const LevelDev<T>&, ResidualOut*, hipStream_t);                          \

@@ -147,6 +147,22 @@ namespace {
 
 // ------------------------------------------------------------------ tables
 inline int qidx(const PatchInfo& p, int li, int lj) { return p.qoff + li + lj * (p.mx + 2); }
+
+// multiplier m with umulhi(c, m) == c / mx for every cell index c of an
+// mx-by-my patch (ceil(2^32/mx), verified), or 0 when none is exact
+unsigned row_magic(int mx, int my) {
+  const unsigned long long m = ((1ULL << 32) + (unsigned long long)mx - 1) / (unsigned long long)mx;
+  if (m > 0xffffffffULL) return 0;
+  for (long long c = 0; c < (long long)mx * my; ++c)
+    if ((long long)(((unsigned long long)c * m) >> 32) != c / mx) return 0;
+  return (unsigned)m;
+}
+
+// h is a power of two: then so is h^2 and q / h^2 == q * (1 / h^2) exactly
+bool is_pow2(double h) {
+  int e = 0;
+  return h > 0 && std::frexp(h, &e) == 0.5 && e > -60 && e < 60;
+}
 inline int cidx(const PatchInfo& p, int i, int j) { return p.coff + i + j * p.mx; }
 
 void make_tiles(HLevel& L) {
@@ -777,6 +793,12 @@ int upload_cast(bsam_hier* h, T** dst, const std::vector<double>& src) {
 template <typename T>
 int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, const LevelDev<double>* d64) {
   D.n[0] = L.n[0]; D.n[1] = L.n[1]; D.h[0] = L.h[0]; D.h[1] = L.h[1];
+  D.ih2[0] = 1.0 / (L.h[0] * L.h[0]);
+  D.ih2[1] = 1.0 / (L.h[1] * L.h[1]);
+  D.pow2 = is_pow2(L.h[0]) && is_pow2(L.h[1]) ? 1 : 0;
+  D.even_mx = 1;
+  for (const PatchInfo& P : L.pi)
+    if (P.mx % 2) D.even_mx = 0;
   D.np = L.np; D.ncell = L.ncell; D.nq = L.nq; D.nx = L.nx; D.ny = L.ny; D.nqc = L.nqc; D.ng = L.ng;
   D.ntile = (int)L.tiles.size(); D.nghost = (int)L.ghost.size();
   D.nring_all = (int)L.ring_all.size(); D.nring_cf = (int)L.ring_cf.size();
@@ -890,7 +912,7 @@ struct Seq {
   }
   void vcycle(int k, int kc) {  // ebVcycle.m:15-35, fused
     if (k <= kc && kc >= 2) {
-      launch_coarse_cycle<T>(LVd, k, h->cst, h->opts.npre, h->opts.npost, (T)h->opts.omega, homog,
+      launch_coarse_cycle<T>(LVd, L.data(), k, h->cst, h->opts.npre, h->opts.npost, (T)h->opts.omega, homog,
                              h->opts.mass_correction, s);
       return;
     }
@@ -997,6 +1019,7 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
         return fail(BSAM_E_BOX, "eb_make_level:box: invalid patch box.");
       PatchInfo P{};
       P.mx = b.ihi - b.ilo + 1; P.my = b.jhi - b.jlo + 1;
+      P.magic = row_magic(P.mx, P.my);
       if (k > 0 && (P.mx % 2 || P.my % 2))
         return fail(BSAM_E_ALIGN, "eb_build_ops:align: patch not aligned to parent cells.");
       if (P.mx < 2 || P.my < 2) return fail(BSAM_E_BOX, "eb_make_level:box: patch narrower than 2 cells.");
