@@ -237,6 +237,7 @@ def main():
     ap.add_argument("--coarse-cells", type=int, default=0)
     ap.add_argument("--sweep", action="store_true", help="tuning sweep of the coarse-cycle cut")
     ap.add_argument("--no-parabolic", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the per-hierarchy option search")
     ap.add_argument("--large-smoother", type=int, default=4096, metavar="N",
                     help="also time the smoother on one uniform N x N level (0: skip)")
     ap.add_argument("--no-serving", action="store_true")
@@ -299,6 +300,20 @@ def main():
                     log(f"sweep {prec} {sm} coarse_cells={cc}: {tot / args.steps * 1e3:.3f} ms/solve "
                         f"({st[-1]['vcycles']} cycles, rel {st[-1]['finalRel']:.2e})")
         return
+
+    tune = None
+    if not args.no_autotune:
+        from bsamgpu.autotune import autotune
+
+        best, table = autotune(G, bit_exact_only=False, repeats=3)
+        exact = [r for r in table if r["smoother"] == "exact" and r["precision"] == "fp64" and
+                 r["finalRel"] <= opts.RelTol]
+        pick = min(exact, key=lambda r: r["seconds"])
+        if args.coarse_cells == 0:
+            args.coarse_cells = pick["coarse_cells"]
+        tune = {"fastest": best, "fastest_bit_exact": {k: pick[k] for k in ("coarse_cells", "smoother", "precision")},
+                "table": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in table]}
+        log(f"autotune: fastest {best}, fastest bit-exact {tune['fastest_bit_exact']}")
 
     log("FP64 solves")
     tot64, lat64, st64 = time_solves(G, args.steps, args.warmup, "fp64", smoother=args.smoother, coarse_cells=args.coarse_cells)
@@ -372,6 +387,8 @@ def main():
                    "rel_tol": opts.RelTol},
         "vcycles": s64["vcycles"],
         "hierarchy_create_s": round(upload_s, 4),
+        "coarse_cells": args.coarse_cells,
+        "autotune": tune,
         "final_rel": s64["finalRel"],
         "vcycle_s": round(ms_step / 1e3 / s64["vcycles"], 7),
         "p50_ms": round(pct(lat64, 50), 3),

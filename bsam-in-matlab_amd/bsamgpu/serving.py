@@ -1,7 +1,9 @@
 """Serving many independent elliptic solves on one GPU with continuous batching.
 
 Each request is one ebSolve (EllipticBSAM/src/ebSolve.m:22-77): solve stages
-separated by host tagging and level construction.  The device advances every
+separated by host tagging and level construction, or one ebpSolve
+(ParabolicBSAM/src/ebpSolve.m:17-98): one stage per time step, with the BDF2
+history resident on the device between steps.  The device advances every
 request that is in a solve stage with one launch per tick
 (``bsam_batch_step``: one workgroup per hierarchy, its stage preamble or one
 whole cycle), so requests that arrive while others are on the GPU join the
@@ -26,7 +28,8 @@ from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
-from . import _lib, diagnostics, hierarchy, solve, tagcluster
+from . import _lib, diagnostics, hierarchy, parabolic, solve, tagcluster
+from .problem import BsamError
 from ._lib import check
 from .device import GpuHierarchy, context, solve_opts
 
@@ -128,54 +131,118 @@ class Request:
                 if len(hist) >= 2 else float("nan")}
 
 
-# ---- host steps (thread pool)
-def _prepare(req):
-    """ebSolve.m:22-36: hierarchy setup and upload; the first stage follows."""
-    req.H = hierarchy.setup(req.prob, req.opts)
-    req.G = GpuHierarchy(req.H)
-    if req.opts.Indicator.lower() == "fun":  # geometric tags: every level before the stage
-        while req.H.nlev - req.H.rootIdx + 1 < req.opts.MaxLevels:
-            boxes = tagcluster.tag_cluster(req.H)
-            if not boxes:
-                break
-            req.G.pull_q(req.H)
-            req.G.close()
-            kNew = hierarchy.add_level(req.H, boxes)
-            req.G = GpuHierarchy(req.H)
-            req.G.init_level_prolong(kNew)
-    req.bytes = device_bytes(req.H)
-    req.phase = 0
-    return req
+    # ---- host steps (run on the server's thread pool)
+    def prepare(self):
+        """ebSolve.m:22-36: hierarchy setup and upload; the first stage follows."""
+        self.H = hierarchy.setup(self.prob, self.opts)
+        self.G = GpuHierarchy(self.H)
+        if self.opts.Indicator.lower() == "fun":  # geometric tags: every level before the stage
+            while self.H.nlev - self.H.rootIdx + 1 < self.opts.MaxLevels:
+                boxes = tagcluster.tag_cluster(self.H)
+                if not boxes:
+                    break
+                self.G.pull_q(self.H)
+                self.G.close()
+                kNew = hierarchy.add_level(self.H, boxes)
+                self.G = GpuHierarchy(self.H)
+                self.G.init_level_prolong(kNew)
+        self.bytes = device_bytes(self.H)
+        self.phase = 0
+        return True
+
+    def after_stage(self):
+        """ebSolve.m:45-53: tag on the stage's solution and add a level, or
+        finish.  Returns True when another stage follows."""
+        self.stages.append(self.stage_record())
+        H, opts = self.H, self.opts
+        self.G.pull_q(H)
+        if opts.Indicator.lower() != "fun" and H.nlev - H.rootIdx + 1 < opts.MaxLevels:
+            boxes = tagcluster.tag_cluster(H)
+            if boxes:
+                self.G.close()
+                kNew = hierarchy.add_level(H, boxes)
+                self.G = GpuHierarchy(H)
+                self.G.init_level_prolong(kNew)
+                self.bytes = device_bytes(H)
+                self.phase = 0
+                return True
+        mz = opts.MeanZero  # ebSolve.m:56-68
+        if isinstance(mz, str) and mz.lower() == "auto":
+            mz = H.singular
+        if mz:
+            m, _ = diagnostics.composite_mean(H)
+            for k in range(1, H.nlev + 1):
+                for p in range(H.L(k).np):
+                    H.L(k).Q[p] = H.L(k).Q[p] - m
+        self.Q = [[q.copy() for q in H.L(k).Q] for k in range(1, H.nlev + 1)]
+        self.G.close()
+        self.G = None
+        return False
 
 
-def _after_stage(req):
-    """ebSolve.m:45-53: tag on the stage's solution and add a level, or finish.
-    Returns True when another stage follows."""
-    req.stages.append(req.stage_record())
-    H, opts = req.H, req.opts
-    req.G.pull_q(H)
-    if opts.Indicator.lower() != "fun" and H.nlev - H.rootIdx + 1 < opts.MaxLevels:
-        boxes = tagcluster.tag_cluster(H)
-        if boxes:
-            req.G.close()
-            kNew = hierarchy.add_level(H, boxes)
-            req.G = GpuHierarchy(H)
-            req.G.init_level_prolong(kNew)
-            req.bytes = device_bytes(H)
-            req.phase = 0
+class ParabolicRequest(Request):
+    """One served ebpSolve (ParabolicBSAM/src/ebpSolve.m:17-98): every time
+    step is one stage of the batch (rhs from the resident history, stage
+    preamble and cycles); history rotation, composite mass, coefficient
+    refresh and regrids run as host steps between stages."""
+
+    def __init__(self, rid, pp, popts, arrival=0.0):
+        super().__init__(rid, None, popts.eb, arrival)
+        self.pp = pp
+        self.popts = popts
+        self.S = None
+        self.n = 0
+        self.mass = []
+        self.nregrids = 0
+
+    def _begin_step(self):
+        """ebpSolve.m:41-49: source f(t1), implicit shift, right-hand side."""
+        self.n += 1
+        t1 = self.n * self.dt
+        scheme = self.scheme1 if self.n == 1 else self.popts.Scheme
+        parabolic._source(self.S, self.pp, t1)
+        parabolic.set_time(self.S, self.pp, t1, parabolic.ALPHA[scheme] / self.dt)
+        self.S.G.parabolic_rhs(self.dt, scheme)
+        self.G = self.S.G
+        self.H = self.S.H
+        self.phase = 0
+
+    def prepare(self):
+        po = self.popts
+        if po.Scheme == "cn":
+            raise BsamError("ebp_rhs:scheme", "scheme cn is not available on the device (be, bdf2)")
+        self.nsteps = max(1, parabolic._matlab_round(po.TFinal / po.Dt))
+        self.dt = po.TFinal / self.nsteps
+        self.scheme1 = "be" if po.Scheme == "bdf2" else po.Scheme
+        self.S = parabolic.ebp_init(self.pp, po, parabolic.ALPHA[self.scheme1] / self.dt)
+        m, a = self.S.G.composite_mean()
+        self.mass = [m * a]
+        self.bytes = device_bytes(self.S.H)
+        self._begin_step()
+        return True
+
+    def after_stage(self):
+        """ebpSolve.m:55-77: history, mass, regrid, next step or finish."""
+        st = self.stage_record()
+        self.stages.append(st)
+        G = self.S.G
+        G.parabolic_advance()
+        m, a = G.composite_mean()
+        self.mass.append(m * a)
+        po = self.popts
+        if po.RegridEvery > 0 and self.n % po.RegridEvery == 0 and self.n < self.nsteps and po.eb.MaxLevels > 1:
+            self.S = parabolic.ebp_regrid(self.S, self.pp, self.n * self.dt, parabolic.ALPHA[po.Scheme] / self.dt, po)
+            self.nregrids += 1
+            self.bytes = device_bytes(self.S.H)
+        if self.n < self.nsteps:
+            self._begin_step()
             return True
-    mz = opts.MeanZero  # ebSolve.m:56-68
-    if isinstance(mz, str) and mz.lower() == "auto":
-        mz = H.singular
-    if mz:
-        m, _ = diagnostics.composite_mean(H)
-        for k in range(1, H.nlev + 1):
-            for p in range(H.L(k).np):
-                H.L(k).Q[p] = H.L(k).Q[p] - m
-    req.Q = [[q.copy() for q in H.L(k).Q] for k in range(1, H.nlev + 1)]
-    req.G.close()
-    req.G = None
-    return False
+        self.S.G.pull_q(self.S.H)
+        self.H = self.S.H
+        self.Q = [[q.copy() for q in self.H.L(k).Q] for k in range(1, self.H.nlev + 1)]
+        self.S.G.close()
+        self.G = None
+        return False
 
 
 class Server:
@@ -239,8 +306,7 @@ class Server:
             fut.add_done_callback(host_done)
 
         def prep(req):
-            _prepare(req)
-            return True
+            return req.prepare()
 
         inflight = 0
         used = 0
@@ -286,7 +352,7 @@ class Server:
             keep = []
             for r, rr in zip(active, res):
                 if r.take(rr):
-                    submit(_after_stage, r)
+                    submit(lambda q: q.after_stage(), r)
                 else:
                     keep.append(r)
             active = keep

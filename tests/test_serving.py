@@ -146,3 +146,33 @@ def test_realtime_poisson_arrivals():
     st = serving.latency_stats(done)
     print(st, f"ticks {srv.ticks}, mean batch {srv.tick_items / max(srv.ticks, 1):.1f}", flush=True)
     assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+@pytest.mark.gpu
+def test_parabolic_requests_in_the_mix_match_ebp_solve():
+    """Parabolic requests (one stage per time step, BDF2 history resident)
+    served together with elliptic ones give exactly what ebp_solve gives."""
+    from bsamgpu import parabolic as par
+    from parabolic_cases import heat_readme, mms_bump
+
+    cases = {"heat": heat_readme(TFinal=0.02), "bump": mms_bump(TFinal=0.2)}
+    ref = {}
+    for name, (pp, po) in cases.items():
+        ref[name] = par.ebp_solve(pp, po)
+        ref[name]["device"].close()
+    reqs = [serving.ParabolicRequest(name, pp, po) for name, (pp, po) in cases.items()] + _requests(PICK[:4])
+    srv = serving.Server(slots=16)
+    done = {r.id: r for r in srv.run(reqs, realtime=False)}
+    srv.close()
+    for name in cases:
+        r = done[name]
+        assert r.error is None, r.error
+        sol = ref[name]
+        assert [s["vcycles"] for s in r.stages] == sol["cycles"]
+        assert [s["resHist"] for s in r.stages] == [s["resHist"] for s in sol["stages"]]
+        assert r.mass == sol["mass"], name
+        assert r.nregrids == sol["nregrids"]
+        H = sol["H"]
+        for k in range(1, H.nlev + 1):
+            for p in range(H.L(k).np):
+                assert np.array_equal(r.Q[k - 1][p], H.L(k).Q[p]), (name, k, p)
