@@ -44,8 +44,20 @@ enum GhostKind : int {
 struct GhostRec {
   int dst, kind;
   int s[5];
-  int gi;    // boundary-data index of c (G_DIR / G_NEU records only)
+  int gi;    // G_DIR / G_NEU: boundary-data index of c; G_COPY on a face: the
+             // sibling's ghost slot that copies this record's own boundary cell
+             // (the cell a smoother pass pushes its new value to); else -1
   double c;  // g (Dirichlet) or h*g (Neumann)
+};
+
+// per ghost record, what the smoother's neighbour-assembled passes need in
+// one 16-byte load: the parent part of a coarse-fine value (rewritten by
+// fills and keeping passes), the record kind and the sibling push target
+template <typename T>
+struct alignas(16) FaceAux {
+  T cf;
+  int kind;
+  int push;  // GhostRec::gi of a G_COPY face record, else -1
 };
 
 struct RingRec {
@@ -55,7 +67,8 @@ struct RingRec {
 // A 256-cell work tile with its patch record inline, so that a kernel reaches
 // every offset it needs with one (scalar) load instead of a tile -> patch
 // chain of dependent loads.
-struct TileRec {
+// 96 bytes, 16-aligned: a kernel reads a whole record with wide loads
+struct alignas(16) TileRec {
   PatchInfo pi;
   int p, start, pad0, pad1;
 };
@@ -94,6 +107,10 @@ struct LevelDev {
   double ih2[2];     // 1/h^2 (exact when pow2)
   int pow2;          // h[0], h[1] are powers of two: q/h^2 == q*(1/h^2) bit for bit
   int even_mx;       // every patch has an even width (colour-compacted passes)
+  int arith;         // face records follow the fixed interpolation stencil and
+                     // sibling faces carry push targets: passes after the first
+                     // of a relax call read materialised sibling ghosts and
+                     // evaluate coarse-fine ghosts from neighbours + cf
   int np, ncell, nq, nx, ny, nqc, ng;
   int ntile, nctile, nghost, nring_all, nring_cf, nfoot, ncorr, ntgt;
   int nstile, stile_lds;  // smoother tiles and their largest LDS region (cells)
@@ -113,6 +130,7 @@ struct LevelDev {
   int* wmap;         // per QC cell: parent Q index
   uint8_t* vis;      // per cell: 1 if not covered by the next level
   T *Q, *F, *FT, *CC, *ID, *DeW, *DnS, *QC, *G;
+  FaceAux<T>* aux;   // per ghost record (null: parent parts not kept)
 };
 
 struct CoarsestDev {

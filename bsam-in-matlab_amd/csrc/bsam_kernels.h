@@ -4,9 +4,18 @@
 
 namespace bsam {
 
-// one colour pass; inl != 0 evaluates internal-face ghosts from the records
+// one colour pass over a level.  Mode bits:
+//   ghost source (mode & 3): 0 the materialised ghost frame; 1 the ghost
+//     records; 2 sibling ghosts from the frame and coarse-fine ghosts from the
+//     cell's neighbours and the stored parent part (levels with L.arith);
+//   kKeepCf: the other colour's face cells store the parent parts of their
+//     coarse-fine records (and, in record mode, so do this colour's);
+//   kPush: after the update, write the new value into every sibling ghost
+//     slot that copies this cell.
+constexpr int kInlRecords = 1, kInlArith = 2, kKeepCf = 4, kPush = 8;
+
 template <typename T>
-void launch_relax(const LevelDev<T>& L, const LevelDev<T>* parent, int colour, T omega, int homog, int inl,
+void launch_relax(const LevelDev<T>& L, const LevelDev<T>* parent, int colour, T omega, int homog, int mode,
                   hipStream_t s);
 // all passes of one relax call, one patch per workgroup staged in LDS
 constexpr long long kTiledLdsBytes = 64 * 1024;

@@ -53,6 +53,28 @@ __device__ __forceinline__ int cell_row(int c, int mx, unsigned magic) {
 }
 
 // ---------------------------------------------------------------- ghosts
+__device__ __forceinline__ bool is_cf_kind(int kind) {
+  return kind == G_QIF || kind == G_QIF_END || kind == G_LIF || kind == G_LIF_CORNER;
+}
+
+// The parent part of a coarse-fine ghost value: the last-added term of the
+// QIF forms and the first term of the LIF forms, so a value assembled from a
+// stored copy of it rounds exactly as the whole expression does.
+template <typename T>
+__device__ __forceinline__ T cf_part(const GhostRec& r, const T* __restrict__ Pq) {
+  switch (r.kind) {
+    case G_QIF:
+      return T(A3_) * Pq[r.s[2]];
+    case G_QIF_END:
+      return T(A3_) * ((T(B1_) * Pq[r.s[2]] + T(B2_) * Pq[r.s[3]]) + T(B3_) * Pq[r.s[4]]);
+    case G_LIF:
+    case G_LIF_CORNER:
+      return T(C23_) * Pq[r.s[2]];
+    default:
+      return T(0);
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T ghost_value(const GhostRec& r, const T* __restrict__ Q,
                                          const T* __restrict__ Pq, int homog) {
@@ -60,14 +82,12 @@ __device__ __forceinline__ T ghost_value(const GhostRec& r, const T* __restrict_
     case G_COPY:
       return Q[r.s[0]];
     case G_QIF:
-      return (T(A1_) * Q[r.s[0]] + T(A2_) * Q[r.s[1]]) + T(A3_) * Pq[r.s[2]];
     case G_QIF_END:
-      return (T(A1_) * Q[r.s[0]] + T(A2_) * Q[r.s[1]]) +
-             T(A3_) * ((T(B1_) * Pq[r.s[2]] + T(B2_) * Pq[r.s[3]]) + T(B3_) * Pq[r.s[4]]);
+      return (T(A1_) * Q[r.s[0]] + T(A2_) * Q[r.s[1]]) + cf_part<T>(r, Pq);
     case G_LIF:
-      return (T(C23_) * Pq[r.s[2]] + T(C23_) * Q[r.s[0]]) - T(C13_) * Q[r.s[1]];
+      return (cf_part<T>(r, Pq) + T(C23_) * Q[r.s[0]]) - T(C13_) * Q[r.s[1]];
     case G_LIF_CORNER:
-      return T(C23_) * Pq[r.s[2]] + T(C13_) * Q[r.s[0]];
+      return cf_part<T>(r, Pq) + T(C13_) * Q[r.s[0]];
     case G_DIR: {
       const T g = homog ? T(0) : T(r.c);
       return (T(C83_) * g - T(2) * Q[r.s[0]]) + T(C13_) * Q[r.s[1]];
@@ -82,7 +102,9 @@ __device__ __forceinline__ T ghost_value(const GhostRec& r, const T* __restrict_
 }
 
 // item t of ebFillGhosts: t < nghost -> one ghost cell; then (when the level
-// has a parent) one QC ring refresh of a patch with coarse-fine faces.
+// has a parent) one QC ring refresh of a patch with coarse-fine faces.  A
+// coarse-fine record also stores its parent part in L.aux for the smoother
+// passes that assemble ghosts from neighbours.
 template <typename T>
 __device__ __forceinline__ void fill_item(const LevelDev<T>& L, const T* __restrict__ Pq, int t,
                                           int homog) {
@@ -96,6 +118,7 @@ __device__ __forceinline__ void fill_item(const LevelDev<T>& L, const T* __restr
       v = (a + b) - L.Q[r.s[2]];
     } else {
       v = ghost_value<T>(r, L.Q, Pq, homog);
+      if (L.aux && is_cf_kind(r.kind)) L.aux[t].cf = cf_part<T>(r, Pq);
     }
     L.Q[r.dst] = v;
   } else if (Pq && t < L.nghost + L.nring_cf) {
@@ -110,76 +133,154 @@ __device__ __forceinline__ int fill_items(const LevelDev<T>& L, const T* Pq) {
 }
 
 // ---------------------------------------------------------------- smoother
-// One cell of a colour pass (ebKernels.m:37-95).  With inl != 0 the
-// internal-face ghost values are evaluated from the ghost records instead of
-// being read from the materialised ghost frame: inside one relax call they
-// equal what ebFillGhosts would have written after the previous pass, since a
-// colour-c cell's ghost depends only on other-colour cells, on the cell
-// itself (read before it is written) and on the parent level.
+// relax pass modes: kInlRecords / kInlArith / kKeepCf / kPush (bsam_kernels.h)
+
+// parent part of a face cell's coarse-fine record, for the passes that
+// assemble the value from neighbours
+template <typename T>
+__device__ __forceinline__ void keep_face_cf(const LevelDev<T>& L, const T* __restrict__ Pq, int idx) {
+  const GhostRec r = L.ghost[idx];
+  if (is_cf_kind(r.kind)) L.aux[idx].cf = cf_part<T>(r, Pq);
+}
+
+template <typename T>
+__device__ __forceinline__ T record_ghost(const LevelDev<T>& L, const T* __restrict__ Pq, int idx, int homog,
+                                          int keep) {
+  const GhostRec r = L.ghost[idx];
+  if (keep && is_cf_kind(r.kind)) L.aux[idx].cf = cf_part<T>(r, Pq);
+  return ghost_value<T>(r, L.Q, Pq, homog);
+}
+
+// Coarse-fine ghost of the face cell x at 1-based position u along a face of
+// length len, from x's own value, its inward neighbour, its neighbours along
+// the face (am: u-1, ap: u+1), the cell two inward-along steps away (diag: at
+// u+2 for even u, u-2 for odd u) and the parent part cf.  These are exactly
+// the cells the interpolation record names (ebInterpCf.m:45-121), in the
+// record's operation order.
+template <typename T>
+__device__ __forceinline__ T cf_face_ghost(int kind, int u, T self, T inward, T am, T ap, T diag, T cf) {
+  const bool even = (u & 1) == 0;
+  switch (kind) {
+    case G_QIF:
+      return (T(A1_) * diag + T(A2_) * (even ? ap : am)) + cf;
+    case G_QIF_END:
+      return (T(A1_) * inward + T(A2_) * self) + cf;
+    default:  // G_LIF
+      return (cf + T(C23_) * self) - T(C13_) * (even ? am : ap);
+  }
+}
+
+// along-face offset (in cells) of the diagonal interpolation source
+__device__ __forceinline__ int cf_diag_step(int u, int len) {
+  return (u == 1 || u == len) ? 0 : ((u & 1) ? -2 : 2);
+}
+
+// a tile record as one block of wide loads (one round trip for every field)
+__device__ __forceinline__ TileRec load_tile(const TileRec* tiles, int idx) {
+  TileRec r;
+  __builtin_memcpy(&r, tiles + idx, sizeof(TileRec));
+  return r;
+}
+
+// One cell of a colour pass (ebKernels.m:37-95).  Inside one relax call the
+// ghost a colour-c cell reads depends only on other-colour cells, on the cell
+// itself (read before it is written) and on the parent level, so every ghost
+// source mode yields what ebFillGhosts would have written after the previous
+// pass.  Every load the cell needs is issued before any value is used, so a
+// pass costs one memory round trip after the tile record (two in record
+// mode, whose sources come from the records).
 template <typename T>
 __device__ __forceinline__ void relax_cell(const LevelDev<T>& L, const T* __restrict__ Pq, const PatchInfo& pi,
-                                           int i, int j, T omega, int homog, int inl) {
+                                           int i, int j, T omega, int homog, int mode) {
   const int mx = pi.mx, my = pi.my;
   T* __restrict__ Q = L.Q;
   const int ldq = mx + 2;
   const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
-  T qe = Q[qc + 1], qw = Q[qc - 1], qn = Q[qc + ldq], qs = Q[qc - ldq];
-  const T hx = (T)L.h[0], hy = (T)L.h[1];
-  // partial physical ghosts: Dirichlet (8/3)g + (1/3)u_2, Neumann h g
-  if (i == 0) {
-    if (pi.bc[0]) {
-      const T g = homog ? T(0) : L.G[pi.goff + j];
-      qw = pi.bc[0] == 1 ? T(C83_) * g + T(C13_) * Q[qc + 1] : hx * g;
-    } else if (inl) {
-      qw = ghost_value<T>(L.ghost[pi.gbase + j], Q, Pq, homog);
-    }
-  }
-  if (i == mx - 1) {
-    if (pi.bc[1]) {
-      const T g = homog ? T(0) : L.G[pi.goff + my + j];
-      qe = pi.bc[1] == 1 ? T(C83_) * g + T(C13_) * Q[qc - 1] : hx * g;
-    } else if (inl) {
-      qe = ghost_value<T>(L.ghost[pi.gbase + my + j], Q, Pq, homog);
-    }
-  }
-  if (j == 0) {
-    if (pi.bc[2]) {
-      const T g = homog ? T(0) : L.G[pi.goff + 2 * my + i];
-      qs = pi.bc[2] == 1 ? T(C83_) * g + T(C13_) * Q[qc + ldq] : hy * g;
-    } else if (inl) {
-      qs = ghost_value<T>(L.ghost[pi.gbase + 2 * my + i], Q, Pq, homog);
-    }
-  }
-  if (j == my - 1) {
-    if (pi.bc[3]) {
-      const T g = homog ? T(0) : L.G[pi.goff + 2 * my + mx + i];
-      qn = pi.bc[3] == 1 ? T(C83_) * g + T(C13_) * Q[qc - ldq] : hy * g;
-    } else if (inl) {
-      qn = ghost_value<T>(L.ghost[pi.gbase + 2 * my + mx + i], Q, Pq, homog);
-    }
-  }
   const int cc = pi.coff + i + j * mx;
   const int fx = pi.xoff + i + j * (mx + 1);
   const int fy = pi.yoff + i + j * mx;
+  const int src = mode & 3;
+  // internal faces this cell lies on, and their records
+  const bool fw = i == 0 && !pi.bc[0], fe = i == mx - 1 && !pi.bc[1];
+  const bool fs = j == 0 && !pi.bc[2], fn = j == my - 1 && !pi.bc[3];
+  const int iw = pi.gbase + j, ie = pi.gbase + my + j;
+  const int is = pi.gbase + 2 * my + i, in_ = pi.gbase + 2 * my + mx + i;
+
+  const T re = Q[qc + 1], rw = Q[qc - 1], rn = Q[qc + ldq], rs = Q[qc - ldq];
+  const T f = L.F[cc], de = L.DeW[fx + 1], dw = L.DeW[fx], dn = L.DnS[fy + mx], ds = L.DnS[fy];
+  const T id = L.ID[cc];
+  const T self = (omega != T(1) || src == kInlArith) ? Q[qc] : T(0);
+  // boundary data: every patch stores all four sides, so these are in bounds
+  T gw = T(0), ge = T(0), gs = T(0), gn = T(0);
+  if (!homog) {
+    if (pi.bc[0]) gw = L.G[pi.goff + j];
+    if (pi.bc[1]) ge = L.G[pi.goff + my + j];
+    if (pi.bc[2]) gs = L.G[pi.goff + 2 * my + i];
+    if (pi.bc[3]) gn = L.G[pi.goff + 2 * my + mx + i];
+  }
+  // face records of the assembled and pushing passes, and the diagonal
+  // interpolation sources
+  FaceAux<T> aw{}, ae{}, as{}, an{};
+  if (src == kInlArith || (mode & kPush)) {
+    if (fw) aw = L.aux[iw];
+    if (fe) ae = L.aux[ie];
+    if (fs) as = L.aux[is];
+    if (fn) an = L.aux[in_];
+  }
+  T xw = T(0), xe = T(0), xs = T(0), xn = T(0);
+  if (src == kInlArith) {
+    if (fw) xw = Q[qc + 1 + cf_diag_step(j + 1, my) * ldq];
+    if (fe) xe = Q[qc - 1 + cf_diag_step(j + 1, my) * ldq];
+    if (fs) xs = Q[qc + ldq + cf_diag_step(i + 1, mx)];
+    if (fn) xn = Q[qc - ldq + cf_diag_step(i + 1, mx)];
+  }
+
+  T qe = re, qw = rw, qn = rn, qs = rs;
+  const T hx = (T)L.h[0], hy = (T)L.h[1];
+  // partial physical ghosts: Dirichlet (8/3)g + (1/3)u_2, Neumann h g
+  if (i == 0 && pi.bc[0]) qw = pi.bc[0] == 1 ? T(C83_) * gw + T(C13_) * re : hx * gw;
+  if (i == mx - 1 && pi.bc[1]) qe = pi.bc[1] == 1 ? T(C83_) * ge + T(C13_) * rw : hx * ge;
+  if (j == 0 && pi.bc[2]) qs = pi.bc[2] == 1 ? T(C83_) * gs + T(C13_) * rn : hy * gs;
+  if (j == my - 1 && pi.bc[3]) qn = pi.bc[3] == 1 ? T(C83_) * gn + T(C13_) * rs : hy * gn;
+  if (src == kInlRecords) {
+    const int keep = mode & kKeepCf;
+    if (fw) qw = record_ghost<T>(L, Pq, iw, homog, keep);
+    if (fe) qe = record_ghost<T>(L, Pq, ie, homog, keep);
+    if (fs) qs = record_ghost<T>(L, Pq, is, homog, keep);
+    if (fn) qn = record_ghost<T>(L, Pq, in_, homog, keep);
+  } else if (src == kInlArith) {
+    if (fw && aw.kind != G_COPY) qw = cf_face_ghost<T>(aw.kind, j + 1, self, re, rs, rn, xw, aw.cf);
+    if (fe && ae.kind != G_COPY) qe = cf_face_ghost<T>(ae.kind, j + 1, self, rw, rs, rn, xe, ae.cf);
+    if (fs && as.kind != G_COPY) qs = cf_face_ghost<T>(as.kind, i + 1, self, rn, rw, re, xs, as.cf);
+    if (fn && an.kind != G_COPY) qn = cf_face_ghost<T>(an.kind, i + 1, self, rs, rw, re, xn, an.cf);
+  }
   const T hx2 = hx * hx, hy2 = hy * hy;
   const T ihx2 = (T)L.ih2[0], ihy2 = (T)L.ih2[1];
   const int p2 = L.pow2;
-  T tmp = L.F[cc] + div_h2<T>(qe * L.DeW[fx + 1], hx2, ihx2, p2);
-  tmp = tmp + div_h2<T>(qw * L.DeW[fx], hx2, ihx2, p2);
-  tmp = tmp + div_h2<T>(qn * L.DnS[fy + mx], hy2, ihy2, p2);
-  tmp = tmp + div_h2<T>(qs * L.DnS[fy], hy2, ihy2, p2);
-  tmp = tmp * L.ID[cc];
-  if (omega == T(1)) {
-    Q[qc] = tmp;
-  } else {
-    Q[qc] = omega * tmp + (T(1) - omega) * Q[qc];
+  T tmp = f + div_h2<T>(qe * de, hx2, ihx2, p2);
+  tmp = tmp + div_h2<T>(qw * dw, hx2, ihx2, p2);
+  tmp = tmp + div_h2<T>(qn * dn, hy2, ihy2, p2);
+  tmp = tmp + div_h2<T>(qs * ds, hy2, ihy2, p2);
+  tmp = tmp * id;
+  const T y = omega == T(1) ? tmp : omega * tmp + (T(1) - omega) * self;
+  Q[qc] = y;
+  if (mode & kPush) {
+    // sibling ghosts that copy this cell; their readers are other-colour
+    // cells, which this pass does not visit
+    if (fw && aw.push >= 0) Q[aw.push] = y;
+    if (fe && ae.push >= 0) Q[ae.push] = y;
+    if (fs && as.push >= 0) Q[as.push] = y;
+    if (fn && an.push >= 0) Q[an.push] = y;
   }
 }
 
+// one lane of a colour pass over a 256-cell tile (tile: uniform across the
+// wave); in a pass that keeps parent parts, the other colour's face cells
+// store those of their records for the pass that follows
 template <typename T>
 __device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __restrict__ Pq, int tile,
-                                           int lane, int colour, T omega, int homog, int inl) {
-  const TileRec& tr = L.tiles[tile];
+                                           int lane, int colour, T omega, int homog, int mode) {
+  const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane(tile));
   const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
   const int c = tr.start + lane;
@@ -187,18 +288,27 @@ __device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __rest
   const int j = cell_row(c, mx, pi.magic);
   const int i = c - j * mx;
   // (li + lj + par0) mod 2 == colour-1 with 1-based local indices
-  if (((i + j + pi.par0) & 1) != colour - 1) return;
-  relax_cell<T>(L, Pq, pi, i, j, omega, homog, inl);
+  if (((i + j + pi.par0) & 1) != colour - 1) {
+    if (mode & kKeepCf) {
+      if (i == 0 && !pi.bc[0]) keep_face_cf<T>(L, Pq, pi.gbase + j);
+      if (i == mx - 1 && !pi.bc[1]) keep_face_cf<T>(L, Pq, pi.gbase + my + j);
+      if (j == 0 && !pi.bc[2]) keep_face_cf<T>(L, Pq, pi.gbase + 2 * my + i);
+      if (j == my - 1 && !pi.bc[3]) keep_face_cf<T>(L, Pq, pi.gbase + 2 * my + mx + i);
+    }
+    return;
+  }
+  relax_cell<T>(L, Pq, pi, i, j, omega, homog, mode);
 }
 
 // Item t of a colour pass over colour-c cells only: the t-th cell of colour c
 // in tile order.  A tile of 256 cells of a patch with even mx holds 128 cells
 // of each colour (every row has mx/2 of them and tiles start on even cells),
-// so half the lanes cover a pass.  Returns false for padding items.
+// so half the lanes cover a pass.  The 64 items of a wave (t aligned to 64)
+// lie in one tile, whose record is therefore read once per wave.
 template <typename T>
 __device__ __forceinline__ void relax_item_colour(const LevelDev<T>& L, const T* __restrict__ Pq, int t,
                                                   int colour, T omega, int homog, int inl) {
-  const TileRec& tr = L.tiles[t >> 7];
+  const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane(t >> 7));
   const PatchInfo& pi = tr.pi;
   const int mx = pi.mx;
   const int a = (tr.start >> 1) + (t & 127);  // index among the patch's colour-c cells
@@ -512,10 +622,21 @@ __device__ __forceinline__ void correct_item(const LevelDev<T>& L, const T* __re
 }
 
 // ---------------------------------------------------------------- kernels
-template <typename T>
+// one colour pass, the mode fixed at compile time so that every variant is
+// straight-line code with all of a cell's loads issued together
+template <typename T, int MODE>
 __global__ void __launch_bounds__(kTile) k_relax(LevelDev<T> L, const T* __restrict__ Pq, int colour,
-                                                 T omega, int homog, int inl) {
-  relax_item<T>(L, Pq, blockIdx.x, threadIdx.x, colour, omega, homog, inl);
+                                                 T omega, int homog) {
+  relax_item<T>(L, Pq, blockIdx.x, threadIdx.x, colour, omega, homog, MODE);
+}
+
+// the same over the colour's cells only (every patch of even width): half
+// the lanes, for passes in which the other colour has nothing to do
+template <typename T, int MODE>
+__global__ void __launch_bounds__(kTile) k_relax_c(LevelDev<T> L, const T* __restrict__ Pq, int colour,
+                                                   T omega, int homog) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if ((t >> 7) < L.ntile) relax_item_colour<T>(L, Pq, t, colour, omega, homog, MODE);
 }
 
 template <typename T>
@@ -724,6 +845,7 @@ __device__ void blk_stage_levels(const LevelDev<T>* __restrict__ LV, int kc, Lev
     int ns = 0, words = 0;
     for (int k = 0; k < kc; ++k) {
       const LevelDev<T>& L = LV[k];
+      LS[k].aux = nullptr;  // the workgroup passes evaluate ghosts from records
 #define BSAM_STAGE_SEG(field, type, count)                             \
   {                                                                    \
     const long long nb = (long long)(count) * (long long)sizeof(type); \
@@ -1135,12 +1257,31 @@ __global__ void __launch_bounds__(kTile) k_history(LevelDev<double> L, double* _
 // ---------------------------------------------------------------- launchers
 static inline unsigned nblk(long long n) { return (unsigned)((n + kTile - 1) / kTile); }
 
+template <typename T, int MODE>
+static void launch_relax_m(const LevelDev<T>& L, const T* Pq, int colour, T omega, int homog, hipStream_t s) {
+  if (L.even_mx && !(MODE & kKeepCf))
+    hipLaunchKernelGGL((k_relax_c<T, MODE>), dim3((L.ntile + 1) / 2), dim3(kTile), 0, s, L, Pq, colour, omega,
+                       homog);
+  else
+    hipLaunchKernelGGL((k_relax<T, MODE>), dim3(L.ntile), dim3(kTile), 0, s, L, Pq, colour, omega, homog);
+}
+
 template <typename T>
-void launch_relax(const LevelDev<T>& L, const LevelDev<T>* parent, int colour, T omega, int homog, int inl,
+void launch_relax(const LevelDev<T>& L, const LevelDev<T>* parent, int colour, T omega, int homog, int mode,
                   hipStream_t s) {
   if (!L.ntile) return;
-  hipLaunchKernelGGL(k_relax<T>, dim3(L.ntile), dim3(kTile), 0, s, L,
-                     parent ? (const T*)parent->Q : (const T*)nullptr, colour, omega, homog, inl);
+  const T* Pq = parent ? (const T*)parent->Q : (const T*)nullptr;
+  switch (mode) {
+#define BSAM_RELAX_MODE(m) \
+  case m: launch_relax_m<T, m>(L, Pq, colour, omega, homog, s); break;
+    BSAM_RELAX_MODE(0) BSAM_RELAX_MODE(kPush) BSAM_RELAX_MODE(kKeepCf) BSAM_RELAX_MODE(kKeepCf | kPush)
+    BSAM_RELAX_MODE(kInlRecords) BSAM_RELAX_MODE(kInlRecords | kPush)
+    BSAM_RELAX_MODE(kInlRecords | kKeepCf) BSAM_RELAX_MODE(kInlRecords | kKeepCf | kPush)
+    BSAM_RELAX_MODE(kInlArith) BSAM_RELAX_MODE(kInlArith | kPush)
+    BSAM_RELAX_MODE(kInlArith | kKeepCf) BSAM_RELAX_MODE(kInlArith | kKeepCf | kPush)
+#undef BSAM_RELAX_MODE
+    default: break;
+  }
 }
 
 template <typename T>

@@ -17,6 +17,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/bsamgpu.h"
@@ -262,6 +263,75 @@ GhostRec cf_rec(const PatchInfo& P, int side, int t, bool qif, Win win) {
   return r;
 }
 
+// Whether every face record of a level has the layout that the smoother's
+// neighbour-assembled passes assume: coarse-fine records name the cells of
+// cf_rec's stencils, and each sibling record's push target is a sibling ghost
+// that copies this patch's boundary cell.  Levels that fail keep evaluating
+// ghosts from the records in every pass.
+bool face_records_arith(const HLevel& L) {
+  std::unordered_map<int, int> by_dst;
+  for (int t = 0; t < (int)L.ghost.size(); ++t) by_dst[L.ghost[t].dst] = t;
+  std::vector<int> pushed(L.ghost.size(), 0), copy_face(L.ghost.size(), 0);
+  for (int p = 0; p < L.np; ++p) {
+    const PatchInfo& P = L.pi[p];
+    const int mx = P.mx, my = P.my;
+    if (mx < 2 || my < 2) return false;
+    int off = P.gbase;
+    for (int side = 0; side < 4; ++side) {
+      const int len = side < 2 ? my : mx;
+      auto q = [&](int depth, int along) {
+        switch (side) {
+          case 0: return qidx(P, depth, along);
+          case 1: return qidx(P, mx + 1 - depth, along);
+          case 2: return qidx(P, along, depth);
+          default: return qidx(P, along, my + 1 - depth);
+        }
+      };
+      for (int t = 0; t < len; ++t) {
+        if (off + t >= (int)L.ghost.size()) return false;
+        const GhostRec& r = L.ghost[off + t];
+        const int u = t + 1;
+        const bool even = (u & 1) == 0;
+        if (r.dst != q(0, u)) return false;
+        if (P.bc[side]) {
+          if (r.kind != G_DIR && r.kind != G_NEU) return false;
+          continue;
+        }
+        switch (r.kind) {
+          case G_COPY: {
+            const auto it = by_dst.find(r.gi);
+            if (it == by_dst.end()) return false;
+            const GhostRec& m = L.ghost[it->second];
+            if (m.kind != G_COPY || m.s[0] != q(1, u)) return false;
+            ++pushed[it->second];
+            copy_face[off + t] = 1;
+            break;
+          }
+          case G_QIF:
+            if (u == 1 || u == len) return false;
+            if (even ? (r.s[0] != q(2, u + 2) || r.s[1] != q(1, u + 1))
+                     : (r.s[0] != q(2, u - 2) || r.s[1] != q(1, u - 1)))
+              return false;
+            break;
+          case G_QIF_END:
+            if ((u != 1 && u != len) || r.s[0] != q(2, u) || r.s[1] != q(1, u)) return false;
+            break;
+          case G_LIF:
+            if (r.s[0] != q(1, u) || r.s[1] != (even ? q(1, u - 1) : q(1, u + 1))) return false;
+            break;
+          default:
+            return false;
+        }
+      }
+      off += len;
+    }
+  }
+  // every sibling face ghost has exactly one cell that pushes to it
+  for (size_t t = 0; t < L.ghost.size(); ++t)
+    if (pushed[t] != copy_face[t]) return false;
+  return true;
+}
+
 struct OwnerMap {
   int n0 = 0, n1 = 0;
   std::vector<int> o;  // p+1, 0 = none
@@ -393,6 +463,10 @@ int build_level_tables(HLevel& F, HLevel& C, bool qif) {
           GhostRec r{};
           r.kind = G_COPY;
           r.s[0] = qidx(F.pi[s - 1], gi - sb.ilo + 1, gj - sb.jlo + 1);
+          // the sibling's ghost slot on the far side of its boundary cell
+          // r.s[0] is where this patch's boundary cell is copied to
+          const int lds = F.pi[s - 1].mx + 2;
+          r.gi = side == 0 ? r.s[0] + 1 : side == 1 ? r.s[0] - 1 : side == 2 ? r.s[0] + lds : r.s[0] - lds;
           const GhostRec tmp = cf_rec(P, side, t, qif, win);  // for dst only
           r.dst = tmp.dst;
           F.ghost.push_back(r);
@@ -416,6 +490,7 @@ int build_level_tables(HLevel& F, HLevel& C, bool qif) {
     for (int c = 0; c < 4; ++c) {
       GhostRec r{};
       r.dst = cdst[c];
+      r.gi = -1;
       const int gi = cgi[c], gj = cgj[c];
       int klass;
       if (gi < 1 || gi > F.n[0] || gj < 1 || gj > F.n[1]) {
@@ -799,6 +874,7 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
   D.even_mx = 1;
   for (const PatchInfo& P : L.pi)
     if (P.mx % 2) D.even_mx = 0;
+  D.arith = tables_from64 ? d64->arith : (face_records_arith(L) ? 1 : 0);
   D.np = L.np; D.ncell = L.ncell; D.nq = L.nq; D.nx = L.nx; D.ny = L.ny; D.nqc = L.nqc; D.ng = L.ng;
   D.ntile = (int)L.tiles.size(); D.nghost = (int)L.ghost.size();
   D.nring_all = (int)L.ring_all.size(); D.nring_cf = (int)L.ring_cf.size();
@@ -837,6 +913,13 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
   if ((rc = upload_cast(h, &D.G, L.G))) return rc;
   std::vector<double> zqc(L.nqc, 0.0);
   if ((rc = upload_cast(h, &D.QC, zqc))) return rc;
+  std::vector<FaceAux<T>> aux(L.ghost.size());
+  for (size_t t = 0; t < L.ghost.size(); ++t) {
+    aux[t].cf = T(0);
+    aux[t].kind = L.ghost[t].kind;
+    aux[t].push = (L.ghost[t].kind == G_COPY && D.arith) ? L.ghost[t].gi : -1;
+  }
+  if ((rc = upload(h, &D.aux, aux))) return rc;
   return 0;
 }
 
@@ -871,8 +954,10 @@ struct Seq {
   const LevelDev<T>* par(int k) { return k > 1 ? &L[k - 2] : nullptr; }
   void fill(int k) { launch_fill<T>(lev(k), par(k), homog, s); }
   // ebRelaxLevel.m:28-38; first_inline: the first pass may also evaluate
-  // ghosts from the records (only when the preceding fill was skipped)
-  void relax(int k, int nsweeps, int first_inline = 0) {
+  // ghosts from the records (only when the preceding fill was skipped);
+  // cf_current: the level was filled after its parent last changed, so the
+  // parent parts that fill stored are those of the current parent
+  void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false) {
     if (nsweeps > 0 && h->opts.smoother == 1 && tiled_fits<T>(lev(k))) {
       // tiled smoother: pass 1 reads the materialised ghost frame
       if (first_inline) fill(k);
@@ -880,10 +965,25 @@ struct Seq {
       fill(k);
       return;
     }
+    // Pass 1 reads the materialised frame or, after a correction, the
+    // records.  On levels with the standard face layout every later pass
+    // assembles coarse-fine ghosts from neighbours and the parent parts of
+    // the current parent, and reads sibling ghosts that the previous pass
+    // pushed; other levels read the records.  The parent parts come from the
+    // preceding fill (cf_current), from the records pass, or from the
+    // other colour's face cells of passes 1 and 2, which store them for the
+    // pass that follows.
+    const bool ar = lev(k).arith && lev(k).aux;
     for (int it = 0; it < nsweeps; ++it)
       for (int colour = 1; colour <= 2; ++colour) {
-        const int inl = (it == 0 && colour == 1) ? first_inline : 1;
-        launch_relax<T>(lev(k), par(k), colour, (T)h->opts.omega, homog, inl, s);
+        const bool first = it == 0 && colour == 1, last = it == nsweeps - 1 && colour == 2;
+        int mode = first ? (first_inline ? kInlRecords : 0) : kInlRecords;
+        if (ar) {
+          if (!first) mode = kInlArith;
+          if (first_inline ? first : (!cf_current && it == 0)) mode |= kKeepCf;
+          if (!last) mode |= kPush;
+        }
+        launch_relax<T>(lev(k), par(k), colour, (T)h->opts.omega, homog, mode, s);
       }
     if (nsweeps > 0) fill(k);
   }
@@ -900,7 +1000,7 @@ struct Seq {
       coarsest();
       return;
     }
-    relax(k, h->opts.npre);
+    relax(k, h->opts.npre, 0, k < h->nlev);
     restrict_(k);
     fill(k - 1);
     ring(k);
@@ -908,7 +1008,7 @@ struct Seq {
     vcycle_ref(k - 1);
     correct(k);
     fill(k);
-    relax(k, h->opts.npost);
+    relax(k, h->opts.npost, 0, true);
   }
   void vcycle(int k, int kc) {  // ebVcycle.m:15-35, fused
     if (k <= kc && kc >= 2) {
@@ -920,7 +1020,8 @@ struct Seq {
       coarsest();
       return;
     }
-    relax(k, h->opts.npre);
+    // below the top level the fill in the previous step is current
+    relax(k, h->opts.npre, 0, k < h->nlev);
     restrict_(k);
     fill(k - 1);
     launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s);
@@ -1503,6 +1604,7 @@ long long bsam_export_size(const bsam_hier* h, int k, int what) {
     case 0: return (long long)L.klass.size();
     case 1: return k >= 2 ? (long long)L.nqc : 0;
     case 2: return 2 * (long long)L.corr.size();
+    case 3: return 1;
     default: return -1;
   }
 }
@@ -1526,6 +1628,9 @@ int bsam_export(const bsam_hier* h, int k, int what, long long* out) {
         out[2 * e] = L.corr_target[e];
         out[2 * e + 1] = L.corr[e].face;
       }
+      break;
+    case 3:
+      out[0] = face_records_arith(L) ? 1 : 0;
       break;
   }
   return 0;
@@ -1563,12 +1668,13 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
   HIPCHK(hipEventRecord(e0, h->stream));
   for (int i = 0; i < nlaunch; ++i) {
     const int colour = 1 + (i & 1);
-    // inline ghosts, as in every pass after the first of a relax call
+    // the form of the passes between the first and the last of a relax call
+    const int mode = h->d64[k - 1].arith ? (kInlArith | kPush) : kInlRecords;
     if (precision == BSAM_MIXED)
-      launch_relax<float>(h->d32[k - 1], k > 1 ? &h->d32[k - 2] : nullptr, colour, (float)h->opts.omega, 1, 1,
+      launch_relax<float>(h->d32[k - 1], k > 1 ? &h->d32[k - 2] : nullptr, colour, (float)h->opts.omega, 1, mode,
                           h->stream);
     else
-      launch_relax<double>(h->d64[k - 1], k > 1 ? &h->d64[k - 2] : nullptr, colour, h->opts.omega, 0, 1,
+      launch_relax<double>(h->d64[k - 1], k > 1 ? &h->d64[k - 2] : nullptr, colour, h->opts.omega, 0, mode,
                            h->stream);
   }
   HIPCHK(hipEventRecord(e1, h->stream));

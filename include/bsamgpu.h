@@ -185,7 +185,10 @@ int bsam_set_level(bsam_hier *h, int k, int field, const double *in);
  *           padded linear index + 1e6*parent patch), ebBuildOps.m:115-142
  * what = 2: reflux entries of level k in order: (target parent cell index
  *           in the packed level-(k-1) cell layout, face 0..3) pairs,
- *           ebBuildOps.m:158-166 */
+ *           ebBuildOps.m:158-166
+ * what = 3: one value, 1 when every face ghost record of level k has the
+ *           standard interpolation stencil and sibling push target, so that
+ *           smoother passes after the first assemble ghosts from neighbours */
 long long bsam_export_size(const bsam_hier *h, int k, int what);
 int bsam_export(const bsam_hier *h, int k, int what, long long *out);
 

@@ -121,6 +121,25 @@ def test_host_tables_match_oracle_descriptors(name):
         assert G.export(k, 0).tolist() == ref
 
 
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_face_records_have_the_neighbour_assembled_layout(name):
+    """Every level of the test layouts qualifies for the smoother passes that
+    assemble coarse-fine ghosts from neighbours and read pushed sibling
+    ghosts (the library checks each face record's stencil and push target)."""
+    H = CASES[name](seed=None)
+    G = _host_only(H)
+    assert [int(G.export(k, 3)[0]) for k in range(1, H.nlev + 1)] == [1] * H.nlev
+
+
+def test_starfish_face_records_have_the_neighbour_assembled_layout():
+    from bsamgpu import solve
+
+    prob, aux, g = configs.starfish_problem()
+    H = solve.build_hierarchy(prob, configs.starfish_options(g))
+    G = _host_only(H)
+    assert [int(G.export(k, 3)[0]) for k in range(1, H.nlev + 1)] == [1] * H.nlev
+
+
 def test_host_only_handle_rejects_device_calls():
     G = _host_only(CASES["plain"](seed=None))
     with pytest.raises(BsamError):
