@@ -29,7 +29,11 @@ template <typename T> void launch_restrict(const LevelDev<T>& L, const LevelDev<
 // ebCoarseLoad alone (QC ring as stored) and the fused ring + load + reflux
 template <typename T> void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
 template <typename T> void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
-template <typename T> void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s);
+// prep != 0 (levels with L.arith): also store the parent parts of the face
+// records and push corrected boundary values into sibling ghosts, for a
+// following smoother pass that assembles its ghosts (kInlArith)
+template <typename T>
+void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s, int prep = 0);
 template <typename T> void launch_coarsest(const CoarsestDev& C, const LevelDev<T>& L1, int homog, hipStream_t s);
 // V-cycle of levels 1..kc in one workgroup (LV_dev: device array of levels,
 // LV_host: the same descriptors on the host).  When the levels' fields and

@@ -175,11 +175,20 @@ __device__ __forceinline__ int cf_diag_step(int u, int len) {
   return (u == 1 || u == len) ? 0 : ((u & 1) ? -2 : 2);
 }
 
-// a tile record as one block of wide loads (one round trip for every field)
+// a tile record as one block of wide loads
 __device__ __forceinline__ TileRec load_tile(const TileRec* tiles, int idx) {
   TileRec r;
   __builtin_memcpy(&r, tiles + idx, sizeof(TileRec));
   return r;
+}
+
+// the tile fields a pass reads, requested together (see relax_args_early):
+// one round trip for the record
+__device__ __forceinline__ void tile_fields_early(const TileRec& t) {
+  const PatchInfo& p = t.pi;
+  asm volatile("" ::"s"(p.mx), "s"(p.my), "s"(p.par0), "s"(t.start), "s"(p.magic), "s"(p.gbase), "s"(p.qoff),
+               "s"(p.coff), "s"(p.xoff), "s"(p.yoff), "s"(p.goff));
+  asm volatile("" ::"s"(p.bc[0]), "s"(p.bc[1]), "s"(p.bc[2]), "s"(p.bc[3]));
 }
 
 // One cell of a colour pass (ebKernels.m:37-95).  Inside one relax call the
@@ -281,6 +290,7 @@ template <typename T>
 __device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __restrict__ Pq, int tile,
                                            int lane, int colour, T omega, int homog, int mode) {
   const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane(tile));
+  tile_fields_early(tr);
   const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
   const int c = tr.start + lane;
@@ -309,6 +319,7 @@ template <typename T>
 __device__ __forceinline__ void relax_item_colour(const LevelDev<T>& L, const T* __restrict__ Pq, int t,
                                                   int colour, T omega, int homog, int inl) {
   const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane(t >> 7));
+  tile_fields_early(tr);
   const PatchInfo& pi = tr.pi;
   const int mx = pi.mx;
   const int a = (tr.start >> 1) + (t & 127);  // index among the patch's colour-c cells
@@ -590,7 +601,7 @@ __device__ __forceinline__ void load_item(const LevelDev<T>& L, const LevelDev<T
 
 template <typename T>
 __device__ __forceinline__ void correct_item(const LevelDev<T>& L, const T* __restrict__ Pq, int tile,
-                                             int lane, int init) {
+                                             int lane, int init, int prep = 0) {
   const TileRec& tr = L.tiles[tile];
   const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
@@ -598,6 +609,19 @@ __device__ __forceinline__ void correct_item(const LevelDev<T>& L, const T* __re
   if (c >= mx * my) return;
   const int j = cell_row(c, mx, pi.magic);
   const int i = c - j * mx;
+  // prep: the first post-smoothing pass assembles its ghosts, so store the
+  // parent parts of this cell's coarse-fine face records (the parent is
+  // final for the level's post-smoothing) and, below, push the corrected
+  // value into the sibling ghosts that copy it
+  const bool fw = prep && i == 0 && !pi.bc[0], fe = prep && i == mx - 1 && !pi.bc[1];
+  const bool fs = prep && j == 0 && !pi.bc[2], fn = prep && j == my - 1 && !pi.bc[3];
+  const int iw = pi.gbase + j, ie = pi.gbase + my + j;
+  const int is = pi.gbase + 2 * my + i, in_ = pi.gbase + 2 * my + mx + i;
+  int pw = -1, pe = -1, ps = -1, pn = -1;
+  if (fw) { keep_face_cf<T>(L, Pq, iw); pw = L.aux[iw].push; }
+  if (fe) { keep_face_cf<T>(L, Pq, ie); pe = L.aux[ie].push; }
+  if (fs) { keep_face_cf<T>(L, Pq, is); ps = L.aux[is].push; }
+  if (fn) { keep_face_cf<T>(L, Pq, in_); pn = L.aux[in_].push; }
   const int ldc = mx / 2 + 2;
   const int w0 = pi.qcoff + ((i >> 1) + 1) + ((j >> 1) + 1) * ldc;
   const int dx = (i & 1) ? 1 : -1;
@@ -618,25 +642,44 @@ __device__ __forceinline__ void correct_item(const LevelDev<T>& L, const T* __re
   }
   const T p = (((T(9) * e_c + T(3) * e_x) + T(3) * e_y) + e_d) / T(16);
   const int qc = pi.qoff + (i + 1) + (j + 1) * (mx + 2);
-  L.Q[qc] = init ? p : L.Q[qc] + p;
+  const T y = init ? p : L.Q[qc] + p;
+  L.Q[qc] = y;
+  if (pw >= 0) L.Q[pw] = y;
+  if (pe >= 0) L.Q[pe] = y;
+  if (ps >= 0) L.Q[ps] = y;
+  if (pn >= 0) L.Q[pn] = y;
 }
 
 // ---------------------------------------------------------------- kernels
+// The level descriptor's fields a pass reads, requested together at kernel
+// entry: an empty asm statement that takes them as scalar operands makes the
+// compiler issue their kernel-argument loads at once instead of each after
+// the branch that first needs it (one round trip instead of several).
+template <typename T>
+__device__ __forceinline__ void relax_args_early(const LevelDev<T>& L) {
+  asm volatile("" ::"s"(L.tiles), "s"(L.Q), "s"(L.F), "s"(L.ID), "s"(L.DeW), "s"(L.DnS), "s"(L.G), "s"(L.aux));
+  asm volatile("" ::"s"(L.h[0]), "s"(L.h[1]), "s"(L.ih2[0]), "s"(L.ih2[1]), "s"(L.pow2), "s"(L.ghost));
+}
+
 // one colour pass, the mode fixed at compile time so that every variant is
 // straight-line code with all of a cell's loads issued together
 template <typename T, int MODE>
 __global__ void __launch_bounds__(kTile) k_relax(LevelDev<T> L, const T* __restrict__ Pq, int colour,
                                                  T omega, int homog) {
+  relax_args_early<T>(L);
+  asm volatile("" ::"s"(Pq), "s"(colour), "s"(omega), "s"(homog));
   relax_item<T>(L, Pq, blockIdx.x, threadIdx.x, colour, omega, homog, MODE);
 }
 
-// the same over the colour's cells only (every patch of even width): half
-// the lanes, for passes in which the other colour has nothing to do
+// the same over the colour's cells only (every patch of even width): one
+// 128-lane workgroup per tile, for passes in which the other colour has
+// nothing to do
 template <typename T, int MODE>
-__global__ void __launch_bounds__(kTile) k_relax_c(LevelDev<T> L, const T* __restrict__ Pq, int colour,
-                                                   T omega, int homog) {
-  const int t = blockIdx.x * kTile + threadIdx.x;
-  if ((t >> 7) < L.ntile) relax_item_colour<T>(L, Pq, t, colour, omega, homog, MODE);
+__global__ void __launch_bounds__(kTile / 2) k_relax_c(LevelDev<T> L, const T* __restrict__ Pq, int colour,
+                                                       T omega, int homog) {
+  relax_args_early<T>(L);
+  asm volatile("" ::"s"(Pq), "s"(colour), "s"(omega), "s"(homog));
+  relax_item_colour<T>(L, Pq, blockIdx.x * (kTile / 2) + threadIdx.x, colour, omega, homog, MODE);
 }
 
 template <typename T>
@@ -676,8 +719,8 @@ __global__ void __launch_bounds__(kTile) k_load(LevelDev<T> L, LevelDev<T> P, in
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_correct(LevelDev<T> L, const T* __restrict__ Pq, int init) {
-  correct_item<T>(L, Pq, blockIdx.x, threadIdx.x, init);
+__global__ void __launch_bounds__(kTile) k_correct(LevelDev<T> L, const T* __restrict__ Pq, int init, int prep) {
+  correct_item<T>(L, Pq, blockIdx.x, threadIdx.x, init, prep);
 }
 
 // ---------------------------------------------------------------- coarsest
@@ -797,38 +840,65 @@ __device__ __forceinline__ void blk_copy_words(void* dst, const void* src, long 
 
 // Copy segments (src, dst, 4-byte words) as one workgroup-wide loop with
 // several loads in flight per thread, so that many small arrays cost about
-// one memory latency rather than one each.
+// one memory latency rather than one each.  Each segment starts on a chunk of
+// 8 words (first is a multiple of 8) and both its ends are 16-byte aligned,
+// so a chunk inside a segment moves as two 16-byte loads and stores.
 struct CopySeg {
   const uint32_t* src;
   uint32_t* dst;
   int words;
-  int first;  // prefix sum of words over earlier segments
+  int first;  // chunk-aligned start in the concatenated word space
 };
 constexpr int kMaxCopySegs = 24 * kStageMaxLevels;
 
+__device__ __forceinline__ int copy_seg_words(int words) { return (words + 7) & ~7; }
+
+// last segment with first <= w: a fixed number of halving steps, so that the
+// searches of several chunks interleave
+__device__ __forceinline__ int copy_seg_find(const CopySeg* seg, int nseg, int w) {
+  int lo = 0;
+#pragma unroll
+  for (int step = 128; step > 0; step >>= 1) {
+    const int m = lo + step;
+    if (m < nseg && seg[m].first <= w) lo = m;
+  }
+  return lo;
+}
+
 __device__ void blk_copy_segments(const CopySeg* seg, int nseg, int total) {
-  constexpr int U = 8;
-  for (int base = threadIdx.x; base < total; base += U * (int)blockDim.x) {
-    uint32_t v[U];
-    int where[U];
+  static_assert(kMaxCopySegs <= 256, "copy_seg_find covers 256 segments");
+  constexpr int U = 4;
+  const int nchunk = total >> 3;
+  for (int base = threadIdx.x; base < nchunk; base += U * (int)blockDim.x) {
+    uint4 v0[U], v1[U];
+    int where[U], off[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int w = base + u * (int)blockDim.x;
+      const int c = base + u * (int)blockDim.x;
       where[u] = -1;
-      if (w < total) {
-        int lo = 0, hi = nseg - 1;  // last segment with first <= w
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (seg[mid].first <= w) lo = mid; else hi = mid - 1;
+      off[u] = 0;
+      if (c < nchunk) {
+        const int k = copy_seg_find(seg, nseg, c << 3);
+        where[u] = k;
+        off[u] = (c << 3) - seg[k].first;
+        if (off[u] + 8 <= seg[k].words) {
+          const uint4* q = reinterpret_cast<const uint4*>(seg[k].src + off[u]);
+          v0[u] = q[0];
+          v1[u] = q[1];
         }
-        where[u] = lo;
-        v[u] = seg[lo].src[w - seg[lo].first];
       }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int w = base + u * (int)blockDim.x;
-      if (where[u] >= 0) seg[where[u]].dst[w - seg[where[u]].first] = v[u];
+      if (where[u] < 0) continue;
+      const CopySeg& S = seg[where[u]];
+      if (off[u] + 8 <= S.words) {
+        uint4* d = reinterpret_cast<uint4*>(S.dst + off[u]);
+        d[0] = v0[u];
+        d[1] = v1[u];
+      } else {
+        for (int w = off[u]; w < S.words; ++w) S.dst[w] = S.src[w];  // the segment's last partial chunk
+      }
     }
   }
 }
@@ -851,7 +921,7 @@ __device__ void blk_stage_levels(const LevelDev<T>* __restrict__ LV, int kc, Lev
     const long long nb = (long long)(count) * (long long)sizeof(type); \
     if (nb > 0) {                                                      \
       seg[ns] = CopySeg{(const uint32_t*)L.field, (uint32_t*)(base + off), (int)(nb >> 2), words}; \
-      words += (int)(nb >> 2);                                         \
+      words += copy_seg_words((int)(nb >> 2));                         \
       LS[k].field = (type*)(base + off);                               \
       off += stage_align(nb);                                          \
       ++ns;                                                            \
@@ -878,7 +948,7 @@ template <typename T>
 __global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __restrict__ LV, int kc,
                                                        CoarsestDev C, int npre, int npost, T omega,
                                                        int homog, int mass_corr, int staged) {
-  extern __shared__ double smem[];
+  extern __shared__ __attribute__((aligned(16))) double smem[];
   double* rhs = smem;
   if (!staged) {
     blk_vcycle<T>(LV, kc, C, npre, npost, omega, homog, mass_corr, rhs);
@@ -901,12 +971,12 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __rest
       const int wf = (int)((long long)G.ncell * (long long)sizeof(T) >> 2);
       const int wc = (int)((long long)G.nqc * (long long)sizeof(T) >> 2);
       seg[ns++] = CopySeg{(const uint32_t*)S.Q, (uint32_t*)G.Q, wq, words};
-      words += wq;
+      words += copy_seg_words(wq);
       seg[ns++] = CopySeg{(const uint32_t*)S.F, (uint32_t*)G.F, wf, words};
-      words += wf;
+      words += copy_seg_words(wf);
       if (wc) {
         seg[ns++] = CopySeg{(const uint32_t*)S.QC, (uint32_t*)G.QC, wc, words};
-        words += wc;
+        words += copy_seg_words(wc);
       }
     }
     nseg_total[0] = ns;
@@ -1260,7 +1330,7 @@ static inline unsigned nblk(long long n) { return (unsigned)((n + kTile - 1) / k
 template <typename T, int MODE>
 static void launch_relax_m(const LevelDev<T>& L, const T* Pq, int colour, T omega, int homog, hipStream_t s) {
   if (L.even_mx && !(MODE & kKeepCf))
-    hipLaunchKernelGGL((k_relax_c<T, MODE>), dim3((L.ntile + 1) / 2), dim3(kTile), 0, s, L, Pq, colour, omega,
+    hipLaunchKernelGGL((k_relax_c<T, MODE>), dim3(L.ntile), dim3(kTile / 2), 0, s, L, Pq, colour, omega,
                        homog);
   else
     hipLaunchKernelGGL((k_relax<T, MODE>), dim3(L.ntile), dim3(kTile), 0, s, L, Pq, colour, omega, homog);
@@ -1332,9 +1402,10 @@ void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr,
 }
 
 template <typename T>
-void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s) {
+void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s, int prep) {
   if (!L.ntile) return;
-  hipLaunchKernelGGL(k_correct<T>, dim3(L.ntile), dim3(kTile), 0, s, L, (const T*)parent.Q, init);
+  hipLaunchKernelGGL(k_correct<T>, dim3(L.ntile), dim3(kTile), 0, s, L, (const T*)parent.Q, init,
+                     prep && L.arith && L.aux ? 1 : 0);
 }
 
 template <typename T>
@@ -1429,7 +1500,7 @@ void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre,
   template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);           \
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
-  template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);            \
+  template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int);            \
   template void launch_coarsest<T>(const CoarsestDev&, const LevelDev<T>&, int, hipStream_t);           \
   template void launch_coarse_cycle<T>(const LevelDev<T>*, const LevelDev<T>*, int, const CoarsestDev&, int, int, T, \
                                        int, int, \

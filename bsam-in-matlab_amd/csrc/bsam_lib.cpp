@@ -953,8 +953,9 @@ struct Seq {
   const LevelDev<T>& lev(int k) { return L[k - 1]; }
   const LevelDev<T>* par(int k) { return k > 1 ? &L[k - 2] : nullptr; }
   void fill(int k) { launch_fill<T>(lev(k), par(k), homog, s); }
-  // ebRelaxLevel.m:28-38; first_inline: the first pass may also evaluate
-  // ghosts from the records (only when the preceding fill was skipped);
+  // ebRelaxLevel.m:28-38; first_inline: 1 = the first pass evaluates ghosts
+  // from the records (the preceding fill was skipped), 2 = the preceding
+  // correction prepared the level for assembled ghosts (correct(k, true));
   // cf_current: the level was filled after its parent last changed, so the
   // parent parts that fill stored are those of the current parent
   void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false) {
@@ -974,13 +975,14 @@ struct Seq {
     // other colour's face cells of passes 1 and 2, which store them for the
     // pass that follows.
     const bool ar = lev(k).arith && lev(k).aux;
+    if (first_inline == 2 && !ar) first_inline = 1;
     for (int it = 0; it < nsweeps; ++it)
       for (int colour = 1; colour <= 2; ++colour) {
         const bool first = it == 0 && colour == 1, last = it == nsweeps - 1 && colour == 2;
         int mode = first ? (first_inline ? kInlRecords : 0) : kInlRecords;
         if (ar) {
-          if (!first) mode = kInlArith;
-          if (first_inline ? first : (!cf_current && it == 0)) mode |= kKeepCf;
+          if (!first || first_inline == 2) mode = kInlArith;
+          if (first_inline == 1 ? first : (first_inline == 0 && !cf_current && it == 0)) mode |= kKeepCf;
           if (!last) mode |= kPush;
         }
         launch_relax<T>(lev(k), par(k), colour, (T)h->opts.omega, homog, mode, s);
@@ -990,7 +992,7 @@ struct Seq {
   void restrict_(int k) { launch_restrict<T>(lev(k), lev(k - 1), 0, s); }
   void ring(int k) { launch_ring<T>(lev(k), lev(k - 1), s); }
   void coarse_load(int k) { launch_coarse_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s); }
-  void correct(int k) { launch_correct<T>(lev(k), lev(k - 1), 0, s); }
+  void correct(int k, bool prep = false) { launch_correct<T>(lev(k), lev(k - 1), 0, s, prep ? 1 : 0); }
   void coarsest() {
     launch_coarsest<T>(h->cst, lev(1), homog, s);
     fill(1);
@@ -1026,9 +1028,10 @@ struct Seq {
     fill(k - 1);
     launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s);
     vcycle(k - 1, kc);
-    correct(k);
+    const bool prep = h->opts.npost > 0 && lev(k).arith && lev(k).aux && h->opts.smoother != 1;
+    correct(k, prep);
     if (h->opts.npost > 0) {
-      relax(k, h->opts.npost, 1);
+      relax(k, h->opts.npost, prep ? 2 : 1);
     } else {
       fill(k);
     }
