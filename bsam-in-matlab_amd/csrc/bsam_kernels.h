@@ -25,6 +25,8 @@ void launch_relax_tiled(const LevelDev<T>& L, const LevelDev<T>* parent, int nsw
                         hipStream_t s);
 template <typename T> void launch_fill(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, hipStream_t s);
 template <typename T> void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
+// restrict k -> k-1 + fill k-1 for k = kfrom..2 in one workgroup
+template <typename T> void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int homog, hipStream_t s);
 template <typename T> void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s);
 // ebCoarseLoad alone (QC ring as stored) and the fused ring + load + reflux
 template <typename T> void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
@@ -68,7 +70,9 @@ template <typename T> void launch_qc_refresh_all(const LevelDev<T>* LV_dev, cons
 template <typename T, typename TO>
 void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, int ntl, int mass_corr,
                          ResidualOut* parts, TO* const* rout, int reset_f, int k0, hipStream_t s);
-void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s);
+// out_host (pinned host memory), when given, receives the result as well
+void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s,
+                   ResidualOut* out_host = nullptr);
 template <typename T> void launch_mass(const LevelDev<T>& L, ResidualOut* parts, hipStream_t s);
 void launch_add_correction(const LevelDev<double>& L, const LevelDev<float>& E, hipStream_t s);
 template <typename T> void launch_fill_value(T* a, long long n, T v, hipStream_t s);

@@ -831,6 +831,22 @@ __device__ void blk_vcycle(const LevelDev<T>* __restrict__ LV, int kc, const Coa
   }
 }
 
+// The bottom of the filldown (ebTransfer.m:73-79) for levels kfrom..1 in one
+// workgroup: restrict k -> k-1, fill k-1, for k = kfrom..2, with barriers
+// between the dependent steps (the small levels' steps would otherwise each
+// cost a launch).
+template <typename T>
+__global__ void __launch_bounds__(1024) k_filldown_small(const LevelDev<T>* __restrict__ LV, int kfrom,
+                                                          int homog) {
+  for (int k = kfrom; k >= 2; --k) {
+    const LevelDev<T>& L = LV[k - 1];
+    const LevelDev<T>& P = LV[k - 2];
+    for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
+    __syncthreads();
+    blk_fill<T>(P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
+  }
+}
+
 __device__ __forceinline__ void blk_copy_words(void* dst, const void* src, long long nbytes) {
   const uint32_t* __restrict__ a = (const uint32_t*)src;
   uint32_t* __restrict__ b = (uint32_t*)dst;
@@ -1004,6 +1020,27 @@ __device__ __forceinline__ long long wave_sum_ll(long long v) {
 }
 
 // block-level reduction of (max, sum, max, count) into out (thread 0 writes)
+__device__ __forceinline__ void block_reduce4(double a, double b, double c, long long n, ResidualOut* out);
+
+// the parts of a composite residual reduced in a fixed order by one
+// workgroup of kTile threads; out_host (pinned host memory) also receives
+// the result when given
+__device__ __forceinline__ void reduce_parts(const ResidualOut* parts, int nparts, ResidualOut* out,
+                                             ResidualOut* out_host) {
+  double a = 0.0, b = 0.0, f = 0.0;
+  long long n = 0;
+  for (int k = threadIdx.x; k < nparts; k += kTile) {
+    const ResidualOut p = parts[k];
+    a = fmax(a, p.linf);
+    b = b + p.l2sq;
+    f = fmax(f, p.fnorm);
+    n += p.ncell;
+  }
+  block_reduce4(a, b, f, n, out);
+  if (out_host && threadIdx.x == 0) *out_host = *out;
+}
+
+// block-level reduction of (max, sum, max, count) into out (thread 0 writes)
 __device__ __forceinline__ void block_reduce4(double a, double b, double c, long long n,
                                               ResidualOut* out) {
   __shared__ double sa[kTile / 64], sb[kTile / 64], sc[kTile / 64];
@@ -1109,18 +1146,14 @@ __global__ void __launch_bounds__(kTile) k_residual_all(const LevelDev<T>* __res
   block_reduce4(a, b, f, n, &part[blockIdx.x]);
 }
 
+// The parts reduced in a fixed order by one workgroup (a separate launch: the
+// kernel boundary orders the parts' writes more cheaply than device-scope
+// fences in every workgroup would); out_host (pinned host memory) also
+// receives the result when given, so no copy follows.
 __global__ void __launch_bounds__(kTile) k_reduce_parts(const ResidualOut* __restrict__ parts,
-                                                        int nparts, ResidualOut* __restrict__ out) {
-  double a = 0.0, b = 0.0, f = 0.0;
-  long long n = 0;
-  for (int k = threadIdx.x; k < nparts; k += kTile) {
-    const ResidualOut p = parts[k];
-    a = fmax(a, p.linf);
-    b = b + p.l2sq;
-    f = fmax(f, p.fnorm);
-    n += p.ncell;
-  }
-  block_reduce4(a, b, f, n, out);
+                                                        int nparts, ResidualOut* __restrict__ out,
+                                                        ResidualOut* __restrict__ out_host) {
+  reduce_parts(parts, nparts, out, out_host);
 }
 
 // composite mean: sum of visible u * da (ebCompositeMean.m:4-23)
@@ -1382,6 +1415,12 @@ void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s)
 }
 
 template <typename T>
+void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int homog, hipStream_t s) {
+  if (kfrom < 2) return;
+  hipLaunchKernelGGL(k_filldown_small<T>, dim3(1), dim3(1024), 0, s, LV_dev, kfrom, homog);
+}
+
+template <typename T>
 void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s) {
   if (!L.nfoot) return;
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(L.nfoot)), dim3(kTile), 0, s, L, parent.Q, qc_only);
@@ -1446,8 +1485,8 @@ void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, in
                      rout, reset_f, k0);
 }
 
-void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kTile), 0, s, parts, nparts, out);
+void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s, ResidualOut* out_host) {
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kTile), 0, s, parts, nparts, out, out_host);
 }
 
 template <typename T>
@@ -1501,6 +1540,7 @@ void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre,
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int);            \
+  template void launch_filldown_small<T>(const LevelDev<T>*, int, int, hipStream_t);            \
   template void launch_coarsest<T>(const CoarsestDev&, const LevelDev<T>&, int, hipStream_t);           \
   template void launch_coarse_cycle<T>(const LevelDev<T>*, const LevelDev<T>*, int, const CoarsestDev&, int, int, T, \
                                        int, int, \

@@ -123,7 +123,7 @@ struct bsam_hier {
   float* qc32all = nullptr;                // FP32 QC of all levels (contiguous)
   long long nq32all = 0, nqc32all = 0;
   ResidualOut* dres = nullptr;
-  ResidualOut* hres = nullptr;             // pinned
+  ResidualOut* hres = nullptr;             // pinned; the reduction writes it
   std::vector<double*> Q0;                 // uploaded initial guess per level
   double* Uo = nullptr;                    // parabolic history (packed cells, all levels)
   double* Uoo = nullptr;
@@ -934,6 +934,15 @@ int coarse_cut(const bsam_hier* h, int cells) {
   return kc;
 }
 
+// Filldown cut: levels 1..kb have at most kFilldownSmallCells cells each;
+// their restrict + fill steps (k = kb..2) run in one workgroup.
+constexpr int kFilldownSmallCells = 16384;
+int filldown_cut(const bsam_hier* h) {
+  int kb = 1;
+  while (kb < h->nlev && h->H[kb].ncell <= kFilldownSmallCells) ++kb;
+  return kb;
+}
+
 // The reference's call sequence as level-wide launches.  Three fusions keep
 // the arithmetic and the final state identical to the reference order:
 //  * inside one relax call, passes after the first evaluate internal-face
@@ -1036,11 +1045,15 @@ struct Seq {
       fill(k);
     }
   }
-  void filldown() {  // ebTransfer.m:73-79
-    for (int kk = h->nlev; kk >= 2; --kk) {
+  // ebTransfer.m:73-79; the steps of the levels with at most
+  // kFilldownSmallCells cells run as one single-workgroup launch
+  void filldown() {
+    const int kb = filldown_cut(h);
+    for (int kk = h->nlev; kk > kb; --kk) {
       restrict_(kk);
       fill(kk - 1);
     }
+    launch_filldown_small<T>(LVd, kb, homog, s);
   }
 };
 
@@ -1053,13 +1066,14 @@ void enqueue_reset_force(bsam_hier* h, const LevelDev<T>* LVd, hipStream_t s) {
 
 // composite residual (ebCompResidual.m:23-84) on the FP64 hierarchy: one
 // launch refreshes every QC (ring + interior), one evaluates every level's
-// residual with its reflux chunks, one reduces; rout (device array of per-level
-// FP32 pointers) receives r when given.
+// residual with its reflux chunks, one reduces the parts in a fixed order into
+// dres and the pinned hres; rout (device array of per-level FP32 pointers)
+// receives r when given.
 void enqueue_residual(bsam_hier* h, float* const* rout, hipStream_t s) {
   if (h->opts.mass_correction) launch_qc_refresh_all<double>(h->lv64, h->qc_items, h->nqc_items, s);
   launch_residual_all<double, float>(h->lv64, h->nlev, h->res_tiles, h->nres_tiles, h->opts.mass_correction,
                                      h->parts, rout, 0, h->root, s);
-  launch_reduce(h->parts, h->nres_tiles, h->dres, s);
+  launch_reduce(h->parts, h->nres_tiles, h->dres, s, h->hres);
 }
 
 }  // namespace
@@ -1381,8 +1395,8 @@ static double residual_rel(const ResidualOut& r) {
   return r.linf / std::max(r.fnorm, 2.2250738585072014e-308);
 }
 
+// after enqueue_residual, whose reduction wrote the scalars to hres
 static int read_residual(bsam_hier* h, bsam_residual* out) {
-  HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(hipStreamSynchronize(h->stream));
   if (out) {
     out->linf = h->hres->linf;
@@ -1409,7 +1423,6 @@ static int enqueue_cycle64(bsam_hier* h, hipStream_t s) {
   q.vcycle(h->nlev, coarse_cut(h, h->opts.coarse_cells));
   q.filldown();
   enqueue_residual(h, nullptr, s);
-  HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, s));
   return 0;
 }
 
@@ -1436,7 +1449,6 @@ static int enqueue_cycle_mixed(bsam_hier* h, hipStream_t s) {
   d.filldown();
   for (int k = 1; k <= h->nlev; ++k) d.fill(k);
   enqueue_residual(h, h->rout32, s);
-  HIPCHK(hipMemcpyAsync(h->hres, h->dres, sizeof(ResidualOut), hipMemcpyDeviceToHost, s));
   return 0;
 }
 
