@@ -312,7 +312,7 @@ def main():
         if args.coarse_cells == 0:
             args.coarse_cells = pick["coarse_cells"]
         tune = {"fastest": best, "fastest_bit_exact": {k: pick[k] for k in ("coarse_cells", "smoother", "precision")},
-                "table": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in table]}
+                "table": [{k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in r.items()} for r in table]}
         log(f"autotune: fastest {best}, fastest bit-exact {tune['fastest_bit_exact']}")
 
     log("FP64 solves")
@@ -403,8 +403,9 @@ def main():
                   "smoother_fp32_GBps": round(gbs32, 1)},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs64 / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic("k_relax_starfish_level7"),
-                     "kernel": f"k_relax<double> level {kf} ({H.L(kf).np} patches, "
+                     "traffic": pmc_traffic("k_relax_c_starfish_level7"),
+                     "kernel": f"k_relax_c<double,10> (colour pass, neighbour-assembled ghosts + sibling push: "
+                               f"passes 2..2n-1 of a relax call) level {kf} ({H.L(kf).np} patches, "
                                f"{sum(H.L(kf).dims(p)[0] * H.L(kf).dims(p)[1] for p in range(H.L(kf).np))} cells)",
                      "bytes_per_launch": sm_bytes, "us_per_launch": round(ms64 * 1e3, 3)},
         "cpu_baseline": base,
