@@ -170,6 +170,56 @@ def bench_parabolic():
             "fp64": r64, "mixed": rmx}
 
 
+def bench_varcoef_refined(steps=5, warmup=2):
+    """Variable-coefficient configuration with the ulap threshold lowered to
+    1e-6 so that the stages refine to the 2048^2 spacing (1.75M cells, a
+    1024^2 single-patch level): a bandwidth-bound hierarchy.  Times the
+    final stage's solve from zero (FP64 and FP32-inner), the smoother on its
+    largest level, and reports the mixed form's error against FP64."""
+    from bsamgpu import _lib, configs, solve
+    from bsamgpu.device import GpuHierarchy
+
+    with open(os.path.join(ROOT, "tests", "golden", "config2_params.json")) as f:
+        par = json.load(f)
+    t = time.perf_counter()
+    sol = solve.eb_solve(configs.varcoef_problem(par), configs.varcoef_refined_options())
+    build_s = time.perf_counter() - t
+    sol["device"].close()
+    H = sol["H"]
+    for k in range(1, H.nlev + 1):  # the final stage from the zero initial guess
+        for p in range(H.L(k).np):
+            H.L(k).Q[p][:] = 0.0
+    G = GpuHierarchy(H)
+    tot64, lat64, st64 = time_solves(G, steps, warmup, "fp64")
+    Q64 = {k: G.get_field(k, _lib.BSAM_Q) for k in range(1, H.nlev + 1)}
+    totmx, latmx, stmx = time_solves(G, steps, warmup, "mixed")
+    Qmx = {k: G.get_field(k, _lib.BSAM_Q) for k in range(1, H.nlev + 1)}
+    cells = [sum(H.L(k).dims(p)[0] * H.L(k).dims(p)[1] for p in range(H.L(k).np)) for k in range(1, H.nlev + 1)]
+    kb = max(range(1, H.nlev + 1), key=lambda k: cells[k - 1])
+    G.time_smoother(kb, 10)
+    ms64 = G.time_smoother(kb, 100)
+    G.time_smoother(kb, 10, mixed=True)
+    ms32 = G.time_smoother(kb, 100, mixed=True)
+    b64, b32 = G.smoother_bytes(kb, 8), G.smoother_bytes(kb, 4)
+    G.close()
+    return {"workload": "variable D, BaseCells 256, MaxLevels 4, ulap threshold 1e-6 (stated change from 1e-3, "
+                        "which tags nothing): final stage solved from zero to rel 1e-10",
+            "levels": H.nlev, "patches_per_level": [H.L(k).np for k in range(1, H.nlev + 1)],
+            "cells": int(sum(cells)), "build_s": round(build_s, 2),
+            "fp64": {"ms_per_solve": round(tot64 / steps * 1e3, 3), "vcycles": st64[-1]["vcycles"],
+                     "final_rel": st64[-1]["finalRel"], "p50_ms": round(pct(lat64, 50), 3),
+                     "p99_ms": round(pct(lat64, 99), 3)},
+            "mixed": {"ms_per_solve": round(totmx / steps * 1e3, 3), "vcycles": stmx[-1]["vcycles"],
+                      "final_rel": stmx[-1]["finalRel"], "speedup_vs_fp64": round(tot64 / totmx, 3),
+                      "rel_l2_vs_fp64": visible_rel_l2(H, Qmx, Q64)},
+            "smoother_largest_level": {"level": kb, "cells": int(cells[kb - 1]),
+                                       "fp64": {"us_per_launch": round(ms64 * 1e3, 2),
+                                                "GBps": round(b64 / ms64 / 1e6, 1),
+                                                "frac": round(b64 / ms64 / 1e6 / HBM_PEAK_GBS, 4)},
+                                       "fp32": {"us_per_launch": round(ms32 * 1e3, 2),
+                                                "GBps": round(b32 / ms32 / 1e6, 1)}}}
+
+
 def bench_serving(nreq, rate):
     """Served stream: nreq requests of the seeded fixture with Poisson
     arrivals at `rate` req/s, continuous batching (FP32-inner and FP64)
@@ -233,7 +283,8 @@ def main():
     ap.add_argument("--kernel-only", action="store_true")
     ap.add_argument("--smoother-launches", type=int, default=400)
     ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 solves (profiling)")
-    ap.add_argument("--smoother", default="exact", choices=["exact", "tiled"])
+    ap.add_argument("--smoother", default=None, choices=["exact", "sweep", "tiled"],
+                    help="smoother form of the headline solves (default: the autotuner's bit-exact pick)")
     ap.add_argument("--coarse-cells", type=int, default=0)
     ap.add_argument("--sweep", action="store_true", help="tuning sweep of the coarse-cycle cut")
     ap.add_argument("--no-parabolic", action="store_true")
@@ -241,9 +292,11 @@ def main():
     ap.add_argument("--large-smoother", type=int, default=4096, metavar="N",
                     help="also time the smoother on one uniform N x N level (0: skip)")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-varcoef", action="store_true", help="skip the refined variable-coefficient workload")
     ap.add_argument("--serve-requests", type=int, default=128)
     ap.add_argument("--serve-rate", type=float, default=100.0)
-    ap.add_argument("--only", choices=["parabolic", "serving"], help="run one secondary workload and print it")
+    ap.add_argument("--only", choices=["parabolic", "serving", "varcoef"],
+                    help="run one secondary workload and print it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain device copy of MIB MiB 20 times (scale for the PMC byte counters)")
     args = ap.parse_args()
@@ -274,6 +327,9 @@ def main():
     if args.only == "serving":
         print(json.dumps(bench_serving(args.serve_requests, args.serve_rate)), flush=True)
         return
+    if args.only == "varcoef":
+        print(json.dumps(bench_varcoef_refined()), flush=True)
+        return
 
     log("building starfish hierarchy on the host")
     t = time.perf_counter()
@@ -294,7 +350,7 @@ def main():
 
     if args.sweep:
         for prec in ("fp64", "mixed"):
-            for sm in ("exact", "tiled"):
+            for sm in ("exact", "sweep", "tiled"):
                 for cc in (-1, 300, 1100, 4100):
                     tot, _, st = time_solves(G, args.steps, args.warmup, prec, coarse_cells=cc, smoother=sm)
                     log(f"sweep {prec} {sm} coarse_cells={cc}: {tot / args.steps * 1e3:.3f} ms/solve "
@@ -306,16 +362,19 @@ def main():
         from bsamgpu.autotune import autotune
 
         best, table = autotune(G, bit_exact_only=False, repeats=3)
-        exact = [r for r in table if r["smoother"] == "exact" and r["precision"] == "fp64" and
+        exact = [r for r in table if r["smoother"] in ("exact", "sweep") and r["precision"] == "fp64" and
                  r["finalRel"] <= opts.RelTol]
         pick = min(exact, key=lambda r: r["seconds"])
         if args.coarse_cells == 0:
             args.coarse_cells = pick["coarse_cells"]
+        if args.smoother is None:
+            args.smoother = pick["smoother"]
         tune = {"fastest": best, "fastest_bit_exact": {k: pick[k] for k in ("coarse_cells", "smoother", "precision")},
                 "table": [{k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in r.items()} for r in table]}
         log(f"autotune: fastest {best}, fastest bit-exact {tune['fastest_bit_exact']}")
 
-    log("FP64 solves")
+    args.smoother = args.smoother or "exact"
+    log(f"FP64 solves (smoother {args.smoother}, coarse_cells {args.coarse_cells})")
     tot64, lat64, st64 = time_solves(G, args.steps, args.warmup, "fp64", smoother=args.smoother, coarse_cells=args.coarse_cells)
     s64 = st64[-1]
     log(f"fp64: {s64['vcycles']} V-cycles, rel {s64['finalRel']:.3e}, {tot64 / args.steps * 1e3:.2f} ms/solve")
@@ -349,6 +408,12 @@ def main():
         log(f"smoother on a uniform {args.large_smoother}^2 level")
         large = bench_large_smoother(args.large_smoother)
         log(f"large smoother: {large}")
+
+    varc = None
+    if not args.no_varcoef:
+        log("variable-coefficient workload refined to 2048^2 spacing")
+        varc = bench_varcoef_refined()
+        log(f"varcoef: fp64 {varc['fp64']}, mixed {varc['mixed']}")
 
     parab = None
     if not args.no_parabolic:
@@ -409,6 +474,7 @@ def main():
                                f"{sum(H.L(kf).dims(p)[0] * H.L(kf).dims(p)[1] for p in range(H.L(kf).np))} cells)",
                      "bytes_per_launch": sm_bytes, "us_per_launch": round(ms64 * 1e3, 3)},
         "cpu_baseline": base,
+        "varcoef_refined": varc,
         "parabolic": parab,
         "serving": serv,
     }

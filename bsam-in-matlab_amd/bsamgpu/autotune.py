@@ -4,8 +4,10 @@ The device path has knobs that change speed but not the converged solution:
 - how many of the coarsest levels run their V-cycle inside one LDS-staged
   workgroup (``coarse_cells``);
 - the smoother form (``exact``: one launch per colour pass, bit-identical to
-  the reference sequence; ``tiled``: every pass of a relax call in one
-  launch on LDS tiles with sibling ghosts frozen);
+  the reference sequence; ``sweep``: the same sequence with pairs of sweeps
+  fused into one launch each on levels whose 32x32 tiles fit one round of
+  workgroups, also bit-identical; ``tiled``: every pass of a relax call in
+  one launch on LDS tiles with sibling ghosts frozen);
 - the arithmetic of the inner V-cycles (``fp64`` or ``mixed``).
 
 ``autotune`` times complete solves from the hierarchy's uploaded state for
@@ -19,7 +21,9 @@ import itertools
 import statistics
 import time
 
-EXACT_CANDIDATES = [dict(coarse_cells=c, smoother="exact", precision="fp64") for c in (-1, 300, 1100, 4100)]
+EXACT_CANDIDATES = [
+    dict(coarse_cells=c, smoother=s, precision="fp64") for s in ("exact", "sweep") for c in (-1, 300, 1100, 4100)
+]
 ALL_CANDIDATES = EXACT_CANDIDATES + [
     dict(coarse_cells=c, smoother=s, precision=p)
     for c, s, p in itertools.product((300, 1100), ("exact", "tiled"), ("fp64", "mixed"))

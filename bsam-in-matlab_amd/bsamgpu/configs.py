@@ -67,6 +67,20 @@ def varcoef_options(**kw):
     return Options(**base)
 
 
+# The ulap indicator is h^2 |lap u| (ebTagCluster.m:53-55); on the 256^2 root of
+# the variable-coefficient problem it peaks at 2.6e-5 and drops about 4x per
+# level, so the specified threshold 1e-3 tags nothing.  At 1e-6 the stages
+# refine to the 2048^2 spacing the configuration names: 8 levels, a 1024^2
+# level as one patch and 20 patches at 2048^2, 1,747,976 cells in all.
+VARCOEF_REFINED_THRESHOLD = 1e-6
+
+
+def varcoef_refined_options(**kw):
+    base = dict(TagThreshold=VARCOEF_REFINED_THRESHOLD)
+    base.update(kw)
+    return varcoef_options(**base)
+
+
 STARFISH_DATA = dict(alpha=3.0, beta=2.0, gamma=1.0, kappa=0.01, g=4.0)
 
 

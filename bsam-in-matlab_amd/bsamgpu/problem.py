@@ -99,7 +99,9 @@ class Options:
     Precision: str = "fp64"
     # smoother form: 'exact' reproduces ebRelaxLevel's sequence bit for bit;
     # 'tiled' runs every colour pass of a relax call in one LDS-staged launch
-    # (same converged solution, sibling ghosts frozen within a call)
+    # (same converged solution, sibling ghosts frozen within a call); 'sweep'
+    # is 'exact' with pairs of red+black sweeps fused into one launch each on
+    # levels whose tiles fit one round of workgroups (bit-identical)
     Smoother: str = "exact"
 
     def __post_init__(self):

@@ -99,6 +99,28 @@ struct STile {
   int p, i0, i1, j0, j1;
 };
 
+// Fused red+black sweep tiles (levels with L.arith).  A tile is a rectangle
+// of one patch, at most kSweepTW x kSweepTH cells.  Its halo lists the
+// colour-1 cells outside the tile that the tile's colour-2 cells read in the
+// second half of a sweep (same-patch neighbours and coarse-fine
+// interpolation sources, sibling cells across copy faces), each with the
+// slot of its position in the tile's (tw+4) x (th+4) region and its patch
+// (0: the tile's patch, s >= 1: sib[s-1]) and local cell coordinates.
+constexpr int kSweepTW = 32, kSweepTH = 32;
+constexpr int kSweepMaxSib = 8;
+constexpr int kSweepMaxHalo = 160;
+constexpr int kSweepRegion = (kSweepTW + 4) * (kSweepTH + 4);
+struct SweepHalo {
+  int slot_ps;  // region slot | (patch slot << 16)
+  int ij;       // i | (j << 16), local to the cell's patch
+};
+struct alignas(16) SweepTile {
+  int p, i0, j0, tw, th, nhalo, nsib, pad;
+  PatchInfo pi;
+  PatchInfo sib[kSweepMaxSib];
+  SweepHalo halo[kSweepMaxHalo];
+};
+
 // device tables and buffers of one level
 template <typename T>
 struct LevelDev {
@@ -115,6 +137,9 @@ struct LevelDev {
   int ntile, nctile, nghost, nring_all, nring_cf, nfoot, ncorr, ntgt;
   int nstile, stile_lds;  // smoother tiles and their largest LDS region (cells)
   STile* stiles;
+  int nsweep;             // fused sweep tiles (0: the level runs colour passes)
+  SweepTile* sweep;
+  T* Qs;                  // the sweep's second copy of Q (same layout)
   PatchInfo* P;
   TileRec* tiles;    // patch record + first cell per 256-cell tile
   int2* ctiles;      // (patch, first coarse cell) per tile of the footprint

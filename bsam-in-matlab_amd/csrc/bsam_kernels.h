@@ -17,6 +17,13 @@ constexpr int kInlRecords = 1, kInlArith = 2, kKeepCf = 4, kPush = 8;
 template <typename T>
 void launch_relax(const LevelDev<T>& L, const LevelDev<T>* parent, int colour, T omega, int homog, int mode,
                   hipStream_t s);
+// one fused red+black sweep of every sweep tile of a level, Qin -> Qout
+// (mode: the red half's ghost source, 0 / kInlRecords / kInlArith; keep: the
+// parent parts are computed from the parent and stored)
+template <typename T>
+void launch_relax_sweep(const LevelDev<T>& L, const LevelDev<T>* parent, const T* Qin, T* Qout, int mode, int keep,
+                        T omega, int homog, hipStream_t s);
+// Q <- Qs on every cell, then the level's fill computed from Qs
 // all passes of one relax call, one patch per workgroup staged in LDS
 constexpr long long kTiledLdsBytes = 64 * 1024;
 template <typename T> bool tiled_fits(const LevelDev<T>& L);

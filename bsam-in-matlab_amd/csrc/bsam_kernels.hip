@@ -144,11 +144,11 @@ __device__ __forceinline__ void keep_face_cf(const LevelDev<T>& L, const T* __re
 }
 
 template <typename T>
-__device__ __forceinline__ T record_ghost(const LevelDev<T>& L, const T* __restrict__ Pq, int idx, int homog,
-                                          int keep) {
+__device__ __forceinline__ T record_ghost(const LevelDev<T>& L, const T* __restrict__ Q, const T* __restrict__ Pq,
+                                          int idx, int homog, int keep) {
   const GhostRec r = L.ghost[idx];
   if (keep && is_cf_kind(r.kind)) L.aux[idx].cf = cf_part<T>(r, Pq);
-  return ghost_value<T>(r, L.Q, Pq, homog);
+  return ghost_value<T>(r, Q, Pq, homog);
 }
 
 // Coarse-fine ghost of the face cell x at 1-based position u along a face of
@@ -198,11 +198,107 @@ __device__ __forceinline__ void tile_fields_early(const TileRec& t) {
 // pass.  Every load the cell needs is issued before any value is used, so a
 // pass costs one memory round trip after the tile record (two in record
 // mode, whose sources come from the records).
+// The update of ebKernels.m:37-95 from resolved neighbour values (ghosts
+// already substituted), in the reference's operation order.
 template <typename T>
-__device__ __forceinline__ void relax_cell(const LevelDev<T>& L, const T* __restrict__ Pq, const PatchInfo& pi,
-                                           int i, int j, T omega, int homog, int mode) {
+__device__ __forceinline__ T relax_combine(const LevelDev<T>& L, T qe, T qw, T qn, T qs, T f, T de, T dw, T dn,
+                                           T ds, T id, T self, T omega) {
+  const T hx = (T)L.h[0], hy = (T)L.h[1];
+  const T hx2 = hx * hx, hy2 = hy * hy;
+  const T ihx2 = (T)L.ih2[0], ihy2 = (T)L.ih2[1];
+  const int p2 = L.pow2;
+  T tmp = f + div_h2<T>(qe * de, hx2, ihx2, p2);
+  tmp = tmp + div_h2<T>(qw * dw, hx2, ihx2, p2);
+  tmp = tmp + div_h2<T>(qn * dn, hy2, ihy2, p2);
+  tmp = tmp + div_h2<T>(qs * ds, hy2, ihy2, p2);
+  tmp = tmp * id;
+  return omega == T(1) ? tmp : omega * tmp + (T(1) - omega) * self;
+}
+
+// Neighbour values of cell (i, j) with its ghosts substituted: partial
+// physical ghosts (Dirichlet (8/3)g + (1/3)u_2, Neumann h g; g: 0 when
+// homogeneous) and, when assembled, coarse-fine ghosts from the cell's own
+// value, its neighbours, the diagonal sources x[] and the face records a[]
+// (sibling ghosts keep the frame value).
+template <typename T>
+__device__ __forceinline__ void resolve_ghosts(const LevelDev<T>& L, const PatchInfo& pi, int i, int j, T re, T rw,
+                                               T rn, T rs, T self, const T* g, const T* x, const FaceAux<T>* a,
+                                               bool assembled, T& qe, T& qw, T& qn, T& qs) {
   const int mx = pi.mx, my = pi.my;
-  T* __restrict__ Q = L.Q;
+  const T hx = (T)L.h[0], hy = (T)L.h[1];
+  qe = re; qw = rw; qn = rn; qs = rs;
+  if (i == 0 && pi.bc[0]) qw = pi.bc[0] == 1 ? T(C83_) * g[0] + T(C13_) * re : hx * g[0];
+  if (i == mx - 1 && pi.bc[1]) qe = pi.bc[1] == 1 ? T(C83_) * g[1] + T(C13_) * rw : hx * g[1];
+  if (j == 0 && pi.bc[2]) qs = pi.bc[2] == 1 ? T(C83_) * g[2] + T(C13_) * rn : hy * g[2];
+  if (j == my - 1 && pi.bc[3]) qn = pi.bc[3] == 1 ? T(C83_) * g[3] + T(C13_) * rs : hy * g[3];
+  if (assembled) {
+    if (i == 0 && !pi.bc[0] && a[0].kind != G_COPY) qw = cf_face_ghost<T>(a[0].kind, j + 1, self, re, rs, rn, x[0], a[0].cf);
+    if (i == mx - 1 && !pi.bc[1] && a[1].kind != G_COPY)
+      qe = cf_face_ghost<T>(a[1].kind, j + 1, self, rw, rs, rn, x[1], a[1].cf);
+    if (j == 0 && !pi.bc[2] && a[2].kind != G_COPY) qs = cf_face_ghost<T>(a[2].kind, i + 1, self, rn, rw, re, x[2], a[2].cf);
+    if (j == my - 1 && !pi.bc[3] && a[3].kind != G_COPY)
+      qn = cf_face_ghost<T>(a[3].kind, i + 1, self, rs, rw, re, x[3], a[3].cf);
+  }
+}
+
+// Every load of one cell for a frame or assembled update, issued at in-bounds
+// addresses whatever the cell's position, so that several cells' loads are in
+// flight at once.  A cell lies on at most one x-face and one y-face (patches
+// are at least 2 wide), so one boundary value, face record and diagonal
+// source per axis suffice: off the faces they stand in for something never
+// read.
+template <typename T>
+struct CellIn {
+  T re, rw, rn, rs, self, f, de, dw, dn, ds, id;
+  T gx, gy, xx, xy;
+  FaceAux<T> ax, ay;
+};
+
+template <typename T>
+__device__ __forceinline__ void cell_load(const LevelDev<T>& L, const T* __restrict__ Q, const PatchInfo& pi, int i,
+                                          int j, bool assembled, CellIn<T>& c) {
+  const int mx = pi.mx, my = pi.my, ldq = mx + 2;
+  const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
+  const int cc = pi.coff + i + j * mx;
+  const int fx = pi.xoff + i + j * (mx + 1);
+  const int fy = pi.yoff + i + j * mx;
+  const bool ee = i == mx - 1, nn = j == my - 1;
+  c.re = Q[qc + 1]; c.rw = Q[qc - 1]; c.rn = Q[qc + ldq]; c.rs = Q[qc - ldq]; c.self = Q[qc];
+  c.f = L.F[cc]; c.id = L.ID[cc];
+  c.de = L.DeW[fx + 1]; c.dw = L.DeW[fx]; c.dn = L.DnS[fy + mx]; c.ds = L.DnS[fy];
+  c.gx = L.G[pi.goff + (ee ? my : 0) + j];
+  c.gy = L.G[pi.goff + 2 * my + (nn ? mx : 0) + i];
+  c.ax = L.aux[pi.gbase + (ee ? my : 0) + j];
+  c.ay = L.aux[pi.gbase + 2 * my + (nn ? mx : 0) + i];
+  if (assembled) {
+    const bool onx = i == 0 || ee, ony = j == 0 || nn;
+    c.xx = Q[qc + (ee ? -1 : 1) + (onx ? cf_diag_step(j + 1, my) : 0) * ldq];
+    c.xy = Q[qc + (nn ? -ldq : ldq) + (ony ? cf_diag_step(i + 1, mx) : 0)];
+  } else {
+    c.xx = c.xy = T(0);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T cell_relax(const LevelDev<T>& L, const PatchInfo& pi, int i, int j, const CellIn<T>& c,
+                                        int homog, bool assembled, T omega) {
+  const T gx = homog ? T(0) : c.gx, gy = homog ? T(0) : c.gy;
+  const T g[4] = {gx, gx, gy, gy}, x[4] = {c.xx, c.xx, c.xy, c.xy};
+  const FaceAux<T> a[4] = {c.ax, c.ax, c.ay, c.ay};
+  T qe, qw, qn, qs;
+  resolve_ghosts<T>(L, pi, i, j, c.re, c.rw, c.rn, c.rs, c.self, g, x, a, assembled, qe, qw, qn, qs);
+  return relax_combine<T>(L, qe, qw, qn, qs, c.f, c.de, c.dw, c.dn, c.ds, c.id, c.self, omega);
+}
+
+// EARLY (multi-workgroup passes): boundary data are loaded for every cell of
+// a patch with that physical side, with the other loads, instead of behind
+// the boundary-cell branch (latency over instruction count; the workgroup
+// passes on LDS-staged levels prefer the opposite).
+template <typename T, bool EARLY = true>
+__device__ __forceinline__ T relax_value(const LevelDev<T>& L, const T* __restrict__ Q, const T* __restrict__ Pq,
+                                         const PatchInfo& pi, int i, int j, T omega, int homog, int mode,
+                                         FaceAux<T>* aux_out = nullptr) {
+  const int mx = pi.mx, my = pi.my;
   const int ldq = mx + 2;
   const int qc = pi.qoff + (i + 1) + (j + 1) * ldq;
   const int cc = pi.coff + i + j * mx;
@@ -222,10 +318,10 @@ __device__ __forceinline__ void relax_cell(const LevelDev<T>& L, const T* __rest
   // boundary data: every patch stores all four sides, so these are in bounds
   T gw = T(0), ge = T(0), gs = T(0), gn = T(0);
   if (!homog) {
-    if (pi.bc[0]) gw = L.G[pi.goff + j];
-    if (pi.bc[1]) ge = L.G[pi.goff + my + j];
-    if (pi.bc[2]) gs = L.G[pi.goff + 2 * my + i];
-    if (pi.bc[3]) gn = L.G[pi.goff + 2 * my + mx + i];
+    if (pi.bc[0] && (EARLY || i == 0)) gw = L.G[pi.goff + j];
+    if (pi.bc[1] && (EARLY || i == mx - 1)) ge = L.G[pi.goff + my + j];
+    if (pi.bc[2] && (EARLY || j == 0)) gs = L.G[pi.goff + 2 * my + i];
+    if (pi.bc[3] && (EARLY || j == my - 1)) gn = L.G[pi.goff + 2 * my + mx + i];
   }
   // face records of the assembled and pushing passes, and the diagonal
   // interpolation sources
@@ -244,49 +340,49 @@ __device__ __forceinline__ void relax_cell(const LevelDev<T>& L, const T* __rest
     if (fn) xn = Q[qc - ldq + cf_diag_step(i + 1, mx)];
   }
 
-  T qe = re, qw = rw, qn = rn, qs = rs;
-  const T hx = (T)L.h[0], hy = (T)L.h[1];
-  // partial physical ghosts: Dirichlet (8/3)g + (1/3)u_2, Neumann h g
-  if (i == 0 && pi.bc[0]) qw = pi.bc[0] == 1 ? T(C83_) * gw + T(C13_) * re : hx * gw;
-  if (i == mx - 1 && pi.bc[1]) qe = pi.bc[1] == 1 ? T(C83_) * ge + T(C13_) * rw : hx * ge;
-  if (j == 0 && pi.bc[2]) qs = pi.bc[2] == 1 ? T(C83_) * gs + T(C13_) * rn : hy * gs;
-  if (j == my - 1 && pi.bc[3]) qn = pi.bc[3] == 1 ? T(C83_) * gn + T(C13_) * rs : hy * gn;
+  T qe, qw, qn, qs;
+  const T gv[4] = {gw, ge, gs, gn}, xv[4] = {xw, xe, xs, xn};
+  const FaceAux<T> av[4] = {aw, ae, as, an};
+  resolve_ghosts<T>(L, pi, i, j, re, rw, rn, rs, self, gv, xv, av, src == kInlArith, qe, qw, qn, qs);
   if (src == kInlRecords) {
     const int keep = mode & kKeepCf;
-    if (fw) qw = record_ghost<T>(L, Pq, iw, homog, keep);
-    if (fe) qe = record_ghost<T>(L, Pq, ie, homog, keep);
-    if (fs) qs = record_ghost<T>(L, Pq, is, homog, keep);
-    if (fn) qn = record_ghost<T>(L, Pq, in_, homog, keep);
-  } else if (src == kInlArith) {
-    if (fw && aw.kind != G_COPY) qw = cf_face_ghost<T>(aw.kind, j + 1, self, re, rs, rn, xw, aw.cf);
-    if (fe && ae.kind != G_COPY) qe = cf_face_ghost<T>(ae.kind, j + 1, self, rw, rs, rn, xe, ae.cf);
-    if (fs && as.kind != G_COPY) qs = cf_face_ghost<T>(as.kind, i + 1, self, rn, rw, re, xs, as.cf);
-    if (fn && an.kind != G_COPY) qn = cf_face_ghost<T>(an.kind, i + 1, self, rs, rw, re, xn, an.cf);
+    if (fw) qw = record_ghost<T>(L, Q, Pq, iw, homog, keep);
+    if (fe) qe = record_ghost<T>(L, Q, Pq, ie, homog, keep);
+    if (fs) qs = record_ghost<T>(L, Q, Pq, is, homog, keep);
+    if (fn) qn = record_ghost<T>(L, Q, Pq, in_, homog, keep);
   }
-  const T hx2 = hx * hx, hy2 = hy * hy;
-  const T ihx2 = (T)L.ih2[0], ihy2 = (T)L.ih2[1];
-  const int p2 = L.pow2;
-  T tmp = f + div_h2<T>(qe * de, hx2, ihx2, p2);
-  tmp = tmp + div_h2<T>(qw * dw, hx2, ihx2, p2);
-  tmp = tmp + div_h2<T>(qn * dn, hy2, ihy2, p2);
-  tmp = tmp + div_h2<T>(qs * ds, hy2, ihy2, p2);
-  tmp = tmp * id;
-  const T y = omega == T(1) ? tmp : omega * tmp + (T(1) - omega) * self;
+  const T tmp = relax_combine<T>(L, qe, qw, qn, qs, f, de, dw, dn, ds, id, self, omega);
+  if (aux_out) {  // the push targets, for a caller that stores the value
+    aux_out[0] = aw; aux_out[1] = ae; aux_out[2] = as; aux_out[3] = an;
+  }
+  return tmp;
+}
+
+// one cell of a colour pass in place on L.Q, with the sibling pushes
+template <typename T, bool EARLY = true>
+__device__ __forceinline__ void relax_cell(const LevelDev<T>& L, const T* __restrict__ Pq, const PatchInfo& pi,
+                                           int i, int j, T omega, int homog, int mode) {
+  T* __restrict__ Q = L.Q;
+  FaceAux<T> a[4];
+  const T y = relax_value<T, EARLY>(L, Q, Pq, pi, i, j, omega, homog, mode, (mode & kPush) ? a : nullptr);
+  const int qc = pi.qoff + (i + 1) + (j + 1) * (pi.mx + 2);
   Q[qc] = y;
   if (mode & kPush) {
     // sibling ghosts that copy this cell; their readers are other-colour
     // cells, which this pass does not visit
-    if (fw && aw.push >= 0) Q[aw.push] = y;
-    if (fe && ae.push >= 0) Q[ae.push] = y;
-    if (fs && as.push >= 0) Q[as.push] = y;
-    if (fn && an.push >= 0) Q[an.push] = y;
+    const bool fw = i == 0 && !pi.bc[0], fe = i == pi.mx - 1 && !pi.bc[1];
+    const bool fs = j == 0 && !pi.bc[2], fn = j == pi.my - 1 && !pi.bc[3];
+    if (fw && a[0].push >= 0) Q[a[0].push] = y;
+    if (fe && a[1].push >= 0) Q[a[1].push] = y;
+    if (fs && a[2].push >= 0) Q[a[2].push] = y;
+    if (fn && a[3].push >= 0) Q[a[3].push] = y;
   }
 }
 
 // one lane of a colour pass over a 256-cell tile (tile: uniform across the
 // wave); in a pass that keeps parent parts, the other colour's face cells
 // store those of their records for the pass that follows
-template <typename T>
+template <typename T, bool EARLY = true>
 __device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __restrict__ Pq, int tile,
                                            int lane, int colour, T omega, int homog, int mode) {
   const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane(tile));
@@ -307,7 +403,7 @@ __device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __rest
     }
     return;
   }
-  relax_cell<T>(L, Pq, pi, i, j, omega, homog, mode);
+  relax_cell<T, EARLY>(L, Pq, pi, i, j, omega, homog, mode);
 }
 
 // Item t of a colour pass over colour-c cells only: the t-th cell of colour c
@@ -315,7 +411,7 @@ __device__ __forceinline__ void relax_item(const LevelDev<T>& L, const T* __rest
 // of each colour (every row has mx/2 of them and tiles start on even cells),
 // so half the lanes cover a pass.  The 64 items of a wave (t aligned to 64)
 // lie in one tile, whose record is therefore read once per wave.
-template <typename T>
+template <typename T, bool EARLY = true>
 __device__ __forceinline__ void relax_item_colour(const LevelDev<T>& L, const T* __restrict__ Pq, int t,
                                                   int colour, T omega, int homog, int inl) {
   const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane(t >> 7));
@@ -326,7 +422,158 @@ __device__ __forceinline__ void relax_item_colour(const LevelDev<T>& L, const T*
   if (2 * a >= mx * pi.my) return;
   const int j = cell_row(2 * a, mx, pi.magic);  // a / (mx/2)
   const int i = 2 * (a - j * (mx >> 1)) + ((colour - 1 + j + pi.par0) & 1);
-  relax_cell<T>(L, Pq, pi, i, j, omega, homog, inl);
+  relax_cell<T, EARLY>(L, Pq, pi, i, j, omega, homog, inl);
+}
+
+// ---------------------------------------------------------------- fused sweep
+// One red+black sweep of a tile per workgroup (levels with L.arith, every
+// patch of even width).  The red half computes the tile's colour-1 cells and
+// the tile's halo (colour-1 cells the colour-2 cells read: neighbours in
+// other tiles or sibling patches, coarse-fine interpolation sources) with
+// relax_value on Qin, exactly as a colour pass would, into an LDS region of
+// the tile plus two rings.  After a barrier the black half updates the
+// tile's colour-2 cells from the region (ghosts: sibling copies from the
+// region, coarse-fine ghosts assembled from region cells and the parent part,
+// partial physical ghosts from the region).  Results go to Qout, which no
+// workgroup reads in this launch, and the tile's values are pushed into the
+// sibling ghosts of Qout that copy them.  MODE is the red half's ghost source
+// (frame / records / assembled); KEEP: the tile's face records' parent parts
+// are computed from the parent and stored (their value for the black half
+// too).
+template <typename T>
+__device__ __forceinline__ void sweep_cell_of(const SweepTile& st, int r, int colour, int& i, int& j) {
+  // r-th cell of the colour in the tile (tile width even: tw/2 per row)
+  const int half = st.tw >> 1;
+  const int jj = r / half;
+  const int ii = 2 * (r - jj * half) + ((colour - 1 + st.i0 + st.j0 + jj + st.pi.par0) & 1);
+  i = st.i0 + ii;
+  j = st.j0 + jj;
+}
+
+template <typename T>
+__device__ __forceinline__ T keep_cf_value(const LevelDev<T>& L, const T* __restrict__ Pq, int idx) {
+  const GhostRec r = L.ghost[idx];
+  const T v = cf_part<T>(r, Pq);
+  if (is_cf_kind(r.kind)) L.aux[idx].cf = v;
+  return v;
+}
+
+template <typename T, int MODE, bool KEEP>
+__global__ void __launch_bounds__(256) k_relax_sweep(LevelDev<T> L, const SweepTile* __restrict__ tiles,
+                                                     const T* __restrict__ Qin, T* __restrict__ Qout,
+                                                     const T* __restrict__ Pq, T omega, int homog) {
+  __shared__ SweepTile st;
+  __shared__ T R[kSweepRegion];
+  {
+    const uint4* __restrict__ src = reinterpret_cast<const uint4*>(tiles + blockIdx.x);
+    uint4* dst = reinterpret_cast<uint4*>(&st);
+    for (int x = threadIdx.x; x < (int)(sizeof(SweepTile) / 16); x += blockDim.x) dst[x] = src[x];
+  }
+  __syncthreads();
+  const PatchInfo& P = st.pi;
+  const int t = threadIdx.x;
+  const int half = (st.tw * st.th) >> 1;
+  const int rw = st.tw + 4;
+  const int mx = P.mx, my = P.my, ldq = mx + 2;
+  constexpr bool ASM = MODE == kInlArith;
+  // ---- red half: two tile cells and one halo cell per lane, every load
+  // issued before any value is used
+  const bool has0 = t < half, has1 = t + 256 < half, hasH = t < st.nhalo;
+  int ri[2], rj[2];
+  sweep_cell_of<T>(st, has0 ? t : 0, 1, ri[0], rj[0]);
+  sweep_cell_of<T>(st, has1 ? t + 256 : 0, 1, ri[1], rj[1]);
+  const SweepHalo he = st.halo[hasH ? t : 0];
+  const int hps = st.nhalo ? he.slot_ps >> 16 : 0;
+  const PatchInfo& HP = hps ? st.sib[hps - 1] : P;
+  const int hi = st.nhalo ? (he.ij & 0xffff) : 0, hj = st.nhalo ? (he.ij >> 16) : 0;
+  T vr[3];
+  FaceAux<T> rax[2], ray[2];
+  if (MODE == kInlRecords) {  // ghosts from the records (the first sweep after a skipped fill)
+    FaceAux<T> a4[4];
+    if (has0) {
+      vr[0] = relax_value<T, true>(L, Qin, Pq, P, ri[0], rj[0], omega, homog, MODE | kPush, a4);
+      rax[0] = ri[0] == mx - 1 ? a4[1] : a4[0]; ray[0] = rj[0] == my - 1 ? a4[3] : a4[2];
+    }
+    if (has1) {
+      vr[1] = relax_value<T, true>(L, Qin, Pq, P, ri[1], rj[1], omega, homog, MODE | kPush, a4);
+      rax[1] = ri[1] == mx - 1 ? a4[1] : a4[0]; ray[1] = rj[1] == my - 1 ? a4[3] : a4[2];
+    }
+    if (hasH) vr[2] = relax_value<T, true>(L, Qin, Pq, HP, hi, hj, omega, homog, MODE);
+  } else {
+    // branch-free: a lane without an item loads a stand-in cell (the
+    // tile's first), so that all three groups issue in one block
+    CellIn<T> c0, c1, ch;
+    cell_load<T>(L, Qin, P, ri[0], rj[0], ASM, c0);
+    cell_load<T>(L, Qin, P, ri[1], rj[1], ASM, c1);
+    cell_load<T>(L, Qin, HP, hi, hj, ASM, ch);
+    if (has0) { vr[0] = cell_relax<T>(L, P, ri[0], rj[0], c0, homog, ASM, omega); rax[0] = c0.ax; ray[0] = c0.ay; }
+    if (has1) { vr[1] = cell_relax<T>(L, P, ri[1], rj[1], c1, homog, ASM, omega); rax[1] = c1.ax; ray[1] = c1.ay; }
+    if (hasH) vr[2] = cell_relax<T>(L, HP, hi, hj, ch, homog, ASM, omega);
+  }
+  if (KEEP) {  // the red tile cells' records, for the sweeps that follow
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      if (!(k ? has1 : has0)) continue;
+      const int i = ri[k], j = rj[k];
+      if (i == 0 && !P.bc[0]) keep_face_cf<T>(L, Pq, P.gbase + j);
+      if (i == mx - 1 && !P.bc[1]) keep_face_cf<T>(L, Pq, P.gbase + my + j);
+      if (j == 0 && !P.bc[2]) keep_face_cf<T>(L, Pq, P.gbase + 2 * my + i);
+      if (j == my - 1 && !P.bc[3]) keep_face_cf<T>(L, Pq, P.gbase + 2 * my + mx + i);
+    }
+  }
+  if (has0) R[(ri[0] - st.i0 + 2) + (rj[0] - st.j0 + 2) * rw] = vr[0];
+  if (has1) R[(ri[1] - st.i0 + 2) + (rj[1] - st.j0 + 2) * rw] = vr[1];
+  if (hasH) R[he.slot_ps & 0xffff] = vr[2];
+  // ---- the black cells' own data (their neighbours come from the region)
+  int bi[2], bj[2];
+  sweep_cell_of<T>(st, has0 ? t : 0, 2, bi[0], bj[0]);
+  sweep_cell_of<T>(st, has1 ? t + 256 : 0, 2, bi[1], bj[1]);
+  CellIn<T> b0, b1;
+  cell_load<T>(L, Qin, P, bi[0], bj[0], false, b0);
+  cell_load<T>(L, Qin, P, bi[1], bj[1], false, b1);
+  if (KEEP) {  // their coarse-fine records' parent parts, from the parent
+    if (has0) {
+      if ((bi[0] == 0 || bi[0] == mx - 1) && !P.bc[bi[0] == 0 ? 0 : 1])
+        b0.ax.cf = keep_cf_value<T>(L, Pq, P.gbase + (bi[0] == mx - 1 ? my : 0) + bj[0]);
+      if ((bj[0] == 0 || bj[0] == my - 1) && !P.bc[bj[0] == 0 ? 2 : 3])
+        b0.ay.cf = keep_cf_value<T>(L, Pq, P.gbase + 2 * my + (bj[0] == my - 1 ? mx : 0) + bi[0]);
+    }
+    if (has1) {
+      if ((bi[1] == 0 || bi[1] == mx - 1) && !P.bc[bi[1] == 0 ? 0 : 1])
+        b1.ax.cf = keep_cf_value<T>(L, Pq, P.gbase + (bi[1] == mx - 1 ? my : 0) + bj[1]);
+      if ((bj[1] == 0 || bj[1] == my - 1) && !P.bc[bj[1] == 0 ? 2 : 3])
+        b1.ay.cf = keep_cf_value<T>(L, Pq, P.gbase + 2 * my + (bj[1] == my - 1 ? mx : 0) + bi[1]);
+    }
+  }
+  __syncthreads();
+  // ---- black half: neighbours and diagonal sources from the region
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (!(k ? has1 : has0)) continue;
+    CellIn<T> c = k ? b1 : b0;
+    const int i = bi[k], j = bj[k];
+    const int sl = (i - st.i0 + 2) + (j - st.j0 + 2) * rw;
+    c.re = R[sl + 1]; c.rw = R[sl - 1]; c.rn = R[sl + rw]; c.rs = R[sl - rw];
+    const bool ee = i == mx - 1, nn = j == my - 1;
+    const bool onx = i == 0 || ee, ony = j == 0 || nn;
+    c.xx = R[sl + (ee ? -1 : 1) + (onx ? cf_diag_step(j + 1, my) : 0) * rw];
+    c.xy = R[sl + (nn ? -rw : rw) + (ony ? cf_diag_step(i + 1, mx) : 0)];
+    const T y = cell_relax<T>(L, P, i, j, c, homog, true, omega);
+    Qout[P.qoff + (i + 1) + (j + 1) * ldq] = y;
+    if (onx && !P.bc[ee ? 1 : 0] && c.ax.push >= 0) Qout[c.ax.push] = y;
+    if (ony && !P.bc[nn ? 3 : 2] && c.ay.push >= 0) Qout[c.ay.push] = y;
+  }
+  // ---- the red tile cells to Qout, with their pushes
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    if (!(k ? has1 : has0)) continue;
+    const int i = ri[k], j = rj[k];
+    const T y = vr[k];
+    Qout[P.qoff + (i + 1) + (j + 1) * ldq] = y;
+    const bool ee = i == mx - 1, nn = j == my - 1;
+    if ((i == 0 || ee) && !P.bc[ee ? 1 : 0] && rax[k].push >= 0) Qout[rax[k].push] = y;
+    if ((j == 0 || nn) && !P.bc[nn ? 3 : 2] && ray[k].push >= 0) Qout[ray[k].push] = y;
+  }
 }
 
 // ---------------------------------------------------------------- tiled smoother
@@ -768,13 +1015,24 @@ __device__ void blk_relax(const LevelDev<T>& L, const T* Pq, int nsweeps, T omeg
   const int ncell = L.ntile * kTile;
   for (int it = 0; it < nsweeps; ++it)
     for (int colour = 1; colour <= 2; ++colour) {
+      // one specialised loop per ghost source (frame / records)
       const int inl = (it == 0 && colour == 1) ? first_inl : 1;
       if (L.even_mx) {
-        for (int x = threadIdx.x; x < (ncell >> 1); x += blockDim.x)
-          relax_item_colour<T>(L, Pq, x, colour, omega, homog, inl);
+        if (inl) {
+          for (int x = threadIdx.x; x < (ncell >> 1); x += blockDim.x)
+            relax_item_colour<T, false>(L, Pq, x, colour, omega, homog, kInlRecords);
+        } else {
+          for (int x = threadIdx.x; x < (ncell >> 1); x += blockDim.x)
+            relax_item_colour<T, false>(L, Pq, x, colour, omega, homog, 0);
+        }
       } else {
-        for (int x = threadIdx.x; x < ncell; x += blockDim.x)
-          relax_item<T>(L, Pq, x / kTile, x % kTile, colour, omega, homog, inl);
+        if (inl) {
+          for (int x = threadIdx.x; x < ncell; x += blockDim.x)
+            relax_item<T, false>(L, Pq, x / kTile, x % kTile, colour, omega, homog, kInlRecords);
+        } else {
+          for (int x = threadIdx.x; x < ncell; x += blockDim.x)
+            relax_item<T, false>(L, Pq, x / kTile, x % kTile, colour, omega, homog, 0);
+        }
       }
       __syncthreads();
     }
@@ -1388,6 +1646,25 @@ void launch_relax(const LevelDev<T>& L, const LevelDev<T>* parent, int colour, T
 }
 
 template <typename T>
+void launch_relax_sweep(const LevelDev<T>& L, const LevelDev<T>* parent, const T* Qin, T* Qout, int mode, int keep,
+                        T omega, int homog, hipStream_t s) {
+  if (!L.nsweep) return;
+  const T* Pq = parent ? (const T*)parent->Q : (const T*)nullptr;
+  const dim3 g(L.nsweep), b(256);
+#define BSAM_SWEEP(m, k) hipLaunchKernelGGL((k_relax_sweep<T, m, k>), g, b, 0, s, L, L.sweep, Qin, Qout, Pq, omega, homog)
+  if (keep) {
+    if (mode == 0) BSAM_SWEEP(0, true);
+    else if (mode == kInlRecords) BSAM_SWEEP(kInlRecords, true);
+    else BSAM_SWEEP(kInlArith, true);
+  } else {
+    if (mode == 0) BSAM_SWEEP(0, false);
+    else if (mode == kInlRecords) BSAM_SWEEP(kInlRecords, false);
+    else BSAM_SWEEP(kInlArith, false);
+  }
+#undef BSAM_SWEEP
+}
+
+template <typename T>
 bool tiled_fits(const LevelDev<T>& L) {
   return L.nstile > 0 && (long long)L.stile_lds * (long long)sizeof(T) <= kTiledLdsBytes;
 }
@@ -1540,7 +1817,9 @@ void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre,
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int);            \
-  template void launch_filldown_small<T>(const LevelDev<T>*, int, int, hipStream_t);            \
+  template void launch_filldown_small<T>(const LevelDev<T>*, int, int, hipStream_t);                       \
+  template void launch_relax_sweep<T>(const LevelDev<T>&, const LevelDev<T>*, const T*, T*, int, int, T, int,   \
+                                      hipStream_t);                                                          \
   template void launch_coarsest<T>(const CoarsestDev&, const LevelDev<T>&, int, hipStream_t);           \
   template void launch_coarse_cycle<T>(const LevelDev<T>*, const LevelDev<T>*, int, const CoarsestDev&, int, int, T, \
                                        int, int, \

@@ -99,6 +99,7 @@ struct bsam_ctx {
 struct bsam_hier {
   bsam_ctx* ctx = nullptr;
   int nlev = 0, root = 0;
+  int ncu = 0;  // compute units of the device (sizes the fused-sweep choice)
   double dom[4] = {0, 0, 0, 0};
   bsam_solve_opts opts{};
   std::vector<HLevel> H;                   // index 0 = level 1
@@ -332,6 +333,10 @@ bool face_records_arith(const HLevel& L) {
   return true;
 }
 
+struct OwnerMap;
+bool build_sweep_tiles(const HLevel& L, std::vector<SweepTile>& out);
+constexpr int kSweepWGsPerCU = 3;  // resident fused-sweep workgroups per CU
+
 struct OwnerMap {
   int n0 = 0, n1 = 0;
   std::vector<int> o;  // p+1, 0 = none
@@ -349,6 +354,121 @@ struct OwnerMap {
     return o[(size_t)(gi - 1) + (size_t)(gj - 1) * n0];
   }
 };
+
+// Fused-sweep tiles of a level: each patch split into tiles of at most
+// kSweepTW x kSweepTH cells, each with the halo its colour-2 cells read in the
+// second half of a sweep (the reads of k_relax_sweep's black phase, mirrored
+// here).  Returns false, and the level keeps colour passes, when a needed cell
+// is not of colour 1, a halo or its patch table overflows, or a position
+// falls outside the tile's region.
+bool build_sweep_tiles(const HLevel& L, std::vector<SweepTile>& out) {
+  out.clear();
+  OwnerMap own;
+  own.build(L);
+  auto colour1 = [&](int p, int i, int j) { return ((i + j + L.pi[p].par0) & 1) == 0; };
+  for (int p = 0; p < L.np; ++p) {
+    const PatchInfo& P = L.pi[p];
+    const Box& b = L.box[p];
+    const int mx = P.mx, my = P.my;
+    if (mx % 2 || mx < 2 || my < 2) return false;
+    const int ntx = (mx + kSweepTW - 1) / kSweepTW, nty = (my + kSweepTH - 1) / kSweepTH;
+    for (int ty = 0; ty < nty; ++ty)
+      for (int tx = 0; tx < ntx; ++tx) {
+        // column boundaries on even cells: every tile row then holds tw/2
+        // cells of each colour (the kernel's cell mapping)
+        const int hm = mx / 2;
+        const int i0 = 2 * (int)((long long)hm * tx / ntx), i1 = 2 * (int)((long long)hm * (tx + 1) / ntx);
+        const int j0 = (int)((long long)my * ty / nty), j1 = (int)((long long)my * (ty + 1) / nty);
+        const int tw = i1 - i0, th = j1 - j0;
+        if (tw < 2 || tw > kSweepTW || tw % 2 || th < 1 || th > kSweepTH) return false;
+        SweepTile st;
+        std::memset(&st, 0, sizeof st);
+        st.p = p; st.i0 = i0; st.j0 = j0; st.tw = tw; st.th = th;
+        st.pi = P;
+        const int rw = tw + 4;
+        std::map<int, int> seen;  // region slot -> halo index
+        std::vector<int> sibs;
+        bool ok = true;
+        // a colour-1 cell at P-local position (ni, nj) (possibly in a sibling)
+        auto need = [&](int ni, int nj) {
+          const int rx = ni - i0 + 2, ry = nj - j0 + 2;
+          if (rx < 0 || rx >= rw || ry < 0 || ry >= th + 4) { ok = false; return; }
+          const int slot = rx + ry * rw;
+          if (ni >= 0 && ni < mx && nj >= 0 && nj < my) {
+            if (!colour1(p, ni, nj)) { ok = false; return; }
+            if (ni >= i0 && ni < i1 && nj >= j0 && nj < j1) return;  // a tile red cell
+            if (seen.count(slot)) return;
+            seen[slot] = 1;
+            if (st.nhalo >= kSweepMaxHalo) { ok = false; return; }
+            st.halo[st.nhalo++] = SweepHalo{slot, ni | (nj << 16)};
+            return;
+          }
+          const int gi = b.ilo + ni, gj = b.jlo + nj;
+          const int q = own.at(gi, gj) - 1;
+          if (q < 0 || q == p) { ok = false; return; }
+          const Box& qb = L.box[q];
+          const int si = gi - qb.ilo, sj = gj - qb.jlo;
+          if (!colour1(q, si, sj)) { ok = false; return; }
+          if (seen.count(slot)) return;
+          seen[slot] = 1;
+          int ps = -1;
+          for (size_t k = 0; k < sibs.size(); ++k)
+            if (sibs[k] == q) ps = (int)k;
+          if (ps < 0) {
+            if ((int)sibs.size() >= kSweepMaxSib) { ok = false; return; }
+            sibs.push_back(q);
+            ps = (int)sibs.size() - 1;
+          }
+          if (st.nhalo >= kSweepMaxHalo) { ok = false; return; }
+          st.halo[st.nhalo++] = SweepHalo{slot | ((ps + 1) << 16), si | (sj << 16)};
+        };
+        for (int j = j0; j < j1 && ok; ++j)
+          for (int i = i0; i < i1 && ok; ++i) {
+            if (colour1(p, i, j)) continue;
+            const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+            for (int d = 0; d < 4 && ok; ++d) {
+              const int ni = i + di[d], nj = j + dj[d];
+              if (ni >= 0 && ni < mx && nj >= 0 && nj < my) { need(ni, nj); continue; }
+              // the cell lies on face d: its ghost (relax_cell's forms)
+              const int ii = i - di[d], ij = j - dj[d];  // inward neighbour
+              if (P.bc[d]) { need(ii, ij); continue; }
+              const int t = d < 2 ? j : i, len = d < 2 ? my : mx, u = t + 1;
+              const int off = d == 0 ? 0 : d == 1 ? my : d == 2 ? 2 * my : 2 * my + mx;
+              const GhostRec& r = L.ghost[P.gbase + off + t];
+              const int ai = d < 2 ? 0 : 1, aj = d < 2 ? 1 : 0;  // along-face step
+              switch (r.kind) {
+                case G_COPY: {
+                  need(ni, nj);
+                  // the record copies the cell just needed
+                  const int q = own.at(b.ilo + ni, b.jlo + nj) - 1;
+                  if (q < 0 || r.s[0] != qidx(L.pi[q], b.ilo + ni - L.box[q].ilo + 1, b.jlo + nj - L.box[q].jlo + 1))
+                    ok = false;
+                  break;
+                }
+                case G_QIF:
+                  if (u == 1 || u == len) { ok = false; break; }
+                  if ((u & 1) == 0) { need(i + ai, j + aj); need(ii + 2 * ai, ij + 2 * aj); }
+                  else { need(i - ai, j - aj); need(ii - 2 * ai, ij - 2 * aj); }
+                  break;
+                case G_QIF_END:
+                  need(ii, ij);
+                  break;
+                case G_LIF:
+                  if ((u & 1) == 0) need(i - ai, j - aj); else need(i + ai, j + aj);
+                  break;
+                default:
+                  ok = false;
+              }
+            }
+          }
+        if (!ok) return false;
+        st.nsib = (int)sibs.size();
+        for (size_t k = 0; k < sibs.size(); ++k) st.sib[k] = L.pi[sibs[k]];
+        out.push_back(st);
+      }
+  }
+  return true;
+}
 
 // Ghost records of a level without a parent (level 1): all faces physical,
 // corners extrapolated (ebFillGhosts.m:53-89).
@@ -913,6 +1033,27 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
   if ((rc = upload_cast(h, &D.G, L.G))) return rc;
   std::vector<double> zqc(L.nqc, 0.0);
   if ((rc = upload_cast(h, &D.QC, zqc))) return rc;
+  // fused sweep tiles (levels with the standard face layout, even widths);
+  // the FP32 copy shares them, each precision has its own second Q
+  D.nsweep = 0; D.sweep = nullptr; D.Qs = nullptr;
+  if (D.arith && D.even_mx && L.np > 0) {
+    if (tables_from64) {
+      D.nsweep = d64->nsweep; D.sweep = d64->sweep;
+    } else {
+      // a level takes fused sweeps when its tiles fit the device in one round
+      // (the sweep kernel holds three 256-lane workgroups per CU); a larger
+      // level runs colour passes, whose launches are all resident at once
+      std::vector<SweepTile> st;
+      if (build_sweep_tiles(L, st) && (long long)st.size() <= (long long)kSweepWGsPerCU * h->ncu) {
+        if ((rc = upload(h, &D.sweep, st))) return rc;
+        D.nsweep = (int)st.size();
+      }
+    }
+    if (D.nsweep) {
+      std::vector<double> zq(L.nq, 0.0);
+      if ((rc = upload_cast(h, &D.Qs, zq))) return rc;
+    }
+  }
   std::vector<FaceAux<T>> aux(L.ghost.size());
   for (size_t t = 0; t < L.ghost.size(); ++t) {
     aux[t].cf = T(0);
@@ -985,7 +1126,24 @@ struct Seq {
     // pass that follows.
     const bool ar = lev(k).arith && lev(k).aux;
     if (first_inline == 2 && !ar) first_inline = 1;
-    for (int it = 0; it < nsweeps; ++it)
+    int it0 = 0;
+    if (h->opts.smoother == 2 && ar && lev(k).nsweep && nsweeps > 1) {
+      // one launch per red+black sweep for an even number of sweeps, Q and
+      // Qs in turn, so that the last of them writes Q; an odd count does its
+      // last sweep as two colour passes on Q.  The parent parts are current
+      // after the preceding fill or the preparing correction, else the first
+      // sweep computes and stores them (and, in record mode, its black half
+      // needs them from the records)
+      const LevelDev<T>& Lk = lev(k);
+      T* buf[2] = {Lk.Q, Lk.Qs};
+      it0 = nsweeps & ~1;
+      for (int it = 0; it < it0; ++it) {
+        const int mode = it ? kInlArith : (first_inline == 1 ? kInlRecords : first_inline == 2 ? kInlArith : 0);
+        const int keep = it == 0 && (first_inline == 1 || (first_inline == 0 && !cf_current));
+        launch_relax_sweep<T>(Lk, par(k), buf[it & 1], buf[(it + 1) & 1], mode, keep, (T)h->opts.omega, homog, s);
+      }
+    }
+    for (int it = it0; it < nsweeps; ++it)
       for (int colour = 1; colour <= 2; ++colour) {
         const bool first = it == 0 && colour == 1, last = it == nsweeps - 1 && colour == 2;
         int mode = first ? (first_inline ? kInlRecords : 0) : kInlRecords;
@@ -1120,7 +1278,10 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
   h->root = d->root_idx;
   for (int i = 0; i < 4; ++i) h->dom[i] = d->domain[i];
   if (o) h->opts = *o; else bsam_default_opts(&h->opts);
-  if (ctx->device >= 0) HIPCHK(hipSetDevice(ctx->device));
+  if (ctx->device >= 0) {
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  }
   h->H.resize(h->nlev);
   size_t pb = 0, oq = 0, oc = 0, ox = 0, oy = 0, og = 0;
   for (int k = 0; k < h->nlev; ++k) {
@@ -1490,6 +1651,7 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
   }
   if (h->opts.npre < 0 || h->opts.npost < 0 || h->opts.max_vcycles < 0)
     return fail(BSAM_E_ARG, "bsamgpu:arg: negative sweep or cycle count");
+  if (h->opts.smoother < 0 || h->opts.smoother > 2) return fail(BSAM_E_ARG, "bsamgpu:arg: smoother must be 0, 1 or 2");
   const bool mixed = h->opts.precision == BSAM_MIXED;
   if (mixed)
     if (int rc = ensure32(h)) return rc;
@@ -1620,6 +1782,7 @@ long long bsam_export_size(const bsam_hier* h, int k, int what) {
     case 1: return k >= 2 ? (long long)L.nqc : 0;
     case 2: return 2 * (long long)L.corr.size();
     case 3: return 1;
+    case 4: return 1;
     default: return -1;
   }
 }
@@ -1647,6 +1810,13 @@ int bsam_export(const bsam_hier* h, int k, int what, long long* out) {
     case 3:
       out[0] = face_records_arith(L) ? 1 : 0;
       break;
+    case 4: {
+      std::vector<SweepTile> st;
+      bool even = true;
+      for (const PatchInfo& P : L.pi) even = even && P.mx % 2 == 0;
+      out[0] = (face_records_arith(L) && even && L.np > 0 && build_sweep_tiles(L, st)) ? (long long)st.size() : 0;
+      break;
+    }
   }
   return 0;
 }
@@ -1669,7 +1839,9 @@ double bsam_smoother_bytes(const bsam_hier* h, int k, int elem_bytes) {
     const double mx = P.mx, my = P.my;
     b += (mx + 2) * (my + 2) + 3 * mx * my + (mx + 1) * my + mx * (my + 1);
   }
-  return b * elem_bytes / 2.0;
+  // a fused sweep launch does the whole red+black sweep, a colour pass half
+  const bool sweep = h->opts.smoother == 2 && (int)h->d64.size() >= k && h->d64[k - 1].nsweep > 0;
+  return sweep ? b * elem_bytes : b * elem_bytes / 2.0;
 }
 
 int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* ms_per_launch) {
@@ -1681,7 +1853,22 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, h->stream));
+  const bool sweep = h->opts.smoother == 2 && h->d64[k - 1].nsweep > 0;
   for (int i = 0; i < nlaunch; ++i) {
+    if (sweep) {
+      // the form of every sweep after the first: a fused red+black sweep,
+      // Q and Qs in turn
+      if (precision == BSAM_MIXED) {
+        LevelDev<float>& D = h->d32[k - 1];
+        launch_relax_sweep<float>(D, k > 1 ? &h->d32[k - 2] : nullptr, (i & 1) ? D.Qs : D.Q, (i & 1) ? D.Q : D.Qs,
+                                  kInlArith, 0, (float)h->opts.omega, 1, h->stream);
+      } else {
+        LevelDev<double>& D = h->d64[k - 1];
+        launch_relax_sweep<double>(D, k > 1 ? &h->d64[k - 2] : nullptr, (i & 1) ? D.Qs : D.Q,
+                                   (i & 1) ? D.Q : D.Qs, kInlArith, 0, h->opts.omega, 0, h->stream);
+      }
+      continue;
+    }
     const int colour = 1 + (i & 1);
     // the form of the passes between the first and the last of a relax call
     const int mode = h->d64[k - 1].arith ? (kInlArith | kPush) : kInlRecords;

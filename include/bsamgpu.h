@@ -98,7 +98,12 @@ typedef struct {
                           per colour, bit-identical to the reference sequence);
                           1: LDS-tiled -- every colour pass of a relax call in one
                           launch, one patch per workgroup, sibling ghosts frozen
-                          for the call (same converged solution) */
+                          for the call (same converged solution);
+                          2: as 0, bit-identical, but on levels whose patches split
+                          into at most 3 x CU-count tiles of <= 32x32 cells, each
+                          pair of red+black sweeps runs as fused sweep launches
+                          (red half in LDS, black half from it), alternating Q
+                          and a second copy of Q */
 } bsam_solve_opts;
 
 /* One solve stage (EllipticBSAM/src/ebSolveCycles.m:58-70) plus device timing. */
@@ -162,13 +167,15 @@ int bsam_composite_mean(bsam_hier *h, double *mean, double *area);
  * restarts a solve from the descriptor's state without host traffic. */
 int bsam_hier_reset(bsam_hier *h);
 
-/* Measurement helpers.  bsam_time_smoother launches `nlaunch` single-colour
- * smoother passes (alternating red/black, no ghost refresh) over every patch
- * of level k on the hierarchy's stream and returns the average launch time
- * from HIP events; precision BSAM_MIXED times the FP32 kernel.
- * bsam_smoother_bytes returns the algorithmic bytes of one colour pass:
- * half of  b*sum_patches[(mx+2)(my+2) + 3 mx my + (mx+1) my + mx (my+1)]
- * (read Q, F, InvDiagS, DeW, DnS, write Q per red+black sweep). */
+/* Measurement helpers.  bsam_time_smoother launches `nlaunch` smoother
+ * launches of the form the solve uses on level k (fused red+black sweeps on
+ * levels that have sweep tiles, else single-colour passes alternating
+ * red/black), with no ghost refresh, on the hierarchy's stream and returns
+ * the average launch time from HIP events; precision BSAM_MIXED times the
+ * FP32 kernel.  bsam_smoother_bytes returns the algorithmic bytes of one such
+ * launch:  B = b*sum_patches[(mx+2)(my+2) + 3 mx my + (mx+1) my + mx (my+1)]
+ * (read Q, F, InvDiagS, DeW, DnS, write Q per red+black sweep) for a sweep,
+ * B/2 for a colour pass. */
 int bsam_time_smoother(bsam_hier *h, int k, int precision, int nlaunch, double *ms_per_launch);
 double bsam_smoother_bytes(const bsam_hier *h, int k, int elem_bytes);
 
@@ -188,7 +195,9 @@ int bsam_set_level(bsam_hier *h, int k, int field, const double *in);
  *           ebBuildOps.m:158-166
  * what = 3: one value, 1 when every face ghost record of level k has the
  *           standard interpolation stencil and sibling push target, so that
- *           smoother passes after the first assemble ghosts from neighbours */
+ *           smoother passes after the first assemble ghosts from neighbours
+ * what = 4: one value, the number of fused red+black sweep tiles of level k
+ *           (0: the level's relax calls run single-colour passes) */
 long long bsam_export_size(const bsam_hier *h, int k, int what);
 int bsam_export(const bsam_hier *h, int k, int what, long long *out);
 

@@ -13,7 +13,7 @@ from bsamgpu import autotune, configs, hierarchy, _lib
 
 def test_exact_candidates_are_fp64_exact_only():
     for c in autotune.EXACT_CANDIDATES:
-        assert c["smoother"] == "exact" and c["precision"] == "fp64"
+        assert c["smoother"] in ("exact", "sweep") and c["precision"] == "fp64"
     keys = {tuple(sorted(c.items())) for c in autotune.ALL_CANDIDATES}
     assert len(keys) == len(autotune.ALL_CANDIDATES)
     assert all(c in autotune.ALL_CANDIDATES for c in autotune.EXACT_CANDIDATES)

@@ -94,16 +94,19 @@ def test_operations_bit_exact(name):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("graph,coarse", [(False, -1), (True, -1), (True, 0), (True, 100000)])
-def test_solve_cycles_bit_exact(name, graph, coarse):
+@pytest.mark.parametrize("graph,coarse,smoother", [(False, -1, "exact"), (True, -1, "exact"), (True, 0, "exact"),
+                                                    (True, 100000, "exact"), (False, -1, "sweep"),
+                                                    (True, 0, "sweep")])
+def test_solve_cycles_bit_exact(name, graph, coarse, smoother):
     """coarse = -1: every level as multi-workgroup launches; 0: default cut;
-    100000: the whole V-cycle inside one workgroup."""
+    100000: the whole V-cycle inside one workgroup.  smoother 'sweep': pairs
+    of red+black sweeps as fused launches where the level's tiles allow."""
     H = CASES[name](seed=None)
-    print(f"case {name} graph={graph} coarse={coarse}", flush=True)
+    print(f"case {name} graph={graph} coarse={coarse} smoother={smoother}", flush=True)
     Orc = O.OracleHierarchy(H)
     G = GpuHierarchy(H)
     so = Orc.solve_cycles()
-    sg = G.solve_cycles(use_graph=graph, coarse_cells=coarse)
+    sg = G.solve_cycles(use_graph=graph, coarse_cells=coarse, smoother=smoother)
     assert sg["vcycles"] == so["vcycles"]
     assert sg["resHist"] == so["resHist"], (sg["resHist"], so["resHist"])
     assert sg["finalRel"] == so["finalRel"]
@@ -155,9 +158,9 @@ def test_tiled_and_mixed_reach_the_same_solution(name, precision):
     H.opts = H.opts.replace(RelTol=1e-11, MaxVCycles=40)
     Orc = O.OracleHierarchy(H)
     so = Orc.solve_cycles()
-    for smoother in ("exact", "tiled"):
-        if precision == "fp64" and smoother == "exact":
-            continue
+    for smoother in ("exact", "tiled", "sweep"):
+        if precision == "fp64" and smoother != "tiled":
+            continue  # bit-identical forms: covered by test_solve_cycles_bit_exact
         G = GpuHierarchy(H)
         sg = G.solve_cycles(smoother=smoother, precision=precision)
         err = _visible_rel_l2(Orc, G)
