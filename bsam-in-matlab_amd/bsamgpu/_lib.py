@@ -43,10 +43,11 @@ EXPORTS = [
     "bsam_time_smoother", "bsam_smoother_bytes", "bsam_update_coeffs", "bsam_parabolic_init",
     "bsam_parabolic_set_history", "bsam_parabolic_get_history", "bsam_parabolic_set_source",
     "bsam_parabolic_rhs", "bsam_parabolic_advance", "bsam_parabolic_step", "bsam_batch_create",
-    "bsam_batch_destroy", "bsam_batch_step",
+    "bsam_batch_destroy", "bsam_batch_step", "bsam_tag_level",
 ]
 
 BSAM_BE, BSAM_BDF2 = 0, 1
+BSAM_TAG_ULAP, BSAM_TAG_RTE = 0, 1
 
 
 class HierDesc(ctypes.Structure):
@@ -128,6 +129,7 @@ def load(path=None):
         "bsam_hier_reset": (c_int, [vp]),
         "bsam_time_smoother": (c_int, [vp, c_int, c_int, c_int, P_double]),
         "bsam_smoother_bytes": (c_double, [vp, c_int, c_int]),
+        "bsam_tag_level": (c_int, [vp, c_int, c_int, c_double, c_int, vp, vp, P_ll]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

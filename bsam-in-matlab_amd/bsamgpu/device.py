@@ -164,6 +164,19 @@ class GpuHierarchy:
         check(self.lib.bsam_composite_mean(self.h, ctypes.byref(m), ctypes.byref(a)), "bsam_composite_mean")
         return m.value, a.value
 
+    def tag_level(self, k, indicator, threshold, buffer):
+        """Device tagging of level k (ebTagCluster.m:27-90): returns (T, Traw),
+        boolean arrays over the level's index space -- the tags after buffer
+        and nesting clip, and the indicator tags before them."""
+        code = {"ulap": _lib.BSAM_TAG_ULAP, "rte": _lib.BSAM_TAG_RTE}[indicator.lower()]
+        n1, n2 = (int(v) for v in self.H.L(k).n)
+        T = np.zeros((n1, n2), dtype=np.uint8, order="F")
+        R = np.zeros((n1, n2), dtype=np.uint8, order="F")
+        cnt = _lib.c_ll()
+        check(self.lib.bsam_tag_level(self.h, k, code, float(threshold), int(buffer), T.ctypes.data, R.ctypes.data,
+                                      ctypes.byref(cnt)), "bsam_tag_level")
+        return T.astype(bool), R.astype(bool)
+
     # ------------------------------------------------------------ data
     def _shapes(self, k, field):
         L = self.H.L(k)

@@ -208,7 +208,7 @@ def ebp_init(pp: ParabolicProblem, opts: ParabolicOptions, aDt1):
         G.fill_ghosts(k)
     G.pull_q(H)
     while H.nlev - H.rootIdx + 1 < opts.eb.MaxLevels:
-        boxes = tagcluster.tag_cluster(H)
+        boxes = tagcluster.tag_cluster(H, device=G)
         if not boxes:
             break
         kNew, G = _add_level(H, G, boxes)
@@ -418,7 +418,7 @@ def ebp_regrid(S: _State, pp: ParabolicProblem, t, aDtNext, opts: ParabolicOptio
         G.fill_ghosts(k)
     G.pull_q(Hn)
     while Hn.nlev - Hn.rootIdx + 1 < opts.eb.MaxLevels:
-        boxes = tagcluster.tag_cluster(Hn)
+        boxes = tagcluster.tag_cluster(Hn, device=G)
         if not boxes:
             break
         kN, G = _add_level(Hn, G, boxes)

@@ -157,7 +157,7 @@ class Request:
         H, opts = self.H, self.opts
         self.G.pull_q(H)
         if opts.Indicator.lower() != "fun" and H.nlev - H.rootIdx + 1 < opts.MaxLevels:
-            boxes = tagcluster.tag_cluster(H)
+            boxes = tagcluster.tag_cluster(H, device=self.G)
             if boxes:
                 self.G.close()
                 kNew = hierarchy.add_level(H, boxes)

@@ -59,7 +59,7 @@ def eb_solve(prob: Problem, opts: Options | None = None, **kw) -> Solution:
     else:
         _stage(H, dev, opts)
         while H.nlev - H.rootIdx + 1 < opts.MaxLevels:
-            boxes = tagcluster.tag_cluster(H)
+            boxes = tagcluster.tag_cluster(H, device=dev)
             if not boxes:
                 if opts.Verbose >= 1:
                     print(f"[ebsam] no cells tagged; stopping at {H.nlev - H.rootIdx + 1} AMR level(s)")

@@ -12,6 +12,7 @@ import warnings
 
 import numpy as np
 
+from .hierarchy import restrict2
 from .problem import BsamError, call_tagfun
 
 
@@ -269,42 +270,85 @@ def ulap_indicator(Q):
     return np.sqrt(sdx ** 2 + sdy ** 2)
 
 
-def tag_cluster(H, with_info=False):
-    """ebTagCluster.m:1-110 -> boxes (list of [ilo ihi jlo jhi], level-k space)."""
+def _op2d(Q, DeW, DnS, CC, hx, hy):
+    """L(u) = -div(D grad u) + C u on a padded array (ebKernels.m:13-25)."""
+    mx, my = CC.shape
+    feW = DeW * (Q[1:mx + 2, 1:my + 1] - Q[0:mx + 1, 1:my + 1])
+    fnS = DnS * (Q[1:mx + 1, 1:my + 2] - Q[1:mx + 1, 0:my + 1])
+    return -(feW[1:, :] - feW[:-1, :]) / (hx * hx) - (fnS[:, 1:] - fnS[:, :-1]) / (hy * hy) + CC * Q[1:mx + 1, 1:my + 1]
+
+
+def rte_indicator(Q, DeW, DnS, CC, hx, hy):
+    """Relative truncation error |L_2h(R u) - R(L_h u)| per fine cell
+    (ebRte.m:14-42).  R u carries a ghost ring averaged from the fine ghost
+    pairs; the 2h operator uses face-averaged D and the 4-cell mean of C."""
+    mx, my = Q.shape[0] - 2, Q.shape[1] - 2
+    if mx % 2 or my % 2:
+        raise BsamError("eb_tag_cluster:rte", "the rte indicator needs even patch widths")
+    cmx, cmy = mx // 2, my // 2
+    rlh = restrict2(_op2d(Q, DeW, DnS, CC, hx, hy))
+    R = np.zeros((cmx + 2, cmy + 2))
+    R[1:-1, 1:-1] = restrict2(Q[1:-1, 1:-1])
+    j = np.arange(1, cmy + 1)
+    R[0, j] = 0.5 * (Q[0, 2 * j - 1] + Q[0, 2 * j])
+    R[cmx + 1, j] = 0.5 * (Q[mx + 1, 2 * j - 1] + Q[mx + 1, 2 * j])
+    i = np.arange(1, cmx + 1)
+    R[i, 0] = 0.5 * (Q[2 * i - 1, 0] + Q[2 * i, 0])
+    R[i, cmy + 1] = 0.5 * (Q[2 * i - 1, my + 1] + Q[2 * i, my + 1])
+    cDeW = 0.5 * (DeW[0:mx + 1:2, 0:my - 1:2] + DeW[0:mx + 1:2, 1:my:2])
+    cDnS = 0.5 * (DnS[0:mx - 1:2, 0:my + 1:2] + DnS[1:mx:2, 0:my + 1:2])
+    tau = _op2d(R, cDeW, cDnS, restrict2(CC), 2 * hx, 2 * hy) - rlh
+    return np.repeat(np.repeat(np.abs(tau), 2, axis=0), 2, axis=1)
+
+
+def _threshold(opts, level):
+    thr = opts.TagThreshold
+    return thr(level) if callable(thr) else thr
+
+
+def tag_cluster(H, with_info=False, device=None):
+    """ebTagCluster.m:1-110 -> boxes (list of [ilo ihi jlo jhi], level-k space).
+
+    ``device``: a GpuHierarchy holding the same solution as ``H``; the 'ulap'
+    and 'rte' indicators, the buffer and the nesting clip then run on the
+    device (bsam_tag_level) and only the tag map comes back."""
     opts = H.opts
     k = H.nlev
     lev = H.L(k)
     n = lev.n
-    T = np.zeros(n, dtype=bool)
     dom = H.prob.domain
     info = {"indicator": [], "Traw": None, "T": None, "allowed": None}
     ind = opts.Indicator.lower()
-    for p in range(lev.np):
-        b = lev.box[p]
-        if ind == "fun":
-            if opts.TagFunction is None:
-                raise BsamError("eb_tag_cluster:tagfun", "Indicator=fun needs TagFunction.")
-            XC, YC = lev.cell_centers(dom, p)
-            tp = call_tagfun(opts.TagFunction, XC, YC, k - H.rootIdx + 1)
-        elif ind == "ulap":
-            s = ulap_indicator(lev.Q[p])
-            thr = opts.TagThreshold
-            if callable(thr):
-                thr = thr(k - H.rootIdx + 1)
-            tp = s > thr
-            if with_info:
-                info["indicator"].append(s)
-        else:
-            raise BsamError("eb_tag_cluster:ind", f"unknown Indicator {opts.Indicator}")
-        T[b[0] - 1:b[1], b[2] - 1:b[3]] |= tp
-    info["Traw"] = T.copy()
-    T = dilate(T, opts.TagBuffer)
     cover = np.zeros(n, dtype=bool)
     for p in range(lev.np):
         b = lev.box[p]
         cover[b[0] - 1:b[1], b[2] - 1:b[3]] = True
     allowed = erode8(cover)
-    T = T & allowed
+    if device is not None and ind in ("ulap", "rte"):
+        T, info["Traw"] = device.tag_level(k, ind, _threshold(opts, k - H.rootIdx + 1), opts.TagBuffer)
+    else:
+        T = np.zeros(n, dtype=bool)
+        for p in range(lev.np):
+            b = lev.box[p]
+            if ind == "fun":
+                if opts.TagFunction is None:
+                    raise BsamError("eb_tag_cluster:tagfun", "Indicator=fun needs TagFunction.")
+                XC, YC = lev.cell_centers(dom, p)
+                tp = call_tagfun(opts.TagFunction, XC, YC, k - H.rootIdx + 1)
+            elif ind in ("ulap", "rte"):
+                if ind == "ulap":
+                    s = ulap_indicator(lev.Q[p])
+                else:
+                    s = rte_indicator(lev.Q[p], lev.DeW[p], lev.DnS[p], lev.CC[p], lev.h[0], lev.h[1])
+                tp = s > _threshold(opts, k - H.rootIdx + 1)
+                if with_info:
+                    info["indicator"].append(s)
+            else:
+                raise BsamError("eb_tag_cluster:ind", f"unknown Indicator {opts.Indicator}")
+            T[b[0] - 1:b[1], b[2] - 1:b[3]] |= tp
+        info["Traw"] = T.copy()
+        T = dilate(T, opts.TagBuffer)
+        T = T & allowed
     info["T"] = T
     info["allowed"] = allowed
     if not T.any():

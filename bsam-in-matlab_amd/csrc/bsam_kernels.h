@@ -23,7 +23,6 @@ void launch_relax(const LevelDev<T>& L, const LevelDev<T>* parent, int colour, T
 template <typename T>
 void launch_relax_sweep(const LevelDev<T>& L, const LevelDev<T>* parent, const T* Qin, T* Qout, int mode, int keep,
                         T omega, int homog, hipStream_t s);
-// Q <- Qs on every cell, then the level's fill computed from Qs
 // all passes of one relax call, one patch per workgroup staged in LDS
 constexpr long long kTiledLdsBytes = 64 * 1024;
 template <typename T> bool tiled_fits(const LevelDev<T>& L);
@@ -31,6 +30,14 @@ template <typename T>
 void launch_relax_tiled(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, T omega, int homog,
                         hipStream_t s);
 template <typename T> void launch_fill(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, hipStream_t s);
+// tag indicator of every cell of a level (0: undivided Laplacian, 1: relative
+// truncation error) against thr, stored with the level's coverage into
+// level-index bitmaps; lo: per patch, the box's lower corner (0-based)
+void launch_tag_cells(const LevelDev<double>& L, const int2* lo, int indicator, double thr, uint8_t* raw,
+                      uint8_t* cover, hipStream_t s);
+// buffer (dilation by a (2b+1)^2 square) and proper-nesting clip of the raw tags
+void launch_tag_finish(const uint8_t* raw, const uint8_t* cover, int n1, int n2, int buffer, uint8_t* out,
+                       hipStream_t s);
 template <typename T> void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
 // restrict k -> k-1 + fill k-1 for k = kfrom..2 in one workgroup
 template <typename T> void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int homog, hipStream_t s);

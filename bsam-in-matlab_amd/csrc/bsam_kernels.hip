@@ -1807,6 +1807,123 @@ void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre,
                      npost, omega, mass_corr, mixed, inner);
 }
 
+// ---------------------------------------------------------------- tagging
+// Relative truncation error of coarse cell (ci, cj) of a patch
+// (EllipticBSAM/src/ebRte.m:14-42): tau = L_2h(R u) - R(L_h u), where R u has
+// a ghost ring averaged from the fine ghost pairs, and the 2h operator uses
+// face-averaged D and the 4-cell mean of C.
+__device__ double rte_tau(const LevelDev<double>& L, const PatchInfo& pi, int ci, int cj) {
+  const int mx = pi.mx, my = pi.my, ldq = mx + 2, cmx = mx / 2, cmy = my / 2;
+  const double* __restrict__ Q = L.Q + pi.qoff;
+  const double* __restrict__ DeW = L.DeW + pi.xoff;  // (mx+1) x my
+  const double* __restrict__ DnS = L.DnS + pi.yoff;  // mx x (my+1)
+  const double* __restrict__ CC = L.CC + pi.coff;
+  const H2<double> g = level_h2<double>(L);
+  double lh[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int i = 2 * ci + (s & 1), j = 2 * cj + (s >> 1);
+    lh[s] = op_at<double>(Q, (i + 1) + (j + 1) * ldq, ldq, DeW[i + j * (mx + 1)], DeW[i + 1 + j * (mx + 1)],
+                          DnS[i + j * mx], DnS[i + (j + 1) * mx], CC[i + j * mx], g);
+  }
+  const double rlh = 0.25 * (((lh[0] + lh[1]) + lh[2]) + lh[3]);
+  // R u at coarse (I, J), I in [-1, cmx], J in [-1, cmy] (not both outside)
+  auto ru = [&](int I, int J) -> double {
+    if (I < 0) return 0.5 * (Q[(2 * J + 1) * ldq] + Q[(2 * J + 2) * ldq]);
+    if (I >= cmx) return 0.5 * (Q[(mx + 1) + (2 * J + 1) * ldq] + Q[(mx + 1) + (2 * J + 2) * ldq]);
+    if (J < 0) return 0.5 * (Q[2 * I + 1] + Q[2 * I + 2]);
+    if (J >= cmy) return 0.5 * (Q[(2 * I + 1) + (my + 1) * ldq] + Q[(2 * I + 2) + (my + 1) * ldq]);
+    const int q = (2 * I + 1) + (2 * J + 1) * ldq;
+    return 0.25 * (((Q[q] + Q[q + 1]) + Q[q + ldq]) + Q[q + 1 + ldq]);
+  };
+  const int i0 = 2 * ci, j0 = 2 * cj;
+  const double dw = 0.5 * (DeW[i0 + j0 * (mx + 1)] + DeW[i0 + (j0 + 1) * (mx + 1)]);
+  const double de = 0.5 * (DeW[i0 + 2 + j0 * (mx + 1)] + DeW[i0 + 2 + (j0 + 1) * (mx + 1)]);
+  const double ds = 0.5 * (DnS[i0 + j0 * mx] + DnS[i0 + 1 + j0 * mx]);
+  const double dn = 0.5 * (DnS[i0 + (j0 + 2) * mx] + DnS[i0 + 1 + (j0 + 2) * mx]);
+  const double cc = 0.25 * (((CC[i0 + j0 * mx] + CC[i0 + 1 + j0 * mx]) + CC[i0 + (j0 + 1) * mx]) +
+                            CC[i0 + 1 + (j0 + 1) * mx]);
+  const double hx = 2.0 * L.h[0], hy = 2.0 * L.h[1];
+  const H2<double> gc{hx * hx, hy * hy, 0.25 * L.ih2[0], 0.25 * L.ih2[1], L.pow2};
+  const double lc = op5<double>(ru(ci, cj), ru(ci - 1, cj), ru(ci + 1, cj), ru(ci, cj - 1), ru(ci, cj + 1), dw, de, ds,
+                                dn, cc, gc);
+  return lc - rlh;
+}
+
+// One item per cell of the level (256-cell tiles): indicator > thr into the
+// level-index raw tag map and the coverage map (EllipticBSAM/src/ebTagCluster.m:27-75).
+// ulap: sqrt(sdx^2 + sdy^2) of the undivided second differences of Q
+// (ebTagCluster.m:50-55); rte: |tau| of the cell's 2x2 block (ebRte.m:40-41).
+__global__ void __launch_bounds__(kTile) k_tag_cells(LevelDev<double> L, const int2* __restrict__ lo, int indicator,
+                                                     double thr, uint8_t* __restrict__ raw,
+                                                     uint8_t* __restrict__ cover) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t >= L.ntile * kTile) return;
+  const TileRec& tr = L.tiles[t / kTile];
+  const PatchInfo& pi = tr.pi;
+  const int c = tr.start + t % kTile;
+  if (c >= pi.mx * pi.my) return;
+  const int j = c / pi.mx, i = c - j * pi.mx;
+  double s;
+  if (indicator == 0) {
+    const int ldq = pi.mx + 2;
+    const double* __restrict__ Q = L.Q + pi.qoff;
+    const int q = (i + 1) + (j + 1) * ldq;
+    const double u = Q[q];
+    const double sdx = (Q[q + 1] - 2.0 * u) + Q[q - 1];
+    const double sdy = (Q[q + ldq] - 2.0 * u) + Q[q - ldq];
+    s = sqrt(sdx * sdx + sdy * sdy);
+  } else {
+    s = fabs(rte_tau(L, pi, i >> 1, j >> 1));
+  }
+  const int2 b = lo[tr.p];
+  const long long at = (long long)(b.x + i) + (long long)(b.y + j) * L.n[0];
+  raw[at] = s > thr ? 1 : 0;
+  cover[at] = 1;
+}
+
+// Buffer and nesting clip per level-index cell (ebTagCluster.m:78-90,
+// dilate :113-127, erode8 :130-141): a cell is allowed when it and every
+// in-domain 8-neighbour are covered; it is tagged when allowed and a raw tag
+// lies within b cells in both directions.
+__global__ void __launch_bounds__(kTile) k_tag_finish(const uint8_t* __restrict__ raw,
+                                                      const uint8_t* __restrict__ cover, int n1, int n2, int b,
+                                                      uint8_t* __restrict__ out) {
+  const long long t = (long long)blockIdx.x * kTile + threadIdx.x;
+  if (t >= (long long)n1 * n2) return;
+  const int I = (int)(t % n1), J = (int)(t / n1);
+  bool allowed = true;
+  for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int x = I + dx, y = J + dy;
+      if (x >= 0 && x < n1 && y >= 0 && y < n2) allowed = allowed && cover[x + (long long)y * n1];
+    }
+  bool tag = false;
+  if (allowed) {
+    const int x0 = max(I - b, 0), x1 = min(I + b, n1 - 1), y0 = max(J - b, 0), y1 = min(J + b, n2 - 1);
+    for (int y = y0; y <= y1 && !tag; ++y)
+      for (int x = x0; x <= x1; ++x)
+        if (raw[x + (long long)y * n1]) {
+          tag = true;
+          break;
+        }
+  }
+  out[t] = tag ? 1 : 0;
+}
+
+void launch_tag_cells(const LevelDev<double>& L, const int2* lo, int indicator, double thr, uint8_t* raw,
+                      uint8_t* cover, hipStream_t s) {
+  if (!L.ntile) return;
+  hipLaunchKernelGGL(k_tag_cells, dim3(L.ntile), dim3(kTile), 0, s, L, lo, indicator, thr, raw, cover);
+}
+
+void launch_tag_finish(const uint8_t* raw, const uint8_t* cover, int n1, int n2, int buffer, uint8_t* out,
+                       hipStream_t s) {
+  const long long n = (long long)n1 * n2;
+  if (!n) return;
+  hipLaunchKernelGGL(k_tag_finish, dim3(nblk(n)), dim3(kTile), 0, s, raw, cover, n1, n2, buffer < 0 ? 0 : buffer, out);
+}
+
 #define BSAM_INST(T)                                                                                    \
   template void launch_relax<T>(const LevelDev<T>&, const LevelDev<T>*, int, T, int, int, hipStream_t); \
   template bool tiled_fits<T>(const LevelDev<T>&);                                                      \

@@ -1728,6 +1728,46 @@ int bsam_composite_mean(bsam_hier* h, double* mean, double* area) {
   return 0;
 }
 
+int bsam_tag_level(bsam_hier* h, int k, int indicator, double threshold, int buffer, unsigned char* tags,
+                   unsigned char* raw, long long* ntag) {
+  if (int rc = check_level(h, k, 1)) return rc;
+  if (indicator != BSAM_TAG_ULAP && indicator != BSAM_TAG_RTE)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: indicator must be BSAM_TAG_ULAP or BSAM_TAG_RTE");
+  const HLevel& L = h->H[k - 1];
+  if (indicator == BSAM_TAG_RTE)
+    for (const PatchInfo& P : L.pi)
+      if ((P.mx | P.my) & 1) return fail(BSAM_E_ARG, "bsamgpu:arg: the rte indicator needs even patch widths");
+  const long long n = (long long)L.n[0] * L.n[1];
+  std::vector<int2> lo(L.np);
+  for (int p = 0; p < L.np; ++p) lo[p] = make_int2(L.box[p].ilo - 1, L.box[p].jlo - 1);
+  HIPCHK(hipSetDevice(h->ctx->device));
+  DevBufs<double> mem;
+  uint8_t* d = nullptr;
+  int2* dlo = nullptr;
+  if (int rc = mem.alloc(&d, (size_t)(3 * n))) return rc;
+  if (int rc = mem.alloc(&dlo, lo.size())) return rc;
+  HIPCHK(hipMemcpyAsync(dlo, lo.data(), lo.size() * sizeof(int2), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemsetAsync(d, 0, (size_t)(2 * n), h->stream));
+  launch_tag_cells(h->d64[k - 1], dlo, indicator, threshold, d, d + n, h->stream);
+  launch_tag_finish(d, d + n, L.n[0], L.n[1], buffer, d + 2 * n, h->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<uint8_t> out;
+  uint8_t* dst = tags;
+  if (!dst && ntag) {
+    out.resize(n);
+    dst = out.data();
+  }
+  if (dst) HIPCHK(hipMemcpyAsync(dst, d + 2 * n, (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  if (raw) HIPCHK(hipMemcpyAsync(raw, d, (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (ntag) {
+    long long c = 0;
+    for (long long i = 0; i < n; ++i) c += dst[i];
+    *ntag = c;
+  }
+  return 0;
+}
+
 long long bsam_level_field_size(const bsam_hier* h, int k, int field) {
   if (!h || k < 1 || k > h->nlev) return -1;
   const HLevel& L = h->H[k - 1];
@@ -1735,6 +1775,7 @@ long long bsam_level_field_size(const bsam_hier* h, int k, int field) {
     case BSAM_Q: return L.nq;
     case BSAM_F: case BSAM_FTRUE: return L.ncell;
     case BSAM_QC: return L.nqc;
+    case BSAM_TAGMAP: return (long long)L.n[0] * L.n[1];
     default: return -1;
   }
 }
@@ -1753,7 +1794,7 @@ static double* field_ptr(bsam_hier* h, int k, int field) {
 int bsam_get_level(bsam_hier* h, int k, int field, double* out) {
   if (int rc = check_level(h, k, 1)) return rc;
   const long long n = bsam_level_field_size(h, k, field);
-  if (n < 0 || !out) return fail(BSAM_E_ARG, "bsamgpu:arg: bad field");
+  if (n < 0 || !out || !field_ptr(h, k, field)) return fail(BSAM_E_ARG, "bsamgpu:arg: bad field");
   if (n == 0) return 0;
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(out, field_ptr(h, k, field), n * sizeof(double), hipMemcpyDeviceToHost));
@@ -1763,7 +1804,7 @@ int bsam_get_level(bsam_hier* h, int k, int field, double* out) {
 int bsam_set_level(bsam_hier* h, int k, int field, const double* in) {
   if (int rc = check_level(h, k, 1)) return rc;
   const long long n = bsam_level_field_size(h, k, field);
-  if (n < 0 || !in) return fail(BSAM_E_ARG, "bsamgpu:arg: bad field");
+  if (n < 0 || !in || !field_ptr(h, k, field)) return fail(BSAM_E_ARG, "bsamgpu:arg: bad field");
   if (n == 0) return 0;
   for (long long i = 0; i < n; ++i)
     if (!std::isfinite(in[i])) return fail(BSAM_E_NONFINITE, "eb_make_level:eval: non-finite input values.");

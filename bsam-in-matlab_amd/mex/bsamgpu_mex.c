@@ -25,6 +25,7 @@
  */
 #include <stdint.h>
 #include <stdlib.h>
+#include <ctype.h>
 #include <string.h>
 
 #include "mex.h"
@@ -391,6 +392,28 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     chk(bsam_parabolic_get_history(h, which, buf));
     plhs[0] = level_cells(h, buf, e->boxes, 0);
     mxFree(buf);
+  } else if (!strcmp(cmd, "tag")) {
+    /* [T, Traw] = bsamgpu_mex('tag', h, k, lev.n, Indicator, threshold, TagBuffer):
+       the tag loop, buffer and nesting clip of ebTagCluster.m:27-90 */
+    if (nrhs < 7 || !mxIsChar(prhs[4]))
+      mexErrMsgIdAndTxt("bsamgpu:arg", "tag(h, k, n, indicator, threshold, buffer)");
+    const int k = (int)mxGetScalar(prhs[2]);
+    char ind[8] = {0};
+    mxGetString(prhs[4], ind, sizeof ind);
+    for (char* c = ind; *c; ++c) *c = (char)tolower((unsigned char)*c);
+    const int code = !strcmp(ind, "ulap") ? BSAM_TAG_ULAP : !strcmp(ind, "rte") ? BSAM_TAG_RTE : -1;
+    const double* n = mxGetPr(prhs[3]);
+    if (mxGetNumberOfElements(prhs[3]) != 2 ||
+        bsam_level_field_size(h, k, BSAM_TAGMAP) != (long long)n[0] * (long long)n[1])
+      mexErrMsgIdAndTxt("bsamgpu:arg", "tag: n does not match level %d", k);
+    plhs[0] = mxCreateLogicalMatrix((mwSize)n[0], (mwSize)n[1]);
+    mxArray* raw = mxCreateLogicalMatrix((mwSize)n[0], (mwSize)n[1]);
+    chk(bsam_tag_level(h, k, code, mxGetScalar(prhs[5]), (int)mxGetScalar(prhs[6]),
+                       (unsigned char*)mxGetLogicals(plhs[0]), (unsigned char*)mxGetLogicals(raw), NULL));
+    if (nlhs > 1)
+      plhs[1] = raw;
+    else
+      mxDestroyArray(raw);
   } else {
     mexErrMsgIdAndTxt("bsamgpu:arg", "unknown command %s", cmd);
   }

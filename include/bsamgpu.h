@@ -52,7 +52,7 @@ extern "C" {
 #define BSAM_E_HIP (-20)       /* HIP runtime failure */
 #define BSAM_E_NOMEM (-21)
 
-enum bsam_field { BSAM_Q = 0, BSAM_F = 1, BSAM_FTRUE = 2, BSAM_QC = 3 };
+enum bsam_field { BSAM_Q = 0, BSAM_F = 1, BSAM_FTRUE = 2, BSAM_QC = 3, BSAM_TAGMAP = 4 /* size query only */ };
 enum bsam_transfer_kind { BSAM_RESTRICT = 0, BSAM_RING = 1, BSAM_CORRECT = 2, BSAM_FILLDOWN = 3 };
 enum bsam_precision { BSAM_FP64 = 0, BSAM_MIXED = 1 };
 
@@ -162,6 +162,22 @@ int bsam_reset_force(bsam_hier *h);
 int bsam_init_level_prolong(bsam_hier *h, int k);
 /* == [m, area] = ebCompositeMean(H)   EllipticBSAM/src/ebCompositeMean.m:1-24 */
 int bsam_composite_mean(bsam_hier *h, double *mean, double *area);
+/* Tagging of level k on its device-resident solution: the indicator, buffer
+   and proper-nesting clip of ebTagCluster (EllipticBSAM/src/ebTagCluster.m:27-90;
+   'rte': EllipticBSAM/src/ebRte.m:14-42).  indicator: BSAM_TAG_ULAP or
+   BSAM_TAG_RTE; a cell is tagged when its indicator exceeds threshold; tags
+   are dilated by `buffer` cells ((2b+1)^2 square, b <= 0: none) and kept only
+   where the cell and its in-domain 8-neighbours are covered by level k.
+   tags / raw (either may be NULL) receive n1*n2 bytes (0 or 1), column-major
+   in the level's index space: the final tags, and the indicator tags before
+   buffer and clip (bsam_level_field_size(h, k, BSAM_TAGMAP) = n1*n2).
+   *ntag (may be NULL): number of final tags.  Clustering
+   of the tags into boxes stays with the caller.  'rte' needs even patch
+   widths (BSAM_E_ARG otherwise). */
+#define BSAM_TAG_ULAP 0
+#define BSAM_TAG_RTE 1
+int bsam_tag_level(bsam_hier *h, int k, int indicator, double threshold, int buffer, unsigned char *tags,
+                   unsigned char *raw, long long *ntag);
 
 /* Restore the uploaded initial guess on every level (Q), F = FTrue, QC = 0:
  * restarts a solve from the descriptor's state without host traffic. */
