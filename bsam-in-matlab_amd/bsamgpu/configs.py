@@ -84,10 +84,12 @@ def varcoef_refined_options(**kw):
 STARFISH_DATA = dict(alpha=3.0, beta=2.0, gamma=1.0, kappa=0.01, g=4.0)
 
 
-def starfish_problem(epsilon=0.05, npts=3000):
+def starfish_problem(epsilon=0.05, npts=3000, device=None):
+    """``device``: evaluate the distance and the diffuse-domain fields on that
+    HIP device (bsam_polygon_fields) instead of the host."""
     from .geometry import starfish_geometry, transmission_problem
 
-    g = starfish_geometry(npts)
+    g = starfish_geometry(npts, device)
     data = dict(STARFISH_DATA)
     data["q"] = lambda x, y: 15 - x ** 2
     data["h"] = lambda x, y: 2.5 * np.sin(x) + np.exp(np.cos(y))

@@ -38,39 +38,72 @@ def _ptr(a):
 
 
 class PolygonGeometry:
-    def __init__(self, vertices, name=""):
+    """A closed, radially ordered polygon.  ``device``: a HIP device index;
+    then distances and the diffuse-domain fields are evaluated by the device
+    library (bsam_polygon_fields), else by the host helper."""
+
+    def __init__(self, vertices, name="", device=None):
         self.vertices = np.ascontiguousarray(vertices, dtype=float)
         self.name = name
+        self.device = device
         self._vx = np.ascontiguousarray(self.vertices[:, 0])
         self._vy = np.ascontiguousarray(self.vertices[:, 1])
 
-    def signed_distance(self, X, Y):
+    @staticmethod
+    def _flat(X, Y):
         X = np.asarray(X, dtype=float)
         Y = np.asarray(Y, dtype=float)
         shape = np.broadcast(X, Y).shape
         x = np.ascontiguousarray(np.broadcast_to(X, shape).reshape(-1, order="F"))
         y = np.ascontiguousarray(np.broadcast_to(Y, shape).reshape(-1, order="F"))
+        return x, y, shape
+
+    def fields(self, X, Y, epsilon=None):
+        """(d, phi, delta) at the points; phi and delta are None unless
+        ``epsilon`` is given (ddTransmissionProblem.m:14-15)."""
+        x, y, shape = self._flat(X, Y)
         d = np.empty(x.size)
-        rc = _lib().bsam_polygon_signed_distance(_ptr(self._vx), _ptr(self._vy), self._vx.size,
-                                                 _ptr(x), _ptr(y), x.size, _ptr(d))
-        if rc != 0:
-            raise BsamError("bsamgpu:geom", f"signed distance failed ({rc})")
-        return d.reshape(shape, order="F")
+        phi = delta = None
+        if self.device is None:
+            rc = _lib().bsam_polygon_signed_distance(_ptr(self._vx), _ptr(self._vy), self._vx.size,
+                                                     _ptr(x), _ptr(y), x.size, _ptr(d))
+            if rc != 0:
+                raise BsamError("bsamgpu:geom", f"signed distance failed ({rc})")
+            if epsilon is not None:
+                phi = (1 + np.tanh(d / epsilon)) / 2
+                delta = (1.0 / np.cosh(d / epsilon)) ** 2 / (2 * epsilon)
+        else:
+            from . import _lib as dev
+
+            if epsilon is not None:
+                phi = np.empty(x.size)
+                delta = np.empty(x.size)
+            null = ctypes.POINTER(ctypes.c_double)()
+            dev.check(dev.load().bsam_polygon_fields(
+                int(self.device), _ptr(self._vx), _ptr(self._vy), self._vx.size, _ptr(x), _ptr(y), x.size,
+                float(epsilon or 0.0), _ptr(d), _ptr(phi) if phi is not None else null,
+                _ptr(delta) if delta is not None else null), "bsam_polygon_fields")
+        out = [d.reshape(shape, order="F")]
+        out += [a.reshape(shape, order="F") if a is not None else None for a in (phi, delta)]
+        return tuple(out)
+
+    def signed_distance(self, X, Y):
+        return self.fields(X, Y)[0]
 
     def inside(self, X, Y):
         return self.signed_distance(X, Y) >= 0
 
 
-def starfish_geometry(n=3000):
+def starfish_geometry(n=3000, device=None):
     """r(theta) = 0.9 (1.2 + 0.7 sin 5 theta), n points incl. the repeated
-    endpoint (ddStarfishGeometry.m:8-21)."""
+    endpoint (ddStarfishGeometry.m:8-21); ``device`` as for PolygonGeometry."""
     if n < 32:
         raise BsamError("dd_starfish:n", "n must be an integer >= 32")
     theta = np.linspace(0.0, 2.0 * np.pi, n)
     r = 0.9 * (1.2 + 0.7 * np.sin(5 * theta))
     v = np.stack([r * np.cos(theta), r * np.sin(theta)], axis=1)
     v[-1, :] = v[0, :]
-    return PolygonGeometry(v, "Helmholtz starfish: r = 0.9(1.2 + 0.7 sin(5 theta))")
+    return PolygonGeometry(v, "Helmholtz starfish: r = 0.9(1.2 + 0.7 sin(5 theta))", device)
 
 
 def _tofun(v):
@@ -91,11 +124,16 @@ def transmission_problem(geom, epsilon, box, data):
         if data[f] < 0:
             raise BsamError("dd_transmission:" + f, f + " must be nonnegative")
     sd = geom.signed_distance
+    fields = getattr(geom, "fields", None)
 
     def phi(X, Y):
+        if fields is not None:
+            return fields(X, Y, epsilon)[1]
         return (1 + np.tanh(sd(X, Y) / epsilon)) / 2
 
     def delta(X, Y):
+        if fields is not None:
+            return fields(X, Y, epsilon)[2]
         return (1.0 / np.cosh(sd(X, Y) / epsilon)) ** 2 / (2 * epsilon)
 
     q = _tofun(data["q"])

@@ -35,6 +35,11 @@ template <typename T> void launch_fill(const LevelDev<T>& L, const LevelDev<T>* 
 // level-index bitmaps; lo: per patch, the box's lower corner (0-based)
 void launch_tag_cells(const LevelDev<double>& L, const int2* lo, int indicator, double thr, uint8_t* raw,
                       uint8_t* cover, hipStream_t s);
+// signed distance to a closed polygon (seg5: ax, ay, ex, ey, len2 of its ns
+// segments, back to back) and, where phi / delta are not null, the diffuse-
+// domain fields (1 + tanh(d/eps))/2 and sech(d/eps)^2/(2 eps)
+void launch_polygon_fields(const double* seg5, int ns, const double* x, const double* y, long long npts, double eps,
+                           double* d, double* phi, double* delta, hipStream_t s);
 // buffer (dilation by a (2b+1)^2 square) and proper-nesting clip of the raw tags
 void launch_tag_finish(const uint8_t* raw, const uint8_t* cover, int n1, int n2, int buffer, uint8_t* out,
                        hipStream_t s);

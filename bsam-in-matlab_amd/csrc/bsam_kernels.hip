@@ -1924,6 +1924,73 @@ void launch_tag_finish(const uint8_t* raw, const uint8_t* cover, int n1, int n2,
   hipLaunchKernelGGL(k_tag_finish, dim3(nblk(n)), dim3(kTile), 0, s, raw, cover, n1, n2, buffer < 0 ? 0 : buffer, out);
 }
 
+// ---------------------------------------------------------------- diffuse-domain geometry
+// Signed distance (positive inside) from each point to a closed, radially
+// ordered polygon, and the diffuse-domain fields of the transmission problem
+// (DiffuseDomainBSAM/src/ddStarfishGeometry.m:35-72, ddTransmissionProblem.m:14-15).
+// The squared distance is the minimum of the point-to-segment distances over
+// every segment (the reference prunes with a box tree; the minimum is the
+// same), each evaluated in the reference's expression order; segments are
+// staged through LDS in chunks shared by the workgroup.
+constexpr int kSegChunk = 256;
+
+__global__ void __launch_bounds__(kTile) k_polygon_fields(const double* __restrict__ ax, const double* __restrict__ ay,
+                                                          const double* __restrict__ ex, const double* __restrict__ ey,
+                                                          const double* __restrict__ len2, int ns,
+                                                          const double* __restrict__ x, const double* __restrict__ y,
+                                                          long long npts, double eps, double* __restrict__ d,
+                                                          double* __restrict__ phi, double* __restrict__ delta) {
+  __shared__ double sa[kSegChunk], sb[kSegChunk], se[kSegChunk], sf[kSegChunk], sl[kSegChunk];
+  const long long i = (long long)blockIdx.x * kTile + threadIdx.x;
+  const bool live = i < npts;
+  const double xi = live ? x[i] : 0.0, yi = live ? y[i] : 0.0;
+  const double twopi = 2.0 * M_PI;
+  double ang = atan2(yi, xi);
+  ang = ang - floor(ang / twopi) * twopi;  // mod(atan2(y, x), 2 pi)
+  int seg = (int)floor(ang * ns / twopi);
+  seg = min(max(seg, 0), ns - 1);
+  const double dx0 = xi - ax[seg], dy0 = yi - ay[seg];
+  double best = dx0 * dx0 + dy0 * dy0;  // a boundary vertex bounds the distance
+  for (int c0 = 0; c0 < ns; c0 += kSegChunk) {
+    const int nc = min(kSegChunk, ns - c0);
+    __syncthreads();
+    for (int j = threadIdx.x; j < nc; j += kTile) {
+      sa[j] = ax[c0 + j];
+      sb[j] = ay[c0 + j];
+      se[j] = ex[c0 + j];
+      sf[j] = ey[c0 + j];
+      sl[j] = len2[c0 + j];
+    }
+    __syncthreads();
+    for (int j = 0; j < nc; ++j) {
+      double t = ((xi - sa[j]) * se[j] + (yi - sb[j]) * sf[j]) / sl[j];
+      t = fmax(0.0, fmin(1.0, t));
+      const double px = xi - sa[j] - t * se[j];
+      const double py = yi - sb[j] - t * sf[j];
+      const double dd = px * px + py * py;
+      best = fmin(best, dd);
+    }
+  }
+  if (!live) return;
+  // the ray at the point's angle meets its enclosing segment at `radius`
+  const double ux = cos(ang), uy = sin(ang);
+  const double radius = (ax[seg] * ey[seg] - ay[seg] * ex[seg]) / (ux * ey[seg] - uy * ex[seg]);
+  const double dist = (hypot(xi, yi) > radius ? -1.0 : 1.0) * sqrt(best);
+  d[i] = dist;
+  if (phi) phi[i] = (1.0 + tanh(dist / eps)) / 2.0;
+  if (delta) {
+    const double sh = 1.0 / cosh(dist / eps);
+    delta[i] = sh * sh / (2.0 * eps);
+  }
+}
+
+void launch_polygon_fields(const double* seg5, int ns, const double* x, const double* y, long long npts, double eps,
+                           double* d, double* phi, double* delta, hipStream_t s) {
+  if (npts <= 0) return;
+  hipLaunchKernelGGL(k_polygon_fields, dim3(nblk(npts)), dim3(kTile), 0, s, seg5, seg5 + ns, seg5 + 2 * ns,
+                     seg5 + 3 * ns, seg5 + 4 * ns, ns, x, y, npts, eps, d, phi, delta);
+}
+
 #define BSAM_INST(T)                                                                                    \
   template void launch_relax<T>(const LevelDev<T>&, const LevelDev<T>*, int, T, int, int, hipStream_t); \
   template bool tiled_fits<T>(const LevelDev<T>&);                                                      \

@@ -1768,6 +1768,48 @@ int bsam_tag_level(bsam_hier* h, int k, int indicator, double threshold, int buf
   return 0;
 }
 
+int bsam_polygon_fields(int device, const double* vx, const double* vy, int nv, const double* x, const double* y,
+                        long long npts, double eps, double* d, double* phi, double* delta) {
+  if (nv < 4 || !vx || !vy || npts < 0 || (npts > 0 && (!x || !y || !d)))
+    return fail(BSAM_E_ARG, "bsamgpu:arg: bad polygon or point buffers");
+  if ((phi || delta) && !(eps > 0)) return fail(BSAM_E_ARG, "dd_transmission:epsilon: epsilon must be positive");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(BSAM_E_ARG, "bsamgpu:arg: no HIP device %d", device);
+  if (npts == 0) return 0;
+  const int ns = nv - 1;
+  std::vector<double> seg(5 * (size_t)ns);
+  for (int j = 0; j < ns; ++j) {
+    const double ex = vx[j + 1] - vx[j], ey = vy[j + 1] - vy[j];
+    seg[j] = vx[j];
+    seg[ns + j] = vy[j];
+    seg[2 * ns + j] = ex;
+    seg[3 * ns + j] = ey;
+    seg[4 * ns + j] = ex * ex + ey * ey;
+  }
+  HIPCHK(hipSetDevice(device));
+  DevBufs<double> mem;
+  double *dseg, *dx, *dy, *dd, *dphi = nullptr, *ddel = nullptr;
+  if (int rc = mem.alloc(&dseg, seg.size())) return rc;
+  if (int rc = mem.alloc(&dx, (size_t)npts)) return rc;
+  if (int rc = mem.alloc(&dy, (size_t)npts)) return rc;
+  if (int rc = mem.alloc(&dd, (size_t)npts)) return rc;
+  if (phi)
+    if (int rc = mem.alloc(&dphi, (size_t)npts)) return rc;
+  if (delta)
+    if (int rc = mem.alloc(&ddel, (size_t)npts)) return rc;
+  const size_t nb = (size_t)npts * sizeof(double);
+  HIPCHK(hipMemcpy(dseg, seg.data(), seg.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dx, x, nb, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dy, y, nb, hipMemcpyHostToDevice));
+  launch_polygon_fields(dseg, ns, dx, dy, npts, eps, dd, dphi, ddel, 0);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(d, dd, nb, hipMemcpyDeviceToHost));
+  if (phi) HIPCHK(hipMemcpy(phi, dphi, nb, hipMemcpyDeviceToHost));
+  if (delta) HIPCHK(hipMemcpy(delta, ddel, nb, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 long long bsam_level_field_size(const bsam_hier* h, int k, int field) {
   if (!h || k < 1 || k > h->nlev) return -1;
   const HLevel& L = h->H[k - 1];

@@ -179,6 +179,17 @@ int bsam_composite_mean(bsam_hier *h, double *mean, double *area);
 int bsam_tag_level(bsam_hier *h, int k, int indicator, double threshold, int buffer, unsigned char *tags,
                    unsigned char *raw, long long *ntag);
 
+/* Diffuse-domain geometry on device `device`: for npts points (x, y), the
+   signed distance d (positive inside) to the closed, radially ordered polygon
+   of nv vertices (vx, vy; the last repeats the first), as
+   geom.signedDistance of DiffuseDomainBSAM/src/ddStarfishGeometry.m:35-72, and
+   where phi / delta are not NULL the transmission fields
+   phi = (1 + tanh(d/eps))/2 and delta = sech(d/eps)^2/(2 eps) of
+   DiffuseDomainBSAM/src/ddTransmissionProblem.m:14-15.  Host buffers; no
+   hierarchy is needed. */
+int bsam_polygon_fields(int device, const double *vx, const double *vy, int nv, const double *x, const double *y,
+                        long long npts, double eps, double *d, double *phi, double *delta);
+
 /* Restore the uploaded initial guess on every level (Q), F = FTrue, QC = 0:
  * restarts a solve from the descriptor's state without host traffic. */
 int bsam_hier_reset(bsam_hier *h);
