@@ -164,6 +164,18 @@ class GpuHierarchy:
         check(self.lib.bsam_composite_mean(self.h, ctypes.byref(m), ctypes.byref(a)), "bsam_composite_mean")
         return m.value, a.value
 
+    def inherit(self, src):
+        """Copy Q and the history of every level whose boxes equal src's
+        (bsam_hier_inherit)."""
+        check(self.lib.bsam_hier_inherit(self.h, src.h), "bsam_hier_inherit")
+
+    def regrid_level(self, k, old):
+        """transferLevel of ebpRegrid.m for level k from the old hierarchy."""
+        check(self.lib.bsam_regrid_level(self.h, k, old.h), "bsam_regrid_level")
+
+    def parabolic_uo_from_q(self):
+        check(self.lib.bsam_parabolic_uo_from_q(self.h), "bsam_parabolic_uo_from_q")
+
     def tag_level(self, k, indicator, threshold, buffer):
         """Device tagging of level k (ebTagCluster.m:27-90): returns (T, Traw),
         boolean arrays over the level's index space -- the tags after buffer

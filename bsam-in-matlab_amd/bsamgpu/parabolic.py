@@ -288,154 +288,37 @@ def _source(S: _State, pp: ParabolicProblem, t1):
 
 
 # ---------------------------------------------------------------- regrid
-def _conslin(W):
-    """Conservative-linear prolongation of a padded coarse window
-    (ebpRegrid.m:95-108): centred slopes, children average to the parent."""
-    C0 = W[1:-1, 1:-1]
-    sx = (W[2:, 1:-1] - W[:-2, 1:-1]) / 2
-    sy = (W[1:-1, 2:] - W[1:-1, :-2]) / 2
-    cm, cn = C0.shape
-    F = np.zeros((2 * cm, 2 * cn))
-    F[0::2, 0::2] = C0 - sx / 4 - sy / 4
-    F[1::2, 0::2] = C0 + sx / 4 - sy / 4
-    F[0::2, 1::2] = C0 - sx / 4 + sy / 4
-    F[1::2, 1::2] = C0 + sx / 4 + sy / 4
-    return F
-
-
-def _conslin_interior(P):
-    """Interior-only conservative-linear prolongation, one-sided slopes at the
-    edges (ebpRegrid.m:111-134)."""
-    cm, cn = P.shape
-    sx = np.zeros((cm, cn))
-    sy = np.zeros((cm, cn))
-    if cm > 1:
-        sx[1:-1, :] = (P[2:, :] - P[:-2, :]) / 2
-        sx[0, :] = P[1, :] - P[0, :]
-        sx[-1, :] = P[-1, :] - P[-2, :]
-    if cn > 1:
-        sy[:, 1:-1] = (P[:, 2:] - P[:, :-2]) / 2
-        sy[:, 0] = P[:, 1] - P[:, 0]
-        sy[:, -1] = P[:, -1] - P[:, -2]
-    F = np.zeros((2 * cm, 2 * cn))
-    F[0::2, 0::2] = P - sx / 4 - sy / 4
-    F[1::2, 0::2] = P + sx / 4 - sy / 4
-    F[0::2, 1::2] = P - sx / 4 + sy / 4
-    F[1::2, 1::2] = P + sx / 4 + sy / 4
-    return F
-
-
-def _windows(G, H, k):
-    """Per patch of level k: the parent window (ebGatherWindow.m) and the
-    footprint parent cells, from the library's window map (parent patch,
-    padded local index per QC cell; -1 for unused window corners)."""
-    wm = G.export(k, 1)
-    Lc = H.L(k - 1)
-    out = []
-    off = 0
-    for p in range(H.L(k).np):
-        mx, my = H.L(k).dims(p)
-        shp = (mx // 2 + 2, my // 2 + 2)
-        n = shp[0] * shp[1]
-        m = wm[off:off + n].reshape(shp, order="F")
-        off += n
-        out.append(m)
-    return out, Lc
-
-
-def _gather(m, Lc, arrs, padded):
-    """Values of per-patch parent arrays at window-map entries m (NaN where
-    the map has no source)."""
-    W = np.full(m.shape, np.nan)
-    ok = m >= 0
-    q = m[ok] // 1000000
-    li = m[ok] % 1000000
-    vals = np.empty(q.size)
-    for qq in np.unique(q):
-        sel = q == qq
-        ld = Lc.dims(int(qq))[0] + 2
-        i = li[sel] % ld
-        j = li[sel] // ld
-        if padded:
-            vals[sel] = arrs[int(qq)][i, j]
-        else:
-            vals[sel] = arrs[int(qq)][i - 1, j - 1]
-    W[ok] = vals
-    return W
-
-
-def _transfer_level(Hn, G, old, kN, UooN, have_uoo):
-    """transferLevel (ebpRegrid.m:62-92): conservative-linear prolongation
-    from the transferred parent, overwritten by old same-level data."""
-    lev = Hn.L(kN)
-    maps, Lc = _windows(G, Hn, kN)
-    row = []
-    for p in range(lev.np):
-        W = _gather(maps[p], Lc, Lc.Q, True)
-        Qf = _conslin(W)
-        if have_uoo:
-            Pf = _gather(maps[p][1:-1, 1:-1], Lc, UooN[kN - 2], False)
-            Uf = _conslin_interior(Pf)
-        b = lev.box[p]
-        if kN <= len(old["Q"]):
-            for q, qb in enumerate(old["box"][kN - 1]):
-                r = [max(b[0], qb[0]), min(b[1], qb[1]), max(b[2], qb[2]), min(b[3], qb[3])]
-                if r[0] > r[1] or r[2] > r[3]:
-                    continue
-                di = slice(r[0] - b[0], r[1] - b[0] + 1)
-                dj = slice(r[2] - b[2], r[3] - b[2] + 1)
-                si = slice(r[0] - qb[0], r[1] - qb[0] + 1)
-                sj = slice(r[2] - qb[2], r[3] - qb[2] + 1)
-                Qf[di, dj] = old["Q"][kN - 1][q][1:-1, 1:-1][si, sj]
-                if have_uoo:
-                    Uf[di, dj] = old["Uoo"][kN - 1][q][si, sj]
-        if not np.all(np.isfinite(Qf)):
-            raise BsamError("ebp_regrid:window", "parent window incomplete for the new level")
-        lev.Q[p][1:-1, 1:-1] = Qf
-        if have_uoo:
-            row.append(Uf)
-    if have_uoo:
-        UooN.append(row)
-
-
 def ebp_regrid(S: _State, pp: ParabolicProblem, t, aDtNext, opts: ParabolicOptions):
     """ebpRegrid.m:15-59: rebuild around the current solution and transfer
-    the state conservatively; returns the new state."""
+    the state conservatively; returns the new state.  The old hierarchy stays
+    on the device until the transfer is done: the root copy, the sub-root
+    restriction, the per-level conservative-linear transfer of Q and Uoo
+    (bsam_regrid_level), tagging and the history all run there; the host
+    builds the new levels' boxes and coefficients."""
     Hold, Gold = S.H, S.G
-    have_uoo = True  # regrids happen after at least one accepted step
-    old = {"Q": [Gold.get_field(k, _lib.BSAM_Q) for k in range(1, Hold.nlev + 1)],
-           "box": [np.array(Hold.L(k).box) for k in range(1, Hold.nlev + 1)],
-           "Uoo": Gold.get_history(1)}
-    Gold.close()
     Hn = hierarchy.setup(eprob(pp, t, aDtNext), opts.eb)
     r = Hn.rootIdx
-    Hn.L(r).Q[0][1:-1, 1:-1] = old["Q"][r - 1][0][1:-1, 1:-1]
-    UooN = [[] for _ in range(r - 1)] + [[old["Uoo"][r - 1][0].copy()]]
-    for k in range(r - 1, 0, -1):
-        Hn.L(k).Q[0][1:-1, 1:-1] = hierarchy.restrict2(Hn.L(k + 1).Q[0][1:-1, 1:-1])
     G = GpuHierarchy(Hn)
+    G.regrid_level(r, Gold)
+    for k in range(r, 1, -1):  # sub-root levels: restriction of the root
+        G.transfer(k, "restrict")
     for k in range(1, r + 1):
         G.fill_ghosts(k)
-    G.pull_q(Hn)
     while Hn.nlev - Hn.rootIdx + 1 < opts.eb.MaxLevels:
         boxes = tagcluster.tag_cluster(Hn, device=G)
         if not boxes:
             break
-        kN, G = _add_level(Hn, G, boxes)
-        _transfer_level(Hn, G, old, kN, UooN, have_uoo)
-        G.set_field(kN, _lib.BSAM_Q, Hn.L(kN).Q)
+        kN = hierarchy.add_level(Hn, boxes)
+        Gn = GpuHierarchy(Hn)
+        Gn.inherit(G)
+        G.close()
+        G = Gn
+        G.regrid_level(kN, Gold)
         G.fill_ghosts(kN)
-        G.pull_q(Hn)
+    Gold.close()
     G.transfer(0, "filldown")
-    G.parabolic_init()
-    # sub-root levels carry no history (their right-hand side is never read)
-    full = []
-    for k in range(1, Hn.nlev + 1):
-        if k < r:
-            full.append([np.zeros(Hn.L(k).dims(p)) for p in range(Hn.L(k).np)])
-        else:
-            full.append(UooN[k - 1])
-    G.set_history(Uoo=full)
+    G.parabolic_uo_from_q()
+    G.pull_q(Hn)
     Hn.stats = Hold.stats
     td = pp.tdD or pp.tdC or pp.tdBC
     return _State(Hn, G, aDtNext, (t * td, aDtNext))

@@ -98,7 +98,12 @@ template <typename T> void launch_fill_value(T* a, long long n, T v, hipStream_t
 template <typename TD, typename TS> void launch_convert(TD* d, const TS* src, long long n, hipStream_t s);
 void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double* Uoo, double dt, int first,
                      long long n, hipStream_t s);
+// rotate 0: Uoo = Uo = interior(Q); 1: Uoo = Uo, Uo = interior(Q); 2: Uo = interior(Q)
 void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rotate, hipStream_t s);
+// one level's regrid transfer (Q interior and, with hist, its history)
+void launch_regrid_level(const LevelDev<double>& L, const double* Pq, int conslin, int hist, double* Uoo_k,
+                         const double* Uoo_all, const int* fc, const int* oq, const int* oc, const double* Qold,
+                         const double* Uoo_old, hipStream_t s);
 // one workgroup (1024 threads) per item: the item's preamble or whole cycle;
 // max_m: largest coarsest order among the items (LDS right side)
 void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre, int npost, double omega,

@@ -1599,7 +1599,8 @@ __global__ void __launch_bounds__(kTile) k_bdf2_rhs(double* __restrict__ FT, con
   FT[t] = first ? fv + Uo[t] / dt : fv + (2.0 * Uo[t] - 0.5 * Uoo[t]) / dt;
 }
 
-// history rotation: Uoo = Uo; Uo = interior(Q)
+// history: rotate 0: Uoo = Uo = interior(Q); 1: Uoo = Uo, Uo = interior(Q);
+// 2: Uo = interior(Q), Uoo kept
 __global__ void __launch_bounds__(kTile) k_history(LevelDev<double> L, double* __restrict__ Uo,
                                                    double* __restrict__ Uoo, int rotate) {
   const TileRec& tr = L.tiles[blockIdx.x];
@@ -1611,8 +1612,69 @@ __global__ void __launch_bounds__(kTile) k_history(LevelDev<double> L, double* _
   const int i = c - j * mx;
   const double u = L.Q[pi.qoff + (i + 1) + (j + 1) * (mx + 2)];
   const int cc = pi.coff + c;
-  Uoo[cc] = rotate ? Uo[cc] : u;
+  if (rotate != 2) Uoo[cc] = rotate ? Uo[cc] : u;
   Uo[cc] = u;
+}
+
+// ---------------------------------------------------------------- regrid transfer
+// Conservative-linear child value (ebpRegrid.m:95-108): the parent value with
+// a quarter of each slope subtracted (first child along the axis) or added.
+__device__ __forceinline__ double conslin_child(double c0, double sx, double sy, int odd_i, int odd_j) {
+  const double a = odd_i ? c0 + sx / 4 : c0 - sx / 4;
+  return odd_j ? a + sy / 4 : a - sy / 4;
+}
+
+// transferLevel (ebpRegrid.m:62-92) for one cell of a new level: Q from the
+// conservative-linear prolongation of the parent window (centred slopes), the
+// history from that of the parent footprint (one-sided slopes at its edges,
+// :111-134), both overwritten by the old hierarchy's same-level values where
+// an old patch covers the cell.  oq / oc: the old Q (padded) and history
+// index of the cell or -1; fc: per footprint cell, the parent's history index.
+__global__ void __launch_bounds__(kTile) k_regrid_level(LevelDev<double> L, const double* __restrict__ Pq,
+                                                        int conslin, int hist, double* __restrict__ Uoo_k,
+                                                        const double* __restrict__ Uoo_all,
+                                                        const int* __restrict__ fc, const int* __restrict__ oq,
+                                                        const int* __restrict__ oc, const double* __restrict__ Qold,
+                                                        const double* __restrict__ Uoo_old) {
+  const TileRec& tr = L.tiles[blockIdx.x];
+  const PatchInfo& pi = tr.pi;
+  const int mx = pi.mx, my = pi.my;
+  const int c = tr.start + (int)threadIdx.x;
+  if (c >= mx * my) return;
+  const int j = cell_row(c, mx, pi.magic);
+  const int i = c - j * mx;
+  const int cc = pi.coff + c;
+  double q = 0.0, u = 0.0;
+  if (conslin) {
+    const int ci = i >> 1, cj = j >> 1, ldc = mx / 2 + 2;
+    auto w = [&](int a, int b) { return Pq[L.wmap[pi.qcoff + a + b * ldc]]; };
+    const double c0 = w(ci + 1, cj + 1);
+    const double sx = (w(ci + 2, cj + 1) - w(ci, cj + 1)) / 2;
+    const double sy = (w(ci + 1, cj + 2) - w(ci + 1, cj)) / 2;
+    q = conslin_child(c0, sx, sy, i & 1, j & 1);
+    if (hist) {
+      const int cmx = mx / 2, cmy = my / 2, base = pi.coff / 4;
+      auto pv = [&](int a, int b) { return Uoo_all[fc[base + a + b * cmx]]; };
+      const double p0 = pv(ci, cj);
+      double px = 0.0, py = 0.0;
+      if (cmx > 1)
+        px = ci == 0 ? pv(1, cj) - pv(0, cj)
+             : ci == cmx - 1 ? pv(cmx - 1, cj) - pv(cmx - 2, cj)
+                             : (pv(ci + 1, cj) - pv(ci - 1, cj)) / 2;
+      if (cmy > 1)
+        py = cj == 0 ? pv(ci, 1) - pv(ci, 0)
+             : cj == cmy - 1 ? pv(ci, cmy - 1) - pv(ci, cmy - 2)
+                             : (pv(ci, cj + 1) - pv(ci, cj - 1)) / 2;
+      u = conslin_child(p0, px, py, i & 1, j & 1);
+    }
+  }
+  const int o = oq[cc];
+  if (o >= 0) {
+    q = Qold[o];
+    if (hist) u = Uoo_old[oc[cc]];
+  }
+  L.Q[pi.qoff + (i + 1) + (j + 1) * (mx + 2)] = q;
+  if (hist) Uoo_k[cc] = u;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1798,6 +1860,14 @@ void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double
 void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rotate, hipStream_t s) {
   if (!L.ntile) return;
   hipLaunchKernelGGL(k_history, dim3(L.ntile), dim3(kTile), 0, s, L, Uo, Uoo, rotate);
+}
+
+void launch_regrid_level(const LevelDev<double>& L, const double* Pq, int conslin, int hist, double* Uoo_k,
+                         const double* Uoo_all, const int* fc, const int* oq, const int* oc, const double* Qold,
+                         const double* Uoo_old, hipStream_t s) {
+  if (!L.ntile) return;
+  hipLaunchKernelGGL(k_regrid_level, dim3(L.ntile), dim3(kTile), 0, s, L, Pq, conslin, hist, Uoo_k, Uoo_all, fc, oq,
+                     oc, Qold, Uoo_old);
 }
 
 void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre, int npost, double omega,

@@ -25,6 +25,9 @@
 
 using namespace bsam;
 
+// explicit inverse of the level-1 matrix (bsam_lu.cpp, host code)
+bool bsam_lu_inverse(double* A, int M, double* Ainv);
+
 namespace {
 
 thread_local std::string g_err;
@@ -838,44 +841,9 @@ int coarsest_assemble(const HLevel& L, bool singular, std::vector<double>& Ainv,
       at(N, c) = 1.0;
     }
   }
-  // LU with partial pivoting, first maximum wins
-  std::vector<int> perm(M);
-  for (int i = 0; i < M; ++i) perm[i] = i;
-  for (int k = 0; k < M; ++k) {
-    int piv = k;
-    double best = std::fabs(at(k, k));
-    for (int i = k + 1; i < M; ++i)
-      if (std::fabs(at(i, k)) > best) { best = std::fabs(at(i, k)); piv = i; }
-    if (at(piv, k) == 0.0) return fail(BSAM_E_SINGULAR, "bsamgpu:coarsest: level-1 matrix is singular");
-    if (piv != k) {
-      for (int c = 0; c < M; ++c) std::swap(at(k, c), at(piv, c));
-      std::swap(perm[k], perm[piv]);
-    }
-    const double akk = at(k, k);
-    for (int i = k + 1; i < M; ++i) at(i, k) = at(i, k) / akk;
-    for (int i = k + 1; i < M; ++i) {
-      const double l = at(i, k);
-      for (int c = k + 1; c < M; ++c) at(i, c) = at(i, c) - l * at(k, c);
-    }
-  }
-  std::vector<double> X((size_t)M * M, 0.0);  // row-major
-  for (int i = 0; i < M; ++i) X[(size_t)i * M + perm[i]] = 1.0;
-  for (int k = 0; k < M; ++k)
-    for (int i = k + 1; i < M; ++i) {
-      const double l = at(i, k);
-      for (int c = 0; c < M; ++c) X[(size_t)i * M + c] = X[(size_t)i * M + c] - l * X[(size_t)k * M + c];
-    }
-  for (int k = M - 1; k >= 0; --k) {
-    const double ukk = at(k, k);
-    for (int c = 0; c < M; ++c) X[(size_t)k * M + c] = X[(size_t)k * M + c] / ukk;
-    for (int i = 0; i < k; ++i) {
-      const double u = at(i, k);
-      for (int c = 0; c < M; ++c) X[(size_t)i * M + c] = X[(size_t)i * M + c] - u * X[(size_t)k * M + c];
-    }
-  }
-  Ainv.assign((size_t)M * M, 0.0);  // column-major for the device
-  for (int i = 0; i < M; ++i)
-    for (int c = 0; c < M; ++c) Ainv[i + (size_t)c * M] = X[(size_t)i * M + c];
+  Ainv.assign((size_t)M * M, 0.0);
+  if (!bsam_lu_inverse(A.data(), M, Ainv.data()))
+    return fail(BSAM_E_SINGULAR, "bsamgpu:coarsest: level-1 matrix is singular");
   return 0;
 }
 
@@ -1044,7 +1012,8 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
       // (the sweep kernel holds three 256-lane workgroups per CU); a larger
       // level runs colour passes, whose launches are all resident at once
       std::vector<SweepTile> st;
-      if (build_sweep_tiles(L, st) && (long long)st.size() <= (long long)kSweepWGsPerCU * h->ncu) {
+      const long long round = (long long)kSweepWGsPerCU * h->ncu;
+      if (build_sweep_tiles(L, st) && ((long long)st.size() <= round || (long long)st.size() >= 16 * round)) {
         if ((rc = upload(h, &D.sweep, st))) return rc;
         D.nsweep = (int)st.size();
       }
@@ -2114,6 +2083,122 @@ int bsam_parabolic_init(bsam_hier* h) {
   for (int k = 0; k < h->nlev; ++k)
     launch_history(h->d64[k], h->Uo + h->cell_base[k], h->Uoo + h->cell_base[k], 0, h->stream);
   h->have_uoo = false;
+  return sync(h);
+}
+
+// ------------------------------------------------------------------ regrid
+static bool same_level(const HLevel& a, const HLevel& b) {
+  if (a.n[0] != b.n[0] || a.n[1] != b.n[1] || a.np != b.np) return false;
+  for (int p = 0; p < a.np; ++p)
+    if (a.box[p].ilo != b.box[p].ilo || a.box[p].ihi != b.box[p].ihi || a.box[p].jlo != b.box[p].jlo ||
+        a.box[p].jhi != b.box[p].jhi)
+      return false;
+  return true;
+}
+
+int bsam_hier_inherit(bsam_hier* h, const bsam_hier* src) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (!src || src->ctx->device != h->ctx->device)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: source hierarchy missing or on another device");
+  int same = 0;
+  while (same < std::min(h->nlev, src->nlev) && same_level(h->H[same], src->H[same])) ++same;
+  HIPCHK(hipSetDevice(h->ctx->device));
+  HIPCHK(hipStreamSynchronize(src->stream));
+  for (int k = 0; k < same; ++k)
+    HIPCHK(hipMemcpyAsync(h->d64[k].Q, src->d64[k].Q, (size_t)h->H[k].nq * sizeof(double), hipMemcpyDeviceToDevice,
+                          h->stream));
+  if (src->Uo && same > 0) {
+    if (int rc = ensure_history(h)) return rc;
+    const size_t nb = (size_t)h->cell_base[same] * sizeof(double);
+    HIPCHK(hipMemcpyAsync(h->Uo, src->Uo, nb, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->Uoo, src->Uoo, nb, hipMemcpyDeviceToDevice, h->stream));
+    h->have_uoo = src->have_uoo;
+  }
+  return sync(h);
+}
+
+int bsam_regrid_level(bsam_hier* h, int k, const bsam_hier* old) {
+  if (int rc = check_level(h, k, h->root)) return rc;
+  if (!old || old->ctx->device != h->ctx->device)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: old hierarchy missing or on another device");
+  const HLevel& L = h->H[k - 1];
+  const bool conslin = k > h->root;
+  const bool hist = old->Uo && old->have_uoo;
+  if (conslin && hist && !h->Uo)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: the parent level's history has not been transferred");
+  if (hist)
+    if (int rc = ensure_history(h)) return rc;
+  // cells an old patch of the same level covers (ebpRegrid.m:72-88)
+  std::vector<int> oq(std::max(1, L.ncell), -1), oc(std::max(1, L.ncell), -1);
+  if (k <= old->nlev) {
+    const HLevel& O = old->H[k - 1];
+    if (O.n[0] != L.n[0] || O.n[1] != L.n[1]) return fail(BSAM_E_ARG, "bsamgpu:arg: old level %d differs in size", k);
+    for (int p = 0; p < L.np; ++p) {
+      const Box& b = L.box[p];
+      const PatchInfo& P = L.pi[p];
+      for (int q = 0; q < O.np; ++q) {
+        const Box& qb = O.box[q];
+        const PatchInfo& Qp = O.pi[q];
+        const int i0 = std::max(b.ilo, qb.ilo), i1 = std::min(b.ihi, qb.ihi);
+        const int j0 = std::max(b.jlo, qb.jlo), j1 = std::min(b.jhi, qb.jhi);
+        for (int j = j0; j <= j1; ++j)
+          for (int i = i0; i <= i1; ++i) {
+            const int c = P.coff + (i - b.ilo) + (j - b.jlo) * P.mx;
+            oq[c] = Qp.qoff + (i - qb.ilo + 1) + (j - qb.jlo + 1) * (Qp.mx + 2);
+            oc[c] = (int)old->cell_base[k - 1] + Qp.coff + (i - qb.ilo) + (j - qb.jlo) * Qp.mx;
+          }
+      }
+    }
+  }
+  if (!conslin)
+    for (int c = 0; c < L.ncell; ++c)
+      if (oq[c] < 0) return fail(BSAM_E_ARG, "bsamgpu:arg: the old root does not cover the new root");
+  // parent history index of every footprint cell (parentFootprint, ebpRegrid.m:140-151)
+  std::vector<int> fc(1, 0);
+  if (conslin && hist) {
+    const HLevel& Pl = h->H[k - 2];
+    fc.assign(std::max(1, L.ncell / 4), -1);
+    for (int p = 0; p < L.np; ++p) {
+      const PatchInfo& P = L.pi[p];
+      const int cmx = P.mx / 2, cmy = P.my / 2, ldc = cmx + 2;
+      for (int cj = 0; cj < cmy; ++cj)
+        for (int ci = 0; ci < cmx; ++ci) {
+          const int w = P.qcoff + (ci + 1) + (cj + 1) * ldc;
+          const int q = L.wmap_patch[w];
+          if (q < 0) return fail(BSAM_E_ARG, "bsamgpu:internal: footprint cell without a parent");
+          const PatchInfo& Q = Pl.pi[q];
+          const int loc = L.wmap[w] - Q.qoff, li = loc % (Q.mx + 2), lj = loc / (Q.mx + 2);
+          if (li < 1 || li > Q.mx || lj < 1 || lj > Q.my)
+            return fail(BSAM_E_ARG, "bsamgpu:internal: footprint cell in a parent ghost");
+          fc[P.coff / 4 + ci + cj * cmx] = (int)h->cell_base[k - 2] + Q.coff + (li - 1) + (lj - 1) * Q.mx;
+        }
+    }
+  }
+  HIPCHK(hipSetDevice(h->ctx->device));
+  HIPCHK(hipStreamSynchronize(old->stream));
+  DevBufs<double> mem;
+  int *doq, *doc, *dfc;
+  if (int rc = mem.alloc(&doq, oq.size())) return rc;
+  if (int rc = mem.alloc(&doc, oc.size())) return rc;
+  if (int rc = mem.alloc(&dfc, fc.size())) return rc;
+  HIPCHK(hipMemcpyAsync(doq, oq.data(), oq.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(doc, oc.data(), oc.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemcpyAsync(dfc, fc.data(), fc.size() * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  launch_regrid_level(h->d64[k - 1], conslin ? h->d64[k - 2].Q : nullptr, conslin ? 1 : 0, hist ? 1 : 0,
+                      hist ? h->Uoo + h->cell_base[k - 1] : nullptr, h->Uoo, dfc, doq, doc,
+                      k <= old->nlev ? old->d64[k - 1].Q : nullptr, old->Uoo, h->stream);
+  if (hist) h->have_uoo = true;
+  return sync(h);
+}
+
+int bsam_parabolic_uo_from_q(bsam_hier* h) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (int rc = ensure_history(h)) return rc;
+  for (int k = 0; k < h->nlev; ++k) {
+    launch_history(h->d64[k], h->Uo + h->cell_base[k], h->Uoo + h->cell_base[k], 2, h->stream);
+    if (k + 1 < h->root)
+      launch_fill_value<double>(h->Uoo + h->cell_base[k], h->cell_base[k + 1] - h->cell_base[k], 0.0, h->stream);
+  }
   return sync(h);
 }
 

@@ -238,6 +238,22 @@ int bsam_export(const bsam_hier *h, int k, int what, long long *out);
  * are not read.  Nothing is changed when an argument is rejected. */
 int bsam_update_coeffs(bsam_hier *h, const bsam_hier_desc *desc);
 
+/* Device state of a hierarchy rebuilt with more levels (ebAddLevel.m:10-40
+   keeps the existing levels): for every level k whose boxes equal those of
+   src, Q (with ghosts) and the parabolic history Uo/Uoo are copied device to
+   device from src.  Both on the same device. */
+int bsam_hier_inherit(bsam_hier *h, const bsam_hier *src);
+/* transferLevel of ParabolicBSAM/src/ebpRegrid.m:62-151 for level k of the new
+   hierarchy h (k = root: the root copied from old, ebpRegrid.m:20-25):
+   conservative-linear prolongation of the parent window into Q's interior
+   and, when old carries Uoo, of the parent footprint's history into Uoo,
+   both overwritten by old's level-k values wherever an old patch covers the
+   cell.  Levels below k must already be transferred; ghosts are left to
+   bsam_fill_ghosts. */
+int bsam_regrid_level(bsam_hier *h, int k, const bsam_hier *old);
+/* H.par.Uo = interiors(H) after a regrid (ebpRegrid.m:48-55); Uoo keeps the
+   transferred history and is zero below the root. */
+int bsam_parabolic_uo_from_q(bsam_hier *h);
 /* Parabolic stepping (ParabolicBSAM/src/ebpSolve.m:23-78).  The history
  * H.par.Uo = u^n, H.par.Uoo = u^{n-1} and the source f(t1) are device-resident
  * arrays packed like FTrue over all levels 1..nlev. */
