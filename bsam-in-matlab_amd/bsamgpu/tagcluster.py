@@ -353,9 +353,39 @@ def tag_cluster(H, with_info=False, device=None):
     info["allowed"] = allowed
     if not T.any():
         return ([], info) if with_info else []
-    boxes = br_cluster(T, opts.MinBoxWidth, opts.MinFillRatio)
-    boxes = clip_allowed(boxes, T, allowed)
-    boxes = grow_min(boxes, allowed, opts.MinBoxWidth, n)
-    boxes = merge_or_drop_thin(boxes, allowed)
+    if device is not None:
+        boxes = cluster_native(T, allowed, opts.MinBoxWidth, opts.MinFillRatio)
+    else:
+        boxes = br_cluster(T, opts.MinBoxWidth, opts.MinFillRatio)
+        boxes = clip_allowed(boxes, T, allowed)
+        boxes = grow_min(boxes, allowed, opts.MinBoxWidth, n)
+        boxes = merge_or_drop_thin(boxes, allowed)
     boxes = [[int(v) for v in bb] for bb in boxes]
     return (boxes, info) if with_info else boxes
+
+
+def cluster_native(T, allowed, min_width, fill_ratio):
+    """The library's clustering (bsam_cluster_tags, host C++): the same steps
+    as br_cluster -> clip_allowed -> grow_min -> merge_or_drop_thin."""
+    import ctypes
+
+    from . import _lib
+
+    lib = _lib.load()
+    t = np.asfortranarray(T, dtype=np.uint8)
+    a = np.asfortranarray(allowed, dtype=np.uint8)
+    n1, n2 = t.shape
+    nb, nd = ctypes.c_int(), ctypes.c_int()
+    cap = 64
+    while True:
+        buf = np.zeros(4 * cap, dtype=np.int32)
+        rc = lib.bsam_cluster_tags(t.ctypes.data, a.ctypes.data, n1, n2, int(min_width), float(fill_ratio),
+                                   buf.ctypes.data_as(_lib.P_int), cap, ctypes.byref(nb), ctypes.byref(nd))
+        if rc != 0 and nb.value > cap:
+            cap = nb.value
+            continue
+        _lib.check(rc, "bsam_cluster_tags")
+        break
+    for _ in range(nd.value):
+        warnings.warn("eb_tag_cluster:dropthin: dropping an ungrowable sliver cluster box")
+    return buf[:4 * nb.value].reshape(-1, 4).tolist()

@@ -27,6 +27,10 @@ using namespace bsam;
 
 // explicit inverse of the level-1 matrix (bsam_lu.cpp, host code)
 bool bsam_lu_inverse(double* A, int M, double* Ainv);
+// tag clustering (bsam_cluster.cpp, host code): returns the box count and
+// writes at most cap boxes
+int bsam_cluster_impl(const unsigned char* tags, const unsigned char* allowed, int n1, int n2, int min_width,
+                      double fill_ratio, int* boxes, int cap, int* ndropped);
 
 namespace {
 
@@ -1776,6 +1780,17 @@ int bsam_polygon_fields(int device, const double* vx, const double* vy, int nv, 
   HIPCHK(hipMemcpy(d, dd, nb, hipMemcpyDeviceToHost));
   if (phi) HIPCHK(hipMemcpy(phi, dphi, nb, hipMemcpyDeviceToHost));
   if (delta) HIPCHK(hipMemcpy(delta, ddel, nb, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int bsam_cluster_tags(const unsigned char* tags, const unsigned char* allowed, int n1, int n2, int min_width,
+                      double fill_ratio, int* boxes, int cap, int* nboxes, int* ndropped) {
+  if (!tags || !allowed || n1 < 1 || n2 < 1 || cap < 0 || (cap > 0 && !boxes) || !nboxes)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: bad tag map or box buffer");
+  if (min_width < 1) return fail(BSAM_E_ARG, "eb_options:minbox: MinBoxWidth must be a positive integer");
+  const int n = bsam_cluster_impl(tags, allowed, n1, n2, min_width, fill_ratio, boxes, cap, ndropped);
+  *nboxes = n;
+  if (n > cap) return fail(BSAM_E_ARG, "bsamgpu:arg: %d boxes do not fit the buffer of %d", n, cap);
   return 0;
 }
 

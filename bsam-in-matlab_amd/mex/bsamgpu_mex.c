@@ -316,6 +316,33 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     *(uint64_t*)mxGetData(plhs[0]) = (uint64_t)(uintptr_t)h;
     return;
   }
+  if (!strcmp(cmd, "cluster")) {
+    /* boxes = bsamgpu_mex('cluster', T, allowed, MinBoxWidth, MinFillRatio):
+       the clustering half of ebTagCluster.m:92-110 (host code, no handle) */
+    if (nrhs < 5 || !mxIsLogical(prhs[1]) || !mxIsLogical(prhs[2]))
+      mexErrMsgIdAndTxt("bsamgpu:arg", "cluster(T, allowed, minw, fill) with logical T and allowed");
+    const int n1 = (int)mxGetM(prhs[1]), n2 = (int)mxGetN(prhs[1]);
+    if ((int)mxGetM(prhs[2]) != n1 || (int)mxGetN(prhs[2]) != n2)
+      mexErrMsgIdAndTxt("bsamgpu:arg", "cluster: T and allowed differ in size");
+    int cap = 64, nb = 0, nd = 0, rc;
+    int* buf = NULL;
+    for (;;) {
+      buf = (int*)mxCalloc((size_t)cap * 4, sizeof(int));
+      rc = bsam_cluster_tags((const unsigned char*)mxGetLogicals(prhs[1]), (const unsigned char*)mxGetLogicals(prhs[2]),
+                             n1, n2, (int)mxGetScalar(prhs[3]), mxGetScalar(prhs[4]), buf, cap, &nb, &nd);
+      if (rc == 0 || nb <= cap) break;
+      mxFree(buf);
+      cap = nb;
+    }
+    chk(rc);
+    if (nd > 0) mexWarnMsgIdAndTxt("eb_tag_cluster:dropthin", "dropping %d ungrowable sliver cluster box(es)", nd);
+    plhs[0] = mxCreateDoubleMatrix((mwSize)nb, 4, mxREAL);
+    double* out = mxGetPr(plhs[0]);
+    for (int b = 0; b < nb; ++b)
+      for (int c = 0; c < 4; ++c) out[b + (size_t)c * nb] = buf[4 * b + c];
+    mxFree(buf);
+    return;
+  }
   if (nrhs < 2) mexErrMsgIdAndTxt("bsamgpu:arg", "%s needs a handle", cmd);
   bsam_hier* h = handle(prhs[1]);
   entry* e = find(h);

@@ -179,6 +179,18 @@ int bsam_composite_mean(bsam_hier *h, double *mean, double *area);
 int bsam_tag_level(bsam_hier *h, int k, int indicator, double threshold, int buffer, unsigned char *tags,
                    unsigned char *raw, long long *ntag);
 
+/* Clustering of a tag map into patch boxes (EllipticBSAM/src/ebTagCluster.m:92-380):
+   Berger-Rigoutsos signature splitting, bisection of boxes that leave the
+   allowed (proper-nesting) region, growth to min_width (MinBoxWidth) and the
+   merge or drop of slivers thinner than 2 cells, with MATLAB's tie-breaking.
+   tags, allowed: n1*n2 bytes, column-major (bsam_tag_level's output and the
+   level's erode8 coverage).  fill_ratio: MinFillRatio.  boxes: room for cap
+   boxes of 4 ints, 1-based inclusive [ilo ihi jlo jhi] in the level's index
+   space; *nboxes: the number of boxes (BSAM_E_ARG when above cap);
+   *ndropped (may be NULL): slivers dropped.  Host code: needs no device. */
+int bsam_cluster_tags(const unsigned char *tags, const unsigned char *allowed, int n1, int n2, int min_width,
+                      double fill_ratio, int *boxes, int cap, int *nboxes, int *ndropped);
+
 /* Diffuse-domain geometry on device `device`: for npts points (x, y), the
    signed distance d (positive inside) to the closed, radially ordered polygon
    of nv vertices (vx, vy; the last repeats the first), as
