@@ -113,8 +113,10 @@ def cpu_baseline(H):
 
 
 def bench_large_smoother(n, launches=40):
-    """The same smoother kernel on one uniform n x n level (D = C = 1): large
-    enough to be HBM-bound rather than launch-bound."""
+    """The smoother on one uniform n x n level (D = C = 1), large enough to be
+    HBM-bound rather than launch-bound: the colour pass (half a sweep per
+    launch) and the streaming sweep (a whole red+black sweep per launch),
+    which the solve uses on such levels."""
     from bsamgpu import Options, Problem, hierarchy
     from bsamgpu.device import GpuHierarchy
 
@@ -123,13 +125,13 @@ def bench_large_smoother(n, launches=40):
     G = GpuHierarchy(H)
     k = H.nlev
     out = {"cells": n * n}
-    for form in ("exact", "sweep"):
+    for form in ("passes", "exact"):
         G.solve_cycles(max_vcycles=0, smoother=form)  # selects the timed form
         for name, mixed, b in (("fp64", False, 8), ("fp32", True, 4)):
             G.time_smoother(k, 5, mixed=mixed)
             ms = G.time_smoother(k, launches, mixed=mixed)
             byt = G.smoother_bytes(k, b)
-            key = name if form == "exact" else f"{name}_sweep"
+            key = f"{name}_colour_pass" if form == "passes" else f"{name}_streaming_sweep"
             out[key] = {"us_per_launch": round(ms * 1e3, 2), "bytes_per_launch": byt,
                         "GBps": round(byt / ms / 1e6, 1), "frac": round(byt / ms / 1e6 / HBM_PEAK_GBS, 4)}
     G.close()

@@ -101,7 +101,9 @@ class Options:
     # 'tiled' runs every colour pass of a relax call in one LDS-staged launch
     # (same converged solution, sibling ghosts frozen within a call); 'sweep'
     # is 'exact' with pairs of red+black sweeps fused into one launch each on
-    # levels whose tiles fit one round of workgroups (bit-identical)
+    # levels whose tiles fit one round of workgroups (bit-identical); 'passes'
+    # runs colour passes everywhere ('exact' streams whole sweeps on large
+    # one-patch levels with physical sides, bit-identical)
     Smoother: str = "exact"
 
     def __post_init__(self):

@@ -137,6 +137,7 @@ struct LevelDev {
   int ntile, nctile, nghost, nring_all, nring_cf, nfoot, ncorr, ntgt;
   int nstile, stile_lds;  // smoother tiles and their largest LDS region (cells)
   STile* stiles;
+  int stream;             // one patch of even width, physical on every side: streaming sweeps apply
   int nsweep;             // fused sweep tiles (0: the level runs colour passes)
   SweepTile* sweep;
   T* Qs;                  // the sweep's second copy of Q (same layout)

@@ -24,6 +24,8 @@
 //   coarsest         EllipticBSAM/src/ebCoarsest.m:79-91
 //   residual         EllipticBSAM/src/ebCompResidual.m:271-332
 //   composite mean   EllipticBSAM/src/ebCompositeMean.m:4-23
+#include <algorithm>
+
 #include "bsam_kernels.h"
 
 namespace bsam {
@@ -573,6 +575,150 @@ __global__ void __launch_bounds__(256) k_relax_sweep(LevelDev<T> L, const SweepT
     const bool ee = i == mx - 1, nn = j == my - 1;
     if ((i == 0 || ee) && !P.bc[ee ? 1 : 0] && rax[k].push >= 0) Qout[rax[k].push] = y;
     if ((j == 0 || nn) && !P.bc[nn ? 3 : 2] && ray[k].push >= 0) Qout[ray[k].push] = y;
+  }
+}
+
+// ---------------------------------------------------------------- streaming sweep
+// One red+black sweep of a level that is one patch with physical boundaries on
+// every side (a uniform level), Qin -> Qout.  Each wave owns a strip of
+// columns, two per lane, and marches up a block of rows: at row step j it
+// updates the red cell of row j and the black cell of row j-1 in the lane's
+// red column of row j.  Red updates read only old black values (the lane's own
+// rows and, across columns, the neighbour lanes' through cross-lane moves);
+// black updates read the red values of this sweep (own registers and the
+// neighbour lanes'), so one pass over the rows does the whole sweep and each
+// value is read from HBM about once.  The strip's first and last lanes are a
+// halo (their black values would need lanes outside the wave and are not
+// stored), and the red rows just outside the block are recomputed.  Every
+// update is the colour pass's expression (relax_combine) with the partial
+// physical ghosts of ebKernels.m:51-70, so the sweep equals two colour passes
+// bit for bit.
+// one row of a lane's two columns A, A+1 (A even, the level's width even):
+// coefficients, the faces (west faces of both, east face of A+1, south faces
+// of both) and the old values; loaded at clamped, in-bounds indices
+template <typename T>
+struct StreamRow {
+  T f0, f1, id0, id1, dw0, dw1, de1, ds0, ds1, q0, q1;
+};
+
+template <typename T>
+__device__ __forceinline__ void stream_row(const LevelDev<T>& L, const PatchInfo& P, const T* __restrict__ Q, int A,
+                                           int r, StreamRow<T>& w) {
+  const int mx = P.mx, my = P.my;
+  A = min(max(A, 0), mx - 2);
+  const int rf = min(max(r, -1), my);  // south faces exist for rows 0..my
+  r = min(max(r, 0), my - 1);
+  const int cc = P.coff + A + r * mx, fx = P.xoff + A + r * (mx + 1), fy = P.yoff + A + max(rf, 0) * mx;
+  const int qc = (A + 1) + (r + 1) * (mx + 2);
+  w.f0 = L.F[cc]; w.f1 = L.F[cc + 1];
+  w.id0 = L.ID[cc]; w.id1 = L.ID[cc + 1];
+  w.dw0 = L.DeW[fx]; w.dw1 = L.DeW[fx + 1]; w.de1 = L.DeW[fx + 2];
+  w.ds0 = L.DnS[fy]; w.ds1 = L.DnS[fy + 1];
+  w.q0 = Q[qc]; w.q1 = Q[qc + 1];
+}
+
+// boundary value of side s at position t (0 when homogeneous)
+template <typename T>
+__device__ __forceinline__ T stream_g(const LevelDev<T>& L, const PatchInfo& P, int side, int t, int homog) {
+  if (homog) return T(0);
+  const int mx = P.mx, my = P.my;
+  const int off = side == 0 ? 0 : side == 1 ? my : side == 2 ? 2 * my : 2 * my + mx;
+  return L.G[P.goff + off + t];
+}
+
+// update of cell (i, j) from its row data (column k of the pair), the south
+// face of the row above (dn) and its resolved-or-raw neighbours
+template <typename T>
+__device__ __forceinline__ T stream_cell(const LevelDev<T>& L, const PatchInfo& P, int i, int j, const StreamRow<T>& w,
+                                         int k, T dn, T e, T wv, T n, T s, T self, T omega, int homog) {
+  const int mx = P.mx, my = P.my;
+  T g[4] = {T(0), T(0), T(0), T(0)};
+  if (i == 0) g[0] = stream_g<T>(L, P, 0, j, homog);
+  if (i == mx - 1) g[1] = stream_g<T>(L, P, 1, j, homog);
+  if (j == 0) g[2] = stream_g<T>(L, P, 2, i, homog);
+  if (j == my - 1) g[3] = stream_g<T>(L, P, 3, i, homog);
+  const T x[4] = {T(0), T(0), T(0), T(0)};
+  const FaceAux<T> a[4] = {};
+  T qe, qw, qn, qs;
+  resolve_ghosts<T>(L, P, i, j, e, wv, n, s, self, g, x, a, false, qe, qw, qn, qs);
+  const T f = k ? w.f1 : w.f0, id = k ? w.id1 : w.id0;
+  const T dw = k ? w.dw1 : w.dw0, de = k ? w.de1 : w.dw1, ds = k ? w.ds1 : w.ds0;
+  return relax_combine<T>(L, qe, qw, qn, qs, f, de, dw, dn, ds, id, self, omega);
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_up(T v) {  // value of lane l-1
+  return __shfl_up(v, 1, 64);
+}
+template <typename T>
+__device__ __forceinline__ T lane_down(T v) {  // value of lane l+1
+  return __shfl_down(v, 1, 64);
+}
+
+// One red+black sweep of a level that is one patch of even width with
+// physical boundaries on every side (a uniform level), Qin -> Qout.  Each
+// 64-lane wave owns a strip of columns, two per lane, and marches up a block
+// of rows.  At row step j it loads row j of its columns and updates the red
+// cell of row j-2 and the black cell of row j-3 (in the red column of row
+// j-2), so each row's loads are in flight for a whole step before use.  Red
+// updates read only old black values (the lane's own rows and, across
+// columns, the neighbour lanes' through cross-lane moves); black updates read
+// the red values of this sweep (own registers and the neighbour lanes'), so
+// one pass over the rows does the whole sweep and every loaded line is used
+// in the step that loads it.  The strip's first and last lanes are a halo
+// (their black values would need lanes outside the wave and are not stored)
+// and the red rows just outside the block are recomputed.  Every update is
+// the colour pass's expression (relax_combine) with the partial physical
+// ghosts of ebKernels.m:51-70, so the sweep equals two colour passes bit for
+// bit.
+template <typename T>
+__global__ void __launch_bounds__(64) k_sweep_stream(LevelDev<T> L, PatchInfo P, const T* __restrict__ Qin,
+                                                     T* __restrict__ Qout, T omega, int homog, int kRows) {
+  const int mx = P.mx, my = P.my, ldq = mx + 2;
+  const int nstrip = (mx + kStreamCols - 1) / kStreamCols;
+  const int strip = blockIdx.x % nstrip, blk = blockIdx.x / nstrip;
+  const int j0 = blk * kRows, j1 = min(j0 + kRows, my);
+  const int own0 = strip * kStreamCols, own1 = min(own0 + kStreamCols, mx);
+  const int A = own0 - 2 + 2 * (int)threadIdx.x;  // the lane's columns A, A+1
+  const T* __restrict__ Q = Qin + P.qoff;
+  T* __restrict__ Qo = Qout + P.qoff;
+  auto q = [&](int i, int j) { return (i + 1) + (j + 1) * ldq; };
+  // rows jr-1, jr, jr+1 (loaded) around the red row jr = j-2, and row j (loading)
+  StreamRow<T> r0, r1, r2, r3;
+  stream_row<T>(L, P, Q, A, j0 - 2, r0);
+  stream_row<T>(L, P, Q, A, j0 - 1, r1);
+  stream_row<T>(L, P, Q, A, j0, r2);
+  T rd1 = T(0), rd2 = T(0);  // red values of this sweep, one and two rows below the red row
+  for (int j = j0 + 1; j <= j1 + 2; ++j) {
+    stream_row<T>(L, P, Q, A, j, r3);
+    const int jr = j - 2;
+    const int k = ((A + jr + P.par0) & 1) == 0 ? 0 : 1;  // red column of row jr (uniform)
+    const int c = A + k;
+    const bool col = c >= 0 && c < mx;
+    const T qo = k ? r1.q0 : r1.q1;  // the other column's old (black) value in row jr
+    const T from_up = lane_up(r1.q1), from_down = lane_down(r1.q0);
+    T red = T(0);
+    if (col && jr >= max(j0 - 1, 0) && jr <= min(j1, my - 1)) {
+      const T e = k ? from_down : qo, w = k ? qo : from_up;
+      const T n = k ? r2.q1 : r2.q0, s = k ? r0.q1 : r0.q0, self = k ? r1.q1 : r1.q0;
+      red = stream_cell<T>(L, P, c, jr, r1, k, k ? r2.ds1 : r2.ds0, e, w, n, s, self, omega, homog);
+      if (jr >= j0 && jr < j1 && c >= own0 && c < own1) Qo[q(c, jr)] = red;
+    }
+    // black cell (c, jr-1): red neighbours of row jr-1 along the row (the
+    // lane's other column and the neighbour lane's), red of rows jr and jr-2
+    const T rup = lane_up(rd1), rdown = lane_down(rd1);
+    const int jb = jr - 1;
+    if (col && jb >= j0 && jb < j1) {
+      const T e = k ? rdown : rd1, w = k ? rd1 : rup;
+      const T self = k ? r0.q1 : r0.q0;
+      const T v = stream_cell<T>(L, P, c, jb, r0, k, k ? r1.ds1 : r1.ds0, e, w, red, rd2, self, omega, homog);
+      if (c >= own0 && c < own1) Qo[q(c, jb)] = v;
+    }
+    rd2 = rd1;
+    rd1 = red;
+    r0 = r1;
+    r1 = r2;
+    r2 = r3;
   }
 }
 
@@ -1862,6 +2008,25 @@ void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rota
   hipLaunchKernelGGL(k_history, dim3(L.ntile), dim3(kTile), 0, s, L, Uo, Uoo, rotate);
 }
 
+template <typename T>
+int stream_rows(const PatchInfo& P, int ncu) {
+  const int nstrip = (P.mx + kStreamCols - 1) / kStreamCols;
+  const long long want = (long long)P.my * nstrip / (8LL * std::max(ncu, 1));  // about 8 waves per CU
+  int rows = kStreamRowsMin;
+  while (rows * 2 <= want && rows * 2 <= StreamRows<T>::value) rows *= 2;
+  return rows;
+}
+
+template <typename T>
+void launch_sweep_stream(const LevelDev<T>& L, const PatchInfo& P, const T* Qin, T* Qout, T omega, int homog,
+                         int ncu, hipStream_t s) {
+  const int rows = stream_rows<T>(P, ncu);
+  const int nstrip = (P.mx + kStreamCols - 1) / kStreamCols;
+  const int nblk_rows = (P.my + rows - 1) / rows;
+  hipLaunchKernelGGL(k_sweep_stream<T>, dim3(nstrip * nblk_rows), dim3(64), 0, s, L, P, Qin, Qout, omega, homog,
+                     rows);
+}
+
 void launch_regrid_level(const LevelDev<double>& L, const double* Pq, int conslin, int hist, double* Uoo_k,
                          const double* Uoo_all, const int* fc, const int* oq, const int* oc, const double* Qold,
                          const double* Uoo_old, hipStream_t s) {
@@ -2072,6 +2237,8 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int);            \
   template void launch_filldown_small<T>(const LevelDev<T>*, int, int, hipStream_t);                       \
+  template void launch_sweep_stream<T>(const LevelDev<T>&, const PatchInfo&, const T*, T*, T, int, int,          \
+                                       hipStream_t);                                                         \
   template void launch_relax_sweep<T>(const LevelDev<T>&, const LevelDev<T>*, const T*, T*, int, int, T, int,   \
                                       hipStream_t);                                                          \
   template void launch_coarsest<T>(const CoarsestDev&, const LevelDev<T>&, int, hipStream_t);           \

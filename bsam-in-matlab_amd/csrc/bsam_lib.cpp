@@ -343,6 +343,7 @@ bool face_records_arith(const HLevel& L) {
 struct OwnerMap;
 bool build_sweep_tiles(const HLevel& L, std::vector<SweepTile>& out);
 constexpr int kSweepWGsPerCU = 3;  // resident fused-sweep workgroups per CU
+constexpr long long kStreamMinCells = 1LL << 20;
 
 struct OwnerMap {
   int n0 = 0, n1 = 0;
@@ -1008,6 +1009,9 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
   // fused sweep tiles (levels with the standard face layout, even widths);
   // the FP32 copy shares them, each precision has its own second Q
   D.nsweep = 0; D.sweep = nullptr; D.Qs = nullptr;
+  // streaming sweeps beat colour passes from about a million cells up
+  D.stream = L.np == 1 && L.pi[0].mx % 2 == 0 && L.pi[0].bc[0] && L.pi[0].bc[1] && L.pi[0].bc[2] &&
+             L.pi[0].bc[3] && (long long)L.pi[0].mx * L.pi[0].my >= kStreamMinCells;
   if (D.arith && D.even_mx && L.np > 0) {
     if (tables_from64) {
       D.nsweep = d64->nsweep; D.sweep = d64->sweep;
@@ -1022,10 +1026,10 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
         D.nsweep = (int)st.size();
       }
     }
-    if (D.nsweep) {
-      std::vector<double> zq(L.nq, 0.0);
-      if ((rc = upload_cast(h, &D.Qs, zq))) return rc;
-    }
+  }
+  if (D.nsweep || D.stream) {  // the sweeps' second copy of Q
+    std::vector<double> zq(L.nq, 0.0);
+    if ((rc = upload_cast(h, &D.Qs, zq))) return rc;
   }
   std::vector<FaceAux<T>> aux(L.ghost.size());
   for (size_t t = 0; t < L.ghost.size(); ++t) {
@@ -1100,7 +1104,8 @@ struct Seq {
     const bool ar = lev(k).arith && lev(k).aux;
     if (first_inline == 2 && !ar) first_inline = 1;
     int it0 = 0;
-    if (h->opts.smoother == 2 && ar && lev(k).nsweep && nsweeps > 1) {
+    const bool stream = lev(k).stream && h->opts.smoother != 3;
+    if (nsweeps > 1 && (stream || (h->opts.smoother == 2 && ar && lev(k).nsweep))) {
       // one launch per red+black sweep for an even number of sweeps, Q and
       // Qs in turn, so that the last of them writes Q; an odd count does its
       // last sweep as two colour passes on Q.  The parent parts are current
@@ -1111,6 +1116,11 @@ struct Seq {
       T* buf[2] = {Lk.Q, Lk.Qs};
       it0 = nsweeps & ~1;
       for (int it = 0; it < it0; ++it) {
+        if (stream) {  // physical boundaries only: no ghost source, parent parts or pushes
+          launch_sweep_stream<T>(Lk, h->H[k - 1].pi[0], buf[it & 1], buf[(it + 1) & 1], (T)h->opts.omega, homog,
+                                 h->ncu, s);
+          continue;
+        }
         const int mode = it ? kInlArith : (first_inline == 1 ? kInlRecords : first_inline == 2 ? kInlArith : 0);
         const int keep = it == 0 && (first_inline == 1 || (first_inline == 0 && !cf_current));
         launch_relax_sweep<T>(Lk, par(k), buf[it & 1], buf[(it + 1) & 1], mode, keep, (T)h->opts.omega, homog, s);
@@ -1624,7 +1634,8 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
   }
   if (h->opts.npre < 0 || h->opts.npost < 0 || h->opts.max_vcycles < 0)
     return fail(BSAM_E_ARG, "bsamgpu:arg: negative sweep or cycle count");
-  if (h->opts.smoother < 0 || h->opts.smoother > 2) return fail(BSAM_E_ARG, "bsamgpu:arg: smoother must be 0, 1 or 2");
+  if (h->opts.smoother < 0 || h->opts.smoother > 3)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: smoother must be 0, 1, 2 or 3");
   const bool mixed = h->opts.precision == BSAM_MIXED;
   if (mixed)
     if (int rc = ensure32(h)) return rc;
@@ -1907,7 +1918,8 @@ double bsam_smoother_bytes(const bsam_hier* h, int k, int elem_bytes) {
     b += (mx + 2) * (my + 2) + 3 * mx * my + (mx + 1) * my + mx * (my + 1);
   }
   // a fused sweep launch does the whole red+black sweep, a colour pass half
-  const bool sweep = h->opts.smoother == 2 && (int)h->d64.size() >= k && h->d64[k - 1].nsweep > 0;
+  const bool sweep = (int)h->d64.size() >= k && ((h->d64[k - 1].stream && h->opts.smoother != 3) ||
+                                                  (h->opts.smoother == 2 && h->d64[k - 1].nsweep > 0));
   return sweep ? b * elem_bytes : b * elem_bytes / 2.0;
 }
 
@@ -1920,8 +1932,22 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, h->stream));
-  const bool sweep = h->opts.smoother == 2 && h->d64[k - 1].nsweep > 0;
+  const bool stream = h->d64[k - 1].stream && h->opts.smoother != 3;
+  const bool sweep = stream || (h->opts.smoother == 2 && h->d64[k - 1].nsweep > 0);
   for (int i = 0; i < nlaunch; ++i) {
+    if (stream) {  // streaming sweeps, Q and Qs in turn
+      const PatchInfo& P = h->H[k - 1].pi[0];
+      if (precision == BSAM_MIXED) {
+        LevelDev<float>& D = h->d32[k - 1];
+        launch_sweep_stream<float>(D, P, (i & 1) ? D.Qs : D.Q, (i & 1) ? D.Q : D.Qs, (float)h->opts.omega, 1,
+                                   h->ncu, h->stream);
+      } else {
+        LevelDev<double>& D = h->d64[k - 1];
+        launch_sweep_stream<double>(D, P, (i & 1) ? D.Qs : D.Q, (i & 1) ? D.Q : D.Qs, h->opts.omega, 0, h->ncu,
+                                    h->stream);
+      }
+      continue;
+    }
     if (sweep) {
       // the form of every sweep after the first: a fused red+black sweep,
       // Q and Qs in turn

@@ -103,7 +103,12 @@ typedef struct {
                           into at most 3 x CU-count tiles of <= 32x32 cells, each
                           pair of red+black sweeps runs as fused sweep launches
                           (red half in LDS, black half from it), alternating Q
-                          and a second copy of Q */
+                          and a second copy of Q;
+                          3: colour passes on every level.
+                          Under 0 and 2, a level that is one patch of even width
+                          with physical sides only and >= 2^20 cells runs pairs of
+                          sweeps as streaming sweep launches (one pass over the
+                          rows per red+black sweep), also bit-identical */
 } bsam_solve_opts;
 
 /* One solve stage (EllipticBSAM/src/ebSolveCycles.m:58-70) plus device timing. */

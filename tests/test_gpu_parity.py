@@ -251,3 +251,34 @@ def test_errors_are_reported():
     with pytest.raises(Exception) as ei:
         GpuHierarchy(H)
     assert "eb_make_level:D" in str(ei.value)
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.3])
+def test_streaming_sweeps_on_a_uniform_level(omega):
+    """A one-patch level with physical sides on every face and at least 2^20
+    cells runs whole red+black sweeps as streaming launches by default: the
+    solve equals the colour-pass solve ('passes') bit for bit (inhomogeneous
+    Dirichlet and Neumann sides, variable D and C, a 2048 x 1024 finest level:
+    16 full strips and a partial one, several row blocks), in FP64 and in the
+    FP32-inner form."""
+    from bsamgpu import Options, Problem, hierarchy
+
+    D = lambda x, y: 1 + 0.4 * np.sin(3 * x) * np.cos(2 * y)  # noqa: E731
+    C = lambda x, y: 0.5 + x * y  # noqa: E731
+    f = lambda x, y: np.sin(4 * x) + y * y  # noqa: E731
+    bc = {"west": ("dirichlet", lambda x, y: 1 + y), "east": ("neumann", lambda x, y: 0.5 * y),
+          "south": ("dirichlet", lambda x, y: x * x), "north": ("neumann", lambda x, y: 0 * x - 1)}
+    H = hierarchy.setup(Problem([0, 1, 0, 1], D, C, f, bc),
+                        Options(BaseCells=[2048, 1024], MaxLevels=1, Omega=omega, RelTol=1e-10, MaxVCycles=4,
+                                Verbose=0))
+    G = GpuHierarchy(H)
+    for prec in ("fp64", "mixed"):
+        out = {}
+        for sm in ("passes", "exact"):
+            G.reset()
+            st = G.solve_cycles(smoother=sm, precision=prec)
+            out[sm] = (st["resHist"], G.get_field(H.nlev, _lib.BSAM_Q))
+        print(f"omega {omega} {prec}: {len(out['exact'][0]) - 1} cycles", flush=True)
+        assert out["exact"][0] == out["passes"][0]
+        assert all(np.array_equal(a, b) for a, b in zip(out["exact"][1], out["passes"][1]))
+    G.close()
