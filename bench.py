@@ -300,7 +300,7 @@ def main():
     ap.add_argument("--no-varcoef", action="store_true", help="skip the refined variable-coefficient workload")
     ap.add_argument("--serve-requests", type=int, default=128)
     ap.add_argument("--serve-rate", type=float, default=100.0)
-    ap.add_argument("--only", choices=["parabolic", "serving", "varcoef"],
+    ap.add_argument("--only", choices=["parabolic", "serving", "varcoef", "uniform"],
                     help="run one secondary workload and print it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain device copy of MIB MiB 20 times (scale for the PMC byte counters)")
@@ -334,6 +334,9 @@ def main():
         return
     if args.only == "varcoef":
         print(json.dumps(bench_varcoef_refined()), flush=True)
+        return
+    if args.only == "uniform":
+        print(json.dumps(bench_large_smoother(args.large_smoother or 4096)), flush=True)
         return
 
     log("building starfish hierarchy on the host")

@@ -579,20 +579,6 @@ __global__ void __launch_bounds__(256) k_relax_sweep(LevelDev<T> L, const SweepT
 }
 
 // ---------------------------------------------------------------- streaming sweep
-// One red+black sweep of a level that is one patch with physical boundaries on
-// every side (a uniform level), Qin -> Qout.  Each wave owns a strip of
-// columns, two per lane, and marches up a block of rows: at row step j it
-// updates the red cell of row j and the black cell of row j-1 in the lane's
-// red column of row j.  Red updates read only old black values (the lane's own
-// rows and, across columns, the neighbour lanes' through cross-lane moves);
-// black updates read the red values of this sweep (own registers and the
-// neighbour lanes'), so one pass over the rows does the whole sweep and each
-// value is read from HBM about once.  The strip's first and last lanes are a
-// halo (their black values would need lanes outside the wave and are not
-// stored), and the red rows just outside the block are recomputed.  Every
-// update is the colour pass's expression (relax_combine) with the partial
-// physical ghosts of ebKernels.m:51-70, so the sweep equals two colour passes
-// bit for bit.
 // one row of a lane's two columns A, A+1 (A even, the level's width even):
 // coefficients, the faces (west faces of both, east face of A+1, south faces
 // of both) and the old values; loaded at clamped, in-bounds indices
@@ -665,12 +651,12 @@ __device__ __forceinline__ T lane_down(T v) {  // value of lane l+1
 // columns, the neighbour lanes' through cross-lane moves); black updates read
 // the red values of this sweep (own registers and the neighbour lanes'), so
 // one pass over the rows does the whole sweep and every loaded line is used
-// in the step that loads it.  The strip's first and last lanes are a halo
-// (their black values would need lanes outside the wave and are not stored)
-// and the red rows just outside the block are recomputed.  Every update is
-// the colour pass's expression (relax_combine) with the partial physical
-// ghosts of ebKernels.m:51-70, so the sweep equals two colour passes bit for
-// bit.
+// in the step that loads it; a row's two columns are stored together.  The
+// strip's first and last lanes are a halo (their black values would need
+// lanes outside the wave and are not stored) and the red rows just outside
+// the block are recomputed.  Every update is the colour pass's expression
+// (relax_combine) with the partial physical ghosts of ebKernels.m:51-70, so
+// the sweep equals two colour passes bit for bit.
 template <typename T>
 __global__ void __launch_bounds__(64) k_sweep_stream(LevelDev<T> L, PatchInfo P, const T* __restrict__ Qin,
                                                      T* __restrict__ Qout, T omega, int homog, int kRows) {
@@ -680,9 +666,9 @@ __global__ void __launch_bounds__(64) k_sweep_stream(LevelDev<T> L, PatchInfo P,
   const int j0 = blk * kRows, j1 = min(j0 + kRows, my);
   const int own0 = strip * kStreamCols, own1 = min(own0 + kStreamCols, mx);
   const int A = own0 - 2 + 2 * (int)threadIdx.x;  // the lane's columns A, A+1
+  const bool owned = A >= own0 && A < own1;       // both columns or neither (own0, own1, A even)
   const T* __restrict__ Q = Qin + P.qoff;
   T* __restrict__ Qo = Qout + P.qoff;
-  auto q = [&](int i, int j) { return (i + 1) + (j + 1) * ldq; };
   // rows jr-1, jr, jr+1 (loaded) around the red row jr = j-2, and row j (loading)
   StreamRow<T> r0, r1, r2, r3;
   stream_row<T>(L, P, Q, A, j0 - 2, r0);
@@ -702,17 +688,22 @@ __global__ void __launch_bounds__(64) k_sweep_stream(LevelDev<T> L, PatchInfo P,
       const T e = k ? from_down : qo, w = k ? qo : from_up;
       const T n = k ? r2.q1 : r2.q0, s = k ? r0.q1 : r0.q0, self = k ? r1.q1 : r1.q0;
       red = stream_cell<T>(L, P, c, jr, r1, k, k ? r2.ds1 : r2.ds0, e, w, n, s, self, omega, homog);
-      if (jr >= j0 && jr < j1 && c >= own0 && c < own1) Qo[q(c, jr)] = red;
     }
     // black cell (c, jr-1): red neighbours of row jr-1 along the row (the
-    // lane's other column and the neighbour lane's), red of rows jr and jr-2
+    // lane's other column, rd1, and the neighbour lane's), red of rows jr and jr-2
     const T rup = lane_up(rd1), rdown = lane_down(rd1);
     const int jb = jr - 1;
     if (col && jb >= j0 && jb < j1) {
       const T e = k ? rdown : rd1, w = k ? rd1 : rup;
       const T self = k ? r0.q1 : r0.q0;
       const T v = stream_cell<T>(L, P, c, jb, r0, k, k ? r1.ds1 : r1.ds0, e, w, red, rd2, self, omega, homog);
-      if (c >= own0 && c < own1) Qo[q(c, jb)] = v;
+      // row jb is complete in both columns (its red cell, in the other column,
+      // was updated one step earlier): both are stored together
+      if (owned) {
+        const int qa = (A + 1) + (jb + 1) * ldq;
+        Qo[qa + k] = v;
+        Qo[qa + 1 - k] = rd1;
+      }
     }
     rd2 = rd1;
     rd1 = red;
