@@ -1417,24 +1417,6 @@ __device__ __forceinline__ long long wave_sum_ll(long long v) {
 // block-level reduction of (max, sum, max, count) into out (thread 0 writes)
 __device__ __forceinline__ void block_reduce4(double a, double b, double c, long long n, ResidualOut* out);
 
-// the parts of a composite residual reduced in a fixed order by one
-// workgroup of kTile threads; out_host (pinned host memory) also receives
-// the result when given
-__device__ __forceinline__ void reduce_parts(const ResidualOut* parts, int nparts, ResidualOut* out,
-                                             ResidualOut* out_host) {
-  double a = 0.0, b = 0.0, f = 0.0;
-  long long n = 0;
-  for (int k = threadIdx.x; k < nparts; k += kTile) {
-    const ResidualOut p = parts[k];
-    a = fmax(a, p.linf);
-    b = b + p.l2sq;
-    f = fmax(f, p.fnorm);
-    n += p.ncell;
-  }
-  block_reduce4(a, b, f, n, out);
-  if (out_host && threadIdx.x == 0) *out_host = *out;
-}
-
 // block-level reduction of (max, sum, max, count) into out (thread 0 writes)
 __device__ __forceinline__ void block_reduce4(double a, double b, double c, long long n,
                                               ResidualOut* out) {
@@ -1545,10 +1527,57 @@ __global__ void __launch_bounds__(kTile) k_residual_all(const LevelDev<T>* __res
 // kernel boundary orders the parts' writes more cheaply than device-scope
 // fences in every workgroup would); out_host (pinned host memory) also
 // receives the result when given, so no copy follows.
-__global__ void __launch_bounds__(kTile) k_reduce_parts(const ResidualOut* __restrict__ parts,
-                                                        int nparts, ResidualOut* __restrict__ out,
-                                                        ResidualOut* __restrict__ out_host) {
-  reduce_parts(parts, nparts, out, out_host);
+// 1024 lanes, each summing its parts (every 1024th, in order) eight loads at a
+// time, then waves and the workgroup in a fixed order: the result is the
+// same on every run.
+constexpr int kReduceThreads = 1024;
+__global__ void __launch_bounds__(kReduceThreads) k_reduce_parts(const ResidualOut* __restrict__ parts,
+                                                                 int nparts, ResidualOut* __restrict__ out,
+                                                                 ResidualOut* __restrict__ out_host) {
+  double a = 0.0, b = 0.0, f = 0.0;
+  long long n = 0;
+  for (int k0 = threadIdx.x; k0 < nparts; k0 += 8 * kReduceThreads) {
+    ResidualOut p[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + u * kReduceThreads;
+      if (k < nparts) {
+        p[u] = parts[k];
+      } else {
+        p[u].linf = p[u].l2sq = p[u].fnorm = 0.0;
+        p[u].ncell = 0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      a = fmax(a, p[u].linf);
+      b = b + p[u].l2sq;
+      f = fmax(f, p[u].fnorm);
+      n += p[u].ncell;
+    }
+  }
+  __shared__ double sa[kReduceThreads / 64], sb[kReduceThreads / 64], sc[kReduceThreads / 64];
+  __shared__ long long sn[kReduceThreads / 64];
+  a = wave_max(a);
+  b = wave_sum(b);
+  f = wave_max(f);
+  n = wave_sum_ll(n);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sa[w] = a; sb[w] = b; sc[w] = f; sn[w] = n;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ResidualOut r;
+    r.linf = sa[0]; r.l2sq = sb[0]; r.fnorm = sc[0]; r.ncell = sn[0];
+    for (int k = 1; k < kReduceThreads / 64; ++k) {
+      r.linf = fmax(r.linf, sa[k]); r.l2sq = r.l2sq + sb[k]; r.fnorm = fmax(r.fnorm, sc[k]); r.ncell += sn[k];
+    }
+    r.pad = 0.0;
+    r.pad2[0] = r.pad2[1] = r.pad2[2] = 0;
+    *out = r;
+    if (out_host) *out_host = r;
+  }
 }
 
 // composite mean: sum of visible u * da (ebCompositeMean.m:4-23)
@@ -1962,7 +1991,7 @@ void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, in
 }
 
 void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s, ResidualOut* out_host) {
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kTile), 0, s, parts, nparts, out, out_host);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kReduceThreads), 0, s, parts, nparts, out, out_host);
 }
 
 template <typename T>

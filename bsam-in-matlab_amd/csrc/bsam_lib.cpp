@@ -1105,7 +1105,23 @@ struct Seq {
     if (first_inline == 2 && !ar) first_inline = 1;
     int it0 = 0;
     const bool stream = lev(k).stream && h->opts.smoother != 3;
-    if (nsweeps > 1 && (stream || (h->opts.smoother == 2 && ar && lev(k).nsweep))) {
+    if (stream && nsweeps > 0) {
+      // streaming sweeps, Q and Qs in turn, ending in Q: an odd count starts
+      // from a copy of Q in Qs (a copy costs a third of a sweep, two colour
+      // passes cost nearly two).  Physical boundaries only: no ghost source,
+      // parent parts or pushes
+      const LevelDev<T>& Lk = lev(k);
+      const int odd = nsweeps & 1;
+      // (a failure is reported by the sequence's hipGetLastError check)
+      if (odd) (void)hipMemcpyAsync(Lk.Qs, Lk.Q, (size_t)Lk.nq * sizeof(T), hipMemcpyDeviceToDevice, s);
+      T* buf[2] = {odd ? Lk.Qs : Lk.Q, odd ? Lk.Q : Lk.Qs};
+      for (int it = 0; it < nsweeps; ++it)
+        launch_sweep_stream<T>(Lk, h->H[k - 1].pi[0], buf[it & 1], buf[(it + 1) & 1], (T)h->opts.omega, homog,
+                               h->ncu, s);
+      fill(k);
+      return;
+    }
+    if (nsweeps > 1 && h->opts.smoother == 2 && ar && lev(k).nsweep) {
       // one launch per red+black sweep for an even number of sweeps, Q and
       // Qs in turn, so that the last of them writes Q; an odd count does its
       // last sweep as two colour passes on Q.  The parent parts are current
@@ -1116,11 +1132,6 @@ struct Seq {
       T* buf[2] = {Lk.Q, Lk.Qs};
       it0 = nsweeps & ~1;
       for (int it = 0; it < it0; ++it) {
-        if (stream) {  // physical boundaries only: no ghost source, parent parts or pushes
-          launch_sweep_stream<T>(Lk, h->H[k - 1].pi[0], buf[it & 1], buf[(it + 1) & 1], (T)h->opts.omega, homog,
-                                 h->ncu, s);
-          continue;
-        }
         const int mode = it ? kInlArith : (first_inline == 1 ? kInlRecords : first_inline == 2 ? kInlArith : 0);
         const int keep = it == 0 && (first_inline == 1 || (first_inline == 0 && !cf_current));
         launch_relax_sweep<T>(Lk, par(k), buf[it & 1], buf[(it + 1) & 1], mode, keep, (T)h->opts.omega, homog, s);
