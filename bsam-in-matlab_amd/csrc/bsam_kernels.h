@@ -105,6 +105,8 @@ void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipSt
 template <typename T> void launch_mass(const LevelDev<T>& L, ResidualOut* parts, hipStream_t s);
 void launch_add_correction(const LevelDev<double>& L, const LevelDev<float>& E, hipStream_t s);
 template <typename T> void launch_fill_value(T* a, long long n, T v, hipStream_t s);
+// F = FTrue on every level; max_cells: the largest level's cell count
+template <typename T> void launch_reset_force(const LevelDev<T>* LV_dev, int nlev, int max_cells, hipStream_t s);
 template <typename TD, typename TS> void launch_convert(TD* d, const TS* src, long long n, hipStream_t s);
 void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double* Uoo, double dt, int first,
                      long long n, hipStream_t s);

@@ -1580,6 +1580,16 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_parts(const ResidualO
   }
 }
 
+// F = FTrue on every level (ebSolveCycles.m:30-34); blockIdx.y is the level
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_reset_force(const LevelDev<T>* __restrict__ LV) {
+  const LevelDev<T>& L = LV[blockIdx.y];
+  const int n = L.ncell;
+  T* __restrict__ F = L.F;
+  const T* __restrict__ FT = L.FT;
+  for (int x = blockIdx.x * kTile + threadIdx.x; x < n; x += gridDim.x * kTile) F[x] = FT[x];
+}
+
 // composite mean: sum of visible u * da (ebCompositeMean.m:4-23)
 template <typename T>
 __global__ void __launch_bounds__(kTile) k_mass(LevelDev<T> L, ResidualOut* __restrict__ part) {
@@ -1990,6 +2000,13 @@ void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, in
                      rout, reset_f, k0);
 }
 
+template <typename T>
+void launch_reset_force(const LevelDev<T>* LV_dev, int nlev, int max_cells, hipStream_t s) {
+  if (nlev <= 0 || max_cells <= 0) return;
+  const unsigned gx = std::min(nblk(max_cells), 4096u);
+  hipLaunchKernelGGL(k_reset_force<T>, dim3(gx, nlev), dim3(kTile), 0, s, LV_dev);
+}
+
 void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s, ResidualOut* out_host) {
   hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kReduceThreads), 0, s, parts, nparts, out, out_host);
 }
@@ -2257,6 +2274,7 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int);            \
   template void launch_filldown_small<T>(const LevelDev<T>*, int, int, hipStream_t);                       \
+  template void launch_reset_force<T>(const LevelDev<T>*, int, int, hipStream_t);                         \
   template void launch_sweep_stream<T>(const LevelDev<T>&, const PatchInfo&, const T*, T*, T, int, int,          \
                                        hipStream_t);                                                         \
   template void launch_relax_sweep<T>(const LevelDev<T>&, const LevelDev<T>*, const T*, T*, int, int, T, int,   \

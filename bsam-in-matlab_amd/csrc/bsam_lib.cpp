@@ -1212,8 +1212,9 @@ struct Seq {
 // F = FTrue on every level in one launch (ebSolveCycles.m:30-34)
 template <typename T>
 void enqueue_reset_force(bsam_hier* h, const LevelDev<T>* LVd, hipStream_t s) {
-  launch_residual_all<T, float>(LVd, h->nlev, h->res_tiles, h->nres_tiles, 0, h->parts, nullptr, 1,
-                                h->nlev + 1, s);
+  int mc = 0;
+  for (const HLevel& L : h->H) mc = std::max(mc, L.ncell);
+  launch_reset_force<T>(LVd, h->nlev, mc, s);
 }
 
 // composite residual (ebCompResidual.m:23-84) on the FP64 hierarchy: one
