@@ -37,15 +37,19 @@ __attribute__((target_clones("avx2", "default"))) bool bsam_lu_inverse(double* _
       for (int c = k + 1; c < M; ++c) ri[c] = ri[c] - l * rk[c];
     }
   }
+  // X = L^-1 P with its columns reordered (column c of X is column perm[c] of
+  // L^-1 P), so that it starts as the identity and row k stays exactly +0
+  // beyond column k during forward substitution: those terms (x - l*0 = x,
+  // +0 - l*0 = +0) are skipped without changing any entry.
   std::vector<double> Xv((size_t)M * M, 0.0);  // row-major
   double* __restrict__ X = Xv.data();
-  for (int i = 0; i < M; ++i) X[(size_t)i * M + perm[i]] = 1.0;
+  for (int i = 0; i < M; ++i) X[(size_t)i * M + i] = 1.0;
   for (int k = 0; k < M; ++k) {
     const double* __restrict__ xk = X + (size_t)k * M;
     for (int i = k + 1; i < M; ++i) {
       const double l = at(i, k);
       double* __restrict__ xi = X + (size_t)i * M;
-      for (int c = 0; c < M; ++c) xi[c] = xi[c] - l * xk[c];
+      for (int c = 0; c <= k; ++c) xi[c] = xi[c] - l * xk[c];
     }
   }
   for (int k = M - 1; k >= 0; --k) {
@@ -58,7 +62,7 @@ __attribute__((target_clones("avx2", "default"))) bool bsam_lu_inverse(double* _
       for (int c = 0; c < M; ++c) xi[c] = xi[c] - u * xk[c];
     }
   }
-  for (int i = 0; i < M; ++i)  // column-major for the device
-    for (int c = 0; c < M; ++c) Ainv[i + (size_t)c * M] = X[(size_t)i * M + c];
+  for (int i = 0; i < M; ++i)  // column-major for the device, columns back in place
+    for (int c = 0; c < M; ++c) Ainv[i + (size_t)perm[c] * M] = X[(size_t)i * M + c];
   return true;
 }
