@@ -98,7 +98,7 @@ __host__ __device__ inline long long stage_level_bytes(const LevelDev<T>& L, int
 template <typename T> void launch_qc_refresh_all(const LevelDev<T>* LV_dev, const int2* items, int nitems, hipStream_t s);
 template <typename T, typename TO>
 void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, int ntl, int mass_corr,
-                         ResidualOut* parts, TO* const* rout, int reset_f, int k0, hipStream_t s);
+                         ResidualOut* parts, TO* const* rout, int k0, hipStream_t s);
 // out_host (pinned host memory), when given, receives the result as well
 void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s,
                    ResidualOut* out_host = nullptr);

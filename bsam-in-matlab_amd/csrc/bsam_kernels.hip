@@ -1458,16 +1458,12 @@ __global__ void __launch_bounds__(kTile) k_qc_refresh_all(const LevelDev<T>* __r
   }
 }
 
-// composite residual of every level in [k0, nlev] (ebCompResidual.m:302-332):
-// r = FTrue - L(Q) + reflux chunks, norms over visible cells; optionally
-// writes r (the defect-correction right side) and resets F = FTrue on the
-// tile's cells.  The tile table is (level, tile) pairs.
 // one cell of the composite residual: r = FTrue - L(Q) + reflux chunks
 // (ebCompResidual.m:302-332).  Accumulates max|r|, sum r^2 da, max|FTrue| and
-// the visible count; optionally writes r to rout[level] and resets F = FTrue.
+// the visible count; optionally writes r to rout[level].
 template <typename T, typename TO>
 __device__ __forceinline__ void residual_item(const LevelDev<T>* __restrict__ LV, int nlev, int lvl, int tile,
-                                              int lane, int mass_corr, TO* const* __restrict__ rout, int reset_f,
+                                              int lane, int mass_corr, TO* const* __restrict__ rout,
                                               int k0, double& a, double& b, double& f, long long& n) {
   const LevelDev<T>& L = LV[lvl];
   const TileRec& tr = L.tiles[tile];
@@ -1476,7 +1472,6 @@ __device__ __forceinline__ void residual_item(const LevelDev<T>* __restrict__ LV
   const int c = tr.start + lane;
   if (c >= mx * my) return;
   const int cc = pi.coff + c;
-  if (reset_f) L.F[cc] = L.FT[cc];
   if (lvl + 1 < k0) return;
   const int j = cell_row(c, mx, pi.magic);
   const int i = c - j * mx;
@@ -1507,19 +1502,18 @@ __device__ __forceinline__ void residual_item(const LevelDev<T>* __restrict__ LV
   }
 }
 
-// composite residual of every level in [k0, nlev]; optionally writes r (the
-// defect-correction right side) and resets F = FTrue on the tile's cells.
-// The tile table is (level, tile) pairs.
+// composite residual of every level in [k0, nlev] (ebCompResidual.m:302-332);
+// optionally writes r (the defect-correction right side).  The tile table is
+// (level, tile) pairs.
 template <typename T, typename TO>
 __global__ void __launch_bounds__(kTile) k_residual_all(const LevelDev<T>* __restrict__ LV, int nlev,
                                                         const int2* __restrict__ tl, int mass_corr,
                                                         ResidualOut* __restrict__ part,
-                                                        TO* const* __restrict__ rout, int reset_f,
-                                                        int k0) {
+                                                        TO* const* __restrict__ rout, int k0) {
   const int2 lt = tl[blockIdx.x];
   double a = 0.0, b = 0.0, f = 0.0;
   long long n = 0;
-  residual_item<T, TO>(LV, nlev, lt.x, lt.y, (int)threadIdx.x, mass_corr, rout, reset_f, k0, a, b, f, n);
+  residual_item<T, TO>(LV, nlev, lt.x, lt.y, (int)threadIdx.x, mass_corr, rout, k0, a, b, f, n);
   block_reduce4(a, b, f, n, &part[blockIdx.x]);
 }
 
@@ -1680,7 +1674,7 @@ __device__ void blk_residual(const LevelDev<double>* __restrict__ LV, int nlev, 
   for (int k = root; k <= nlev; ++k) {
     const int ncell = LV[k - 1].ntile * kTile;
     for (int x = threadIdx.x; x < ncell; x += blockDim.x)
-      residual_item<double, float>(LV, nlev, k - 1, x / kTile, x % kTile, mass_corr, rout, 0, root, a, b, f, n);
+      residual_item<double, float>(LV, nlev, k - 1, x / kTile, x % kTile, mass_corr, rout, root, a, b, f, n);
   }
   block_reduce4_wg(a, b, f, n, out);
 }
@@ -1994,10 +1988,10 @@ void launch_qc_refresh_all(const LevelDev<T>* LV_dev, const int2* items, int nit
 
 template <typename T, typename TO>
 void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, int ntl, int mass_corr,
-                         ResidualOut* parts, TO* const* rout, int reset_f, int k0, hipStream_t s) {
+                         ResidualOut* parts, TO* const* rout, int k0, hipStream_t s) {
   if (!ntl) return;
   hipLaunchKernelGGL((k_residual_all<T, TO>), dim3(ntl), dim3(kTile), 0, s, LV_dev, nlev, tl, mass_corr, parts,
-                     rout, reset_f, k0);
+                     rout, k0);
 }
 
 template <typename T>
@@ -2289,9 +2283,9 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
 BSAM_INST(double)
 BSAM_INST(float)
 template void launch_residual_all<double, float>(const LevelDev<double>*, int, const int2*, int, int, ResidualOut*,
-                                                 float* const*, int, int, hipStream_t);
+                                                 float* const*, int, hipStream_t);
 template void launch_residual_all<float, float>(const LevelDev<float>*, int, const int2*, int, int, ResidualOut*,
-                                                float* const*, int, int, hipStream_t);
+                                                float* const*, int, hipStream_t);
 template void launch_convert<float, double>(float*, const double*, long long, hipStream_t);
 template void launch_convert<double, float>(double*, const float*, long long, hipStream_t);
 

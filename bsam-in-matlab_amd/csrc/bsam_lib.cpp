@@ -1225,7 +1225,7 @@ void enqueue_reset_force(bsam_hier* h, const LevelDev<T>* LVd, hipStream_t s) {
 void enqueue_residual(bsam_hier* h, float* const* rout, hipStream_t s) {
   if (h->opts.mass_correction) launch_qc_refresh_all<double>(h->lv64, h->qc_items, h->nqc_items, s);
   launch_residual_all<double, float>(h->lv64, h->nlev, h->res_tiles, h->nres_tiles, h->opts.mass_correction,
-                                     h->parts, rout, 0, h->root, s);
+                                     h->parts, rout, h->root, s);
   launch_reduce(h->parts, h->nres_tiles, h->dres, s, h->hres);
 }
 
