@@ -96,6 +96,8 @@ __host__ __device__ inline long long stage_level_bytes(const LevelDev<T>& L, int
   return b;
 }
 template <typename T> void launch_qc_refresh_all(const LevelDev<T>* LV_dev, const int2* items, int nitems, hipStream_t s);
+// the composite residual writes min(ntl, kResidualParts) parts
+constexpr int kResidualParts = 2048;
 template <typename T, typename TO>
 void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, int ntl, int mass_corr,
                          ResidualOut* parts, TO* const* rout, int k0, hipStream_t s);

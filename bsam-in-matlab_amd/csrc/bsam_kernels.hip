@@ -1504,16 +1504,19 @@ __device__ __forceinline__ void residual_item(const LevelDev<T>* __restrict__ LV
 
 // composite residual of every level in [k0, nlev] (ebCompResidual.m:302-332);
 // optionally writes r (the defect-correction right side).  The tile table is
-// (level, tile) pairs.
+// (level, tile) pairs; a workgroup takes every gridDim.x-th tile and writes one
+// part, so the final reduction reads at most kResidualParts parts.
 template <typename T, typename TO>
 __global__ void __launch_bounds__(kTile) k_residual_all(const LevelDev<T>* __restrict__ LV, int nlev,
-                                                        const int2* __restrict__ tl, int mass_corr,
+                                                        const int2* __restrict__ tl, int ntl, int mass_corr,
                                                         ResidualOut* __restrict__ part,
                                                         TO* const* __restrict__ rout, int k0) {
-  const int2 lt = tl[blockIdx.x];
   double a = 0.0, b = 0.0, f = 0.0;
   long long n = 0;
-  residual_item<T, TO>(LV, nlev, lt.x, lt.y, (int)threadIdx.x, mass_corr, rout, k0, a, b, f, n);
+  for (int t = blockIdx.x; t < ntl; t += gridDim.x) {
+    const int2 lt = tl[t];
+    residual_item<T, TO>(LV, nlev, lt.x, lt.y, (int)threadIdx.x, mass_corr, rout, k0, a, b, f, n);
+  }
   block_reduce4(a, b, f, n, &part[blockIdx.x]);
 }
 
@@ -1990,8 +1993,8 @@ template <typename T, typename TO>
 void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, int ntl, int mass_corr,
                          ResidualOut* parts, TO* const* rout, int k0, hipStream_t s) {
   if (!ntl) return;
-  hipLaunchKernelGGL((k_residual_all<T, TO>), dim3(ntl), dim3(kTile), 0, s, LV_dev, nlev, tl, mass_corr, parts,
-                     rout, k0);
+  hipLaunchKernelGGL((k_residual_all<T, TO>), dim3(std::min(ntl, kResidualParts)), dim3(kTile), 0, s, LV_dev, nlev,
+                     tl, ntl, mass_corr, parts, rout, k0);
 }
 
 template <typename T>

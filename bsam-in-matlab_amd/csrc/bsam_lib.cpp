@@ -1226,7 +1226,7 @@ void enqueue_residual(bsam_hier* h, float* const* rout, hipStream_t s) {
   if (h->opts.mass_correction) launch_qc_refresh_all<double>(h->lv64, h->qc_items, h->nqc_items, s);
   launch_residual_all<double, float>(h->lv64, h->nlev, h->res_tiles, h->nres_tiles, h->opts.mass_correction,
                                      h->parts, rout, h->root, s);
-  launch_reduce(h->parts, h->nres_tiles, h->dres, s, h->hres);
+  launch_reduce(h->parts, std::min(h->nres_tiles, kResidualParts), h->dres, s, h->hres);
 }
 
 }  // namespace
