@@ -42,12 +42,12 @@ EXPORTS = [
     "bsam_get_level", "bsam_set_level", "bsam_export_size", "bsam_export", "bsam_hier_reset",
     "bsam_time_smoother", "bsam_smoother_bytes", "bsam_update_coeffs", "bsam_parabolic_init",
     "bsam_parabolic_set_history", "bsam_parabolic_get_history", "bsam_parabolic_set_source",
-    "bsam_parabolic_rhs", "bsam_parabolic_advance", "bsam_parabolic_step", "bsam_batch_create",
+    "bsam_parabolic_rhs", "bsam_parabolic_set_shift", "bsam_parabolic_advance", "bsam_parabolic_step", "bsam_batch_create",
     "bsam_batch_destroy", "bsam_batch_step", "bsam_tag_level", "bsam_polygon_fields",
     "bsam_hier_inherit", "bsam_regrid_level", "bsam_parabolic_uo_from_q", "bsam_cluster_tags",
 ]
 
-BSAM_BE, BSAM_BDF2 = 0, 1
+BSAM_BE, BSAM_BDF2, BSAM_CN = 0, 1, 2
 BSAM_TAG_ULAP, BSAM_TAG_RTE = 0, 1
 
 
@@ -120,6 +120,7 @@ def load(path=None):
         "bsam_parabolic_get_history": (c_int, [vp, c_int, P_double]),
         "bsam_parabolic_set_source": (c_int, [vp, P_double]),
         "bsam_parabolic_rhs": (c_int, [vp, c_double, c_int]),
+        "bsam_parabolic_set_shift": (c_int, [vp, c_double]),
         "bsam_parabolic_advance": (c_int, [vp]),
         "bsam_parabolic_step": (c_int, [vp, c_double, c_int, ctypes.POINTER(SolveOpts),
                                         ctypes.POINTER(StageStats), P_double]),

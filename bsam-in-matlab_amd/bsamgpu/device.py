@@ -292,6 +292,10 @@ class GpuHierarchy:
         check(self.lib.bsam_parabolic_set_source(self.h, dptr(a) if a is not None else None),
               "bsam_parabolic_set_source")
 
+    def parabolic_set_shift(self, aDt):
+        """The shift alpha/Dt that the resident CC carries (H.par.aDt)."""
+        check(self.lib.bsam_parabolic_set_shift(self.h, float(aDt)), "bsam_parabolic_set_shift")
+
     def parabolic_rhs(self, dt, scheme):
         check(self.lib.bsam_parabolic_rhs(self.h, float(dt), _scheme_code(scheme)), "bsam_parabolic_rhs")
 
@@ -314,9 +318,9 @@ class GpuHierarchy:
 
 
 def _scheme_code(scheme):
-    codes = {"be": _lib.BSAM_BE, "bdf2": _lib.BSAM_BDF2}
+    codes = {"be": _lib.BSAM_BE, "bdf2": _lib.BSAM_BDF2, "cn": _lib.BSAM_CN}
     if scheme not in codes:
-        raise _lib.BsamError("ebp_rhs:scheme", f"scheme {scheme} is not available on the device (be, bdf2)")
+        raise _lib.BsamError("ebp_rhs:scheme", f"unknown scheme {scheme}")
     return codes[scheme]
 
 

@@ -17,10 +17,10 @@ evaluates from user functions or decides structurally:
   * ``ebp_regrid``         <- ParabolicBSAM/src/ebpRegrid.m:1-162
   * ``ebp_solve``          <- ParabolicBSAM/src/ebpSolve.m:1-107
 
-Schemes: 'be' and 'bdf2' (BE bootstrap on the first step).  Crank-Nicolson
-needs the explicit old-time composite operator with its flux corrections on
-the right-hand side, which the device library does not provide; asking for
-it raises ``ebp_rhs:scheme``.
+Schemes: 'be', 'bdf2' (BE bootstrap on the first step) and 'cn'.  The
+Crank-Nicolson right-hand side applies the old-time composite operator, flux
+corrections included, on the device before the coefficients move to t1
+(``cn_rhs``; ParabolicBSAM/src/ebpRhs.m:15-67).
 """
 from __future__ import annotations
 
@@ -287,6 +287,25 @@ def _source(S: _State, pp: ParabolicProblem, t1):
     S.src_key = key
 
 
+def cn_rhs(S: _State, pp: ParabolicProblem, t1, Dt):
+    """ebpRhs.m cn branch: the explicit half with the operator of the old
+    time, so it runs before set_time moves the coefficients to t1; the
+    source is taken at the midpoint t1 - Dt/2."""
+    _source(S, pp, t1 - Dt / 2)
+    S.G.parabolic_set_shift(S.aDt)
+    S.G.parabolic_rhs(Dt, "cn")
+
+
+def _cn_step(S: _State, pp: ParabolicProblem, t1, Dt, eopts):
+    """ebpSolve.m:46-58 for cn: rhs, operator of t1, solve, history, mass."""
+    cn_rhs(S, pp, t1, Dt)
+    set_time(S, pp, t1, ALPHA["cn"] / Dt)
+    st = S.G.solve_cycles(eopts)
+    S.G.parabolic_advance()
+    m, a = S.G.composite_mean()
+    return st, m * a
+
+
 # ---------------------------------------------------------------- regrid
 def ebp_regrid(S: _State, pp: ParabolicProblem, t, aDtNext, opts: ParabolicOptions):
     """ebpRegrid.m:15-59: rebuild around the current solution and transfer
@@ -336,8 +355,6 @@ def _matlab_round(x):
 def ebp_solve(pp: ParabolicProblem, opts: ParabolicOptions | None = None, **kw) -> Solution:
     """Implicit time stepping, one AFAS solve per step (ebpSolve.m:17-98)."""
     opts = opts if opts is not None else parabolic_options(**kw)
-    if opts.Scheme == "cn":
-        raise BsamError("ebp_rhs:scheme", "scheme cn is not available on the device (be, bdf2)")
     nsteps = max(1, _matlab_round(opts.TFinal / opts.Dt))
     Dt = opts.TFinal / nsteps
     scheme1 = "be" if opts.Scheme == "bdf2" else opts.Scheme
@@ -349,11 +366,14 @@ def ebp_solve(pp: ParabolicProblem, opts: ParabolicOptions | None = None, **kw) 
     for n in range(1, nsteps + 1):
         t1 = n * Dt
         scheme = scheme1 if n == 1 else opts.Scheme
-        _source(S, pp, t1)
-        # be/bdf2 right-hand sides do not read the operator, so the refresh
-        # may precede them (the reference refreshes after ebpRhs)
-        set_time(S, pp, t1, ALPHA[scheme] / Dt)
-        st, mn = S.G.parabolic_step(Dt, scheme, opts.eb)
+        if scheme == "cn":
+            st, mn = _cn_step(S, pp, t1, Dt, opts.eb)
+        else:
+            _source(S, pp, t1)
+            # be/bdf2 right-hand sides do not read the operator, so the
+            # refresh may precede them (the reference refreshes after ebpRhs)
+            set_time(S, pp, t1, ALPHA[scheme] / Dt)
+            st, mn = S.G.parabolic_step(Dt, scheme, opts.eb)
         S.H.stats["stages"].append(st)
         stages.append(st)
         cycles.append(st["vcycles"])

@@ -29,7 +29,6 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 from . import _lib, diagnostics, hierarchy, parabolic, solve, tagcluster
-from .problem import BsamError
 from ._lib import check
 from .device import GpuHierarchy, context, solve_opts
 
@@ -200,17 +199,19 @@ class ParabolicRequest(Request):
         self.n += 1
         t1 = self.n * self.dt
         scheme = self.scheme1 if self.n == 1 else self.popts.Scheme
-        parabolic._source(self.S, self.pp, t1)
-        parabolic.set_time(self.S, self.pp, t1, parabolic.ALPHA[scheme] / self.dt)
-        self.S.G.parabolic_rhs(self.dt, scheme)
+        if scheme == "cn":  # explicit half with the old operator first
+            parabolic.cn_rhs(self.S, self.pp, t1, self.dt)
+            parabolic.set_time(self.S, self.pp, t1, parabolic.ALPHA[scheme] / self.dt)
+        else:
+            parabolic._source(self.S, self.pp, t1)
+            parabolic.set_time(self.S, self.pp, t1, parabolic.ALPHA[scheme] / self.dt)
+            self.S.G.parabolic_rhs(self.dt, scheme)
         self.G = self.S.G
         self.H = self.S.H
         self.phase = 0
 
     def prepare(self):
         po = self.popts
-        if po.Scheme == "cn":
-            raise BsamError("ebp_rhs:scheme", "scheme cn is not available on the device (be, bdf2)")
         self.nsteps = max(1, parabolic._matlab_round(po.TFinal / po.Dt))
         self.dt = po.TFinal / self.nsteps
         self.scheme1 = "be" if po.Scheme == "bdf2" else po.Scheme

@@ -112,6 +112,8 @@ template <typename T> void launch_reset_force(const LevelDev<T>* LV_dev, int nle
 template <typename TD, typename TS> void launch_convert(TD* d, const TS* src, long long n, hipStream_t s);
 void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double* Uoo, double dt, int first,
                      long long n, hipStream_t s);
+// FT = 2 f + c Uo (f may be null: 0), the pointwise part of the CN right side
+void launch_cn_rhs(double* FT, const double* f, const double* Uo, double c, long long n, hipStream_t s);
 // rotate 0: Uoo = Uo = interior(Q); 1: Uoo = Uo, Uo = interior(Q); 2: Uo = interior(Q)
 void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rotate, hipStream_t s);
 // one level's regrid transfer (Q interior and, with hist, its history)

@@ -1761,7 +1761,7 @@ __global__ void __launch_bounds__(kTile) k_convert(TD* __restrict__ d, const TS*
   if (t < n) d[t] = (TD)s[t];
 }
 
-// BDF2 / BE right-hand side (ParabolicBSAM/src/ebpRhs.m:149-151)
+// BDF2 / BE right-hand side (ParabolicBSAM/src/ebpRhs.m:31-34)
 __global__ void __launch_bounds__(kTile) k_bdf2_rhs(double* __restrict__ FT, const double* __restrict__ f,
                                                     const double* __restrict__ Uo,
                                                     const double* __restrict__ Uoo, double dt,
@@ -1770,6 +1770,16 @@ __global__ void __launch_bounds__(kTile) k_bdf2_rhs(double* __restrict__ FT, con
   if (t >= n) return;
   const double fv = f ? f[t] : 0.0;
   FT[t] = first ? fv + Uo[t] / dt : fv + (2.0 * Uo[t] - 0.5 * Uoo[t]) / dt;
+}
+
+// the pointwise part of the Crank-Nicolson right-hand side, 2 f + c Uo with
+// c = 2/dt + aDt (ParabolicBSAM/src/ebpRhs.m:35-38); the operator part follows
+__global__ void __launch_bounds__(kTile) k_cn_rhs(double* __restrict__ FT, const double* __restrict__ f,
+                                                  const double* __restrict__ Uo, double c, long long n) {
+  const long long t = (long long)blockIdx.x * kTile + threadIdx.x;
+  if (t >= n) return;
+  const double fv = f ? f[t] : 0.0;
+  FT[t] = 2.0 * fv + c * Uo[t];
 }
 
 // history: rotate 0: Uoo = Uo = interior(Q); 1: Uoo = Uo, Uo = interior(Q);
@@ -2037,6 +2047,11 @@ void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double
   hipLaunchKernelGGL(k_bdf2_rhs, dim3(nblk(n)), dim3(kTile), 0, s, FT, f, Uo, Uoo, dt, first, n);
 }
 
+void launch_cn_rhs(double* FT, const double* f, const double* Uo, double c, long long n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_cn_rhs, dim3(nblk(n)), dim3(kTile), 0, s, FT, f, Uo, c, n);
+}
+
 void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rotate, hipStream_t s) {
   if (!L.ntile) return;
   hipLaunchKernelGGL(k_history, dim3(L.ntile), dim3(kTile), 0, s, L, Uo, Uoo, rotate);
@@ -2287,6 +2302,8 @@ BSAM_INST(double)
 BSAM_INST(float)
 template void launch_residual_all<double, float>(const LevelDev<double>*, int, const int2*, int, int, ResidualOut*,
                                                  float* const*, int, hipStream_t);
+template void launch_residual_all<double, double>(const LevelDev<double>*, int, const int2*, int, int, ResidualOut*,
+                                                 double* const*, int, hipStream_t);
 template void launch_residual_all<float, float>(const LevelDev<float>*, int, const int2*, int, int, ResidualOut*,
                                                 float* const*, int, hipStream_t);
 template void launch_convert<float, double>(float*, const double*, long long, hipStream_t);

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -127,6 +128,7 @@ struct bsam_hier {
   int2* qc_items = nullptr;                // (level, item): ring + QC interior, k >= max(2, root+1)
   int nqc_items = 0;
   float** rout32 = nullptr;                // per level FP32 FTrue (defect right side)
+  double** rout64 = nullptr;               // per level FP64 FTrue (cn right side)
   float* q32all = nullptr;                 // FP32 Q of all levels (contiguous)
   float* qc32all = nullptr;                // FP32 QC of all levels (contiguous)
   long long nq32all = 0, nqc32all = 0;
@@ -138,6 +140,7 @@ struct bsam_hier {
   bool have_uoo = false;                   // Uoo holds u^{n-1} (after one step or an upload)
   double* src = nullptr;                   // resident source f(t1), packed cells
   bool have_src = false;
+  double par_shift = std::numeric_limits<double>::quiet_NaN();  // H.par.aDt (cn right side)
   std::vector<long long> cell_base;        // per level offset in the history arrays
   long long ncell_all = 0;
   hipGraph_t graph = nullptr;
@@ -2291,13 +2294,51 @@ int bsam_parabolic_set_source(bsam_hier* h, const double* f) {
   return sync(h);
 }
 
+int bsam_parabolic_set_shift(bsam_hier* h, double aDt) {
+  if (int rc_ = need_dev(h)) return rc_;
+  if (!std::isfinite(aDt)) return fail(BSAM_E_ARG, "bsamgpu:arg: the shift must be finite");
+  h->par_shift = aDt;
+  return 0;
+}
+
+// cn (ebpRhs.m:17-22, 41-43, 52-66): fill-down and ghost fill of u^n, the
+// pointwise part 2 f + (2/dt + aDt) Uo into FTrue, then the composite
+// residual machinery turns it into FTrue - L(u^n) + reflux chunks in place:
+// the QC refresh (ring, and the interior R(u) the fill-down has just written)
+// and the per-cell expression and chunk order are those of ebCompResidual.m,
+// which accumulates the chunks in the same (level, patch, face, entry) order
+// as the cn branch.
+static int enqueue_cn_rhs(bsam_hier* h, double dt) {
+  if (!std::isfinite(h->par_shift))
+    return fail(BSAM_E_ARG, "ebp_rhs:shift: cn needs the operator's shift (bsam_parabolic_set_shift)");
+  if (!h->rout64) {
+    std::vector<double*> ft(h->nlev);
+    for (int k = 0; k < h->nlev; ++k) ft[k] = h->d64[k].FT;
+    if (int rc = upload(h, &h->rout64, ft)) return rc;
+  }
+  Seq<double> q = seq64(h, h->stream);
+  q.filldown();
+  for (int k = 1; k <= h->nlev; ++k) q.fill(k);
+  const double c = 2.0 / dt + h->par_shift;
+  for (int k = h->root; k <= h->nlev; ++k) {
+    const long long b = h->cell_base[k - 1];
+    launch_cn_rhs(h->d64[k - 1].FT, h->have_src ? h->src + b : nullptr, h->Uo + b, c, h->H[k - 1].ncell, h->stream);
+  }
+  if (h->opts.mass_correction) launch_qc_refresh_all<double>(h->lv64, h->qc_items, h->nqc_items, h->stream);
+  launch_residual_all<double, double>(h->lv64, h->nlev, h->res_tiles, h->nres_tiles, h->opts.mass_correction,
+                                      h->parts, h->rout64, h->root, h->stream);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 static int enqueue_rhs(bsam_hier* h, double dt, int scheme) {
   if (!h->Uo) return fail(BSAM_E_ARG, "bsamgpu:arg: history not initialised");
   if (!(dt > 0) || !std::isfinite(dt)) return fail(BSAM_E_ARG, "ebp_options:Dt: Dt must be finite and positive");
-  if (scheme != BSAM_BE && scheme != BSAM_BDF2)
-    return fail(BSAM_E_ARG, "ebp_rhs:scheme: scheme %d is not available on the device (be, bdf2)", scheme);
+  if (scheme != BSAM_BE && scheme != BSAM_BDF2 && scheme != BSAM_CN)
+    return fail(BSAM_E_ARG, "ebp_rhs:scheme: unknown scheme %d", scheme);
   if (scheme == BSAM_BDF2 && !h->have_uoo)
     return fail(BSAM_E_ARG, "ebp_rhs:history: bdf2 needs u^{n-1} (take a be step first)");
+  if (scheme == BSAM_CN) return enqueue_cn_rhs(h, dt);
   // sub-root levels keep their FTrue: the FAS load overwrites them (ebpRhs.m:25-27)
   for (int k = h->root; k <= h->nlev; ++k) {
     const long long b = h->cell_base[k - 1];

@@ -16,7 +16,10 @@
  *   bsamgpu_mex('parabolic_init', h)             Uo = interior(Q)
  *   bsamgpu_mex('set_history', h, Uo, Uoo)       U{k}{p} mx x my ([] keeps)
  *   bsamgpu_mex('set_source', h, F)              F{k}{p} mx x my, [] for f = 0
- *   [st, mass] = bsamgpu_mex('parabolic_step', h, dt, scheme, opts)   scheme 'be' | 'bdf2'
+ *   [st, mass] = bsamgpu_mex('parabolic_step', h, dt, scheme, opts)   scheme 'be' | 'bdf2' | 'cn'
+ *   bsamgpu_mex('parabolic_rhs', h, dt, scheme [, aDt])   ebpRhs alone (cn: aDt = H.par.aDt,
+ *                                                         before ebpSetTime moves to t1)
+ *   mass = bsamgpu_mex('parabolic_advance', h)   history rotation and compMass after a 'solve'
  *   U = bsamgpu_mex('get_history', h, which)     0: Uo, 1: Uoo
  *   bsamgpu_mex('destroy', h)
  *
@@ -32,6 +35,13 @@
 #include "bsamgpu.h"
 
 #define MAXDEV 16
+
+/* 'be' | 'bdf2' | 'cn' -> enum bsam_scheme (-1: the library rejects it) */
+static int scheme_code(const mxArray* a) {
+  char sch[8] = {0};
+  mxGetString(a, sch, sizeof sch);
+  return !strcmp(sch, "be") ? BSAM_BE : !strcmp(sch, "bdf2") ? BSAM_BDF2 : !strcmp(sch, "cn") ? BSAM_CN : -1;
+}
 static bsam_ctx* g_ctx[MAXDEV];
 
 static void at_exit(void) {
@@ -364,10 +374,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     double mass = 0;
     if (step) {
       if (nrhs < 4 || !mxIsChar(prhs[3])) mexErrMsgIdAndTxt("bsamgpu:arg", "parabolic_step(h, dt, scheme, opts)");
-      char sch[8] = {0};
-      mxGetString(prhs[3], sch, sizeof sch);
-      const int scheme = !strcmp(sch, "be") ? BSAM_BE : !strcmp(sch, "bdf2") ? BSAM_BDF2 : -1;
-      chk(bsam_parabolic_step(h, mxGetScalar(prhs[2]), scheme, &so, &st, &mass));
+      chk(bsam_parabolic_step(h, mxGetScalar(prhs[2]), scheme_code(prhs[3]), &so, &st, &mass));
     } else {
       chk(bsam_solve_cycles(h, &so, &st));
     }
@@ -401,6 +408,15 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     chk(rc);
   } else if (!strcmp(cmd, "parabolic_init")) {
     chk(bsam_parabolic_init(h));
+  } else if (!strcmp(cmd, "parabolic_rhs")) {
+    if (nrhs < 4 || !mxIsChar(prhs[3])) mexErrMsgIdAndTxt("bsamgpu:arg", "parabolic_rhs(h, dt, scheme [, aDt])");
+    if (nrhs > 4) chk(bsam_parabolic_set_shift(h, mxGetScalar(prhs[4])));
+    chk(bsam_parabolic_rhs(h, mxGetScalar(prhs[2]), scheme_code(prhs[3])));
+  } else if (!strcmp(cmd, "parabolic_advance")) {
+    double mbar = 0, area = 0;
+    chk(bsam_parabolic_advance(h));
+    chk(bsam_composite_mean(h, &mbar, &area));
+    plhs[0] = mxCreateDoubleScalar(mbar * area);
   } else if (!strcmp(cmd, "set_history")) {
     double* a = nrhs > 2 ? pack_cells(prhs[2], e) : NULL;
     double* b = nrhs > 3 ? pack_cells(prhs[3], e) : NULL;

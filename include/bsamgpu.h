@@ -274,7 +274,7 @@ int bsam_parabolic_uo_from_q(bsam_hier *h);
 /* Parabolic stepping (ParabolicBSAM/src/ebpSolve.m:23-78).  The history
  * H.par.Uo = u^n, H.par.Uoo = u^{n-1} and the source f(t1) are device-resident
  * arrays packed like FTrue over all levels 1..nlev. */
-enum bsam_scheme { BSAM_BE = 0, BSAM_BDF2 = 1 };
+enum bsam_scheme { BSAM_BE = 0, BSAM_BDF2 = 1, BSAM_CN = 2 };
 /* Uo = interior(Q), no u^{n-1} yet (ebpInit.m:32-35) */
 int bsam_parabolic_init(bsam_hier *h);
 /* Upload Uo and/or Uoo (NULL keeps the resident array); after a regrid
@@ -285,16 +285,28 @@ int bsam_parabolic_get_history(bsam_hier *h, int which, double *out);
 /* f(x,y,t1) at cell centres, packed like FTrue (sub-root entries unused);
  * kept resident until replaced; NULL means f = 0 (ebpRhs.m:31-36) */
 int bsam_parabolic_set_source(bsam_hier *h, const double *f);
-/* FTrue on levels >= root (ParabolicBSAM/src/ebpRhs.m:25-48):
+/* The shift aDt = alpha/Dt that the resident CC carries (C_eff = C + aDt),
+ * H.par.aDt of ebpSetTime.m:54; only the cn right side reads it. */
+int bsam_parabolic_set_shift(bsam_hier *h, double aDt);
+/* FTrue on levels >= root (ParabolicBSAM/src/ebpRhs.m:15-67):
  *   BSAM_BE:   f + Uo/dt
- *   BSAM_BDF2: f + (2 Uo - Uoo/2)/dt */
+ *   BSAM_BDF2: f + (2 Uo - Uoo/2)/dt
+ *   BSAM_CN:   2 f + (2/dt + aDt) Uo - L(u^n), with f taken at t1 - dt/2, L
+ *              the operator in place (old coefficients: call before the
+ *              coefficients move to t1) after a fill-down and ghost fill of
+ *              u^n, plus the old state's coarse-fine flux corrections added
+ *              to the parent cells when mass correction is on. */
 int bsam_parabolic_rhs(bsam_hier *h, double dt, int scheme);
 /* Uoo = Uo, Uo = interior(Q) after an accepted step (ebpSolve.m:55-57) */
 int bsam_parabolic_advance(bsam_hier *h);
 /* One accepted step of ebpSolve.m:40-58 without host traffic beyond the
  * per-cycle residual: rhs, ebSolveCycles (opts as in bsam_solve_cycles),
  * history rotation and compMass = composite mean * area (ebpSolve.m:103-106;
- * mass may be NULL). */
+ * mass may be NULL).  be and bdf2 read no operator in the rhs, so the
+ * coefficients of t1 are set before the call; cn reads the operator in place
+ * for both halves, which is the step only when D, C and the boundary data do
+ * not depend on t (else: bsam_parabolic_rhs, coefficients of t1,
+ * bsam_solve_cycles, bsam_parabolic_advance). */
 int bsam_parabolic_step(bsam_hier *h, double dt, int scheme, const bsam_solve_opts *opts,
                         bsam_stage_stats *out, double *mass);
 

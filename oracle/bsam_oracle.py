@@ -1012,7 +1012,8 @@ def eb_solve(prob, opts, host, verbose=0):
 # clustering, level construction, coefficient fill-down) come from the host
 # module passed in, exactly as for eb_solve above.
 def ebp_rhs(f, Uo, Uoo, dt, scheme):
-    """ebpRhs.m:31-36 (be / bdf2 branches)."""
+    """ebpRhs.m:31-36 (be / bdf2 branches; cn needs the hierarchy, see
+    ParabolicOracle.cn_rhs)."""
     if scheme == "be":
         return f + Uo / dt
     if scheme == "bdf2":
@@ -1166,10 +1167,49 @@ class ParabolicOracle:
         O.coarsest_assemble()
         self.stateKey = key
 
+    # ---- ebpRhs.m:16-67, the cn branch: 2 f(t1 - dt/2) + (2/dt + aDt) Uo - L(u^n)
+    # with the current (old-time) operator, plus the old state's coarse-fine
+    # flux corrections added to the parent cells
+    def cn_rhs(self, t1, dt):
+        O = self.O
+        dom = self.pp.domain
+        O.transfer(0, "filldown")
+        for k in range(1, O.nlev + 1):
+            O.fill_ghosts(k)
+        aDt = self.stateKey[1]
+        for k in range(O.rootIdx, O.nlev + 1):
+            lev = O.L(k)
+            for p in range(lev.np):
+                b = lev.box[p]
+                X, Y = np.meshgrid(dom[0] + (np.arange(b[0], b[1] + 1, dtype=float) - 0.5) * lev.h[0],
+                                   dom[2] + (np.arange(b[2], b[3] + 1, dtype=float) - 0.5) * lev.h[1],
+                                   indexing="ij")
+                f = np.broadcast_to(self.pp.f(X, Y, t1 - dt / 2), X.shape).astype(float)
+                Le = op2d(lev.Q[p], lev.DeW[p], lev.DnS[p], lev.CC[p], lev.h[0], lev.h[1])
+                lev.FTrue[p] = 2 * f + (2 / dt + aDt) * self.Uo[k - 1][p] - Le
+        if O.opts.MassCorrection:
+            for k in range(O.rootIdx + 1, O.nlev + 1):
+                O.transfer(k, "ring")
+                lev = O.L(k)
+                FTpar = O.L(k - 1).FTrue
+                for p in range(lev.np):
+                    vals = O.face_corrections(k, p)
+                    for fc in range(4):
+                        for q, src, di, dj in lev.ops[p]["corr"][fc]:
+                            FTpar[q][di, dj] = FTpar[q][di, dj] + vals[fc][src]
+
     # ---- one step of ebpSolve.m:40-58
     def step(self, t1, dt, scheme):
         O = self.O
         dom = self.pp.domain
+        if scheme == "cn":
+            self.cn_rhs(t1, dt)
+            self.set_time(t1, ALPHA_OF[scheme] / dt)
+            st = O.solve_cycles()
+            self.Uoo = self.Uo
+            self.Uo = self.interiors(O)
+            m, area, _ = O.composite_mean()
+            return st, m * area
         for k in range(O.rootIdx, O.nlev + 1):
             lev = O.L(k)
             for p in range(lev.np):

@@ -7,6 +7,7 @@
   reaction_t    runParabolicRegressions.m:6-8 (C = 1 + t, f = t e^-t, Neumann)
   diffusion_t   runParabolicRegressions.m:22-25 (D = 1 + t, time-dependent Dirichlet data)
   fixed_patch   runParabolicRegressions.m:43-50 (D = 1 + 0.2x, a fixed fine patch)
+  *_cn          the last three with Scheme 'cn', as runParabolicRegressions.m runs them
 """
 import numpy as np
 
@@ -100,5 +101,18 @@ def fixed_patch(**kw):
     return pp, par.parabolic_options(**o)
 
 
+def reaction_t_cn(**kw):
+    return reaction_t(**dict(dict(Scheme="cn"), **kw))
+
+
+def diffusion_t_cn(**kw):
+    return diffusion_t(**dict(dict(Scheme="cn"), **kw))
+
+
+def fixed_patch_cn(**kw):
+    return fixed_patch(**dict(dict(Scheme="cn"), **kw))
+
+
 CASES = {"heat_readme": heat_readme, "mms_bump": mms_bump, "reaction_t": reaction_t,
-         "diffusion_t": diffusion_t, "fixed_patch": fixed_patch}
+         "diffusion_t": diffusion_t, "fixed_patch": fixed_patch, "reaction_t_cn": reaction_t_cn,
+         "diffusion_t_cn": diffusion_t_cn, "fixed_patch_cn": fixed_patch_cn}

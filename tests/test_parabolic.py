@@ -16,7 +16,7 @@ from bsamgpu import _lib
 from bsamgpu import parabolic as par
 from bsamgpu.problem import BsamError
 from oracle import bsam_oracle as O
-from parabolic_cases import CASES, Host, heat_config, heat_readme
+from parabolic_cases import CASES, Host, diffusion_t_cn, fixed_patch_cn, heat_config, heat_readme, reaction_t_cn
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "published.json")))
 
@@ -53,6 +53,36 @@ def test_oracle_heat_mass_drift_at_tight_tolerance():
     _, R, _ = _oracle(heat_config)
     m = np.array(R["mass"])
     assert np.max(np.abs(m - m[0])) / abs(m[0]) <= 1e-12
+
+
+def test_oracle_cn_published_regressions():
+    """Crank-Nicolson (ebpRhs.m cn branch) against the published regression
+    log (verification/suite-output.txt:178-181, runParabolicRegressions.m):
+    errors of the time-dependent reaction at 6 digits, Cauchy differences
+    with time-dependent D and boundary data at 6 digits, and the composite
+    mass of the fixed-patch heat problem (drift below the reference's 2e-10,
+    largest per-step residual 1.157e-12 at 4 digits, without and with regrids)."""
+    errs = []
+    for dt in (0.1, 0.05, 0.025, 0.0125):
+        po, _, _ = _oracle(reaction_t_cn, Dt=dt)
+        q = po.O.L(po.O.nlev).Q[0][1:-1, 1:-1]
+        errs.append(float(np.max(np.abs(q - np.exp(-0.4)))))
+    assert [float(f"{e:.6g}") for e in errs] == [0.000359769, 8.98676e-05, 2.24623e-05, 5.61527e-06], errs
+    U = []
+    for dt in (0.1, 0.05, 0.025, 0.0125):
+        po, _, _ = _oracle(diffusion_t_cn, Dt=dt)
+        U.append(po.O.L(po.O.nlev).Q[0][1:-1, 1:-1].copy())
+    d = [float(np.max(np.abs(U[j] - U[j + 1]))) for j in range(3)]
+    assert [float(f"{x:.6g}") for x in d] == [7.6541e-05, 1.80441e-05, 4.53075e-06], d
+    for regrid in (0, 2):
+        po, R, _ = _oracle(fixed_patch_cn, RegridEvery=regrid)
+        assert po.O.nlev - po.O.rootIdx + 1 == 2
+        m = np.array(R["mass"])
+        drift = float(np.max(np.abs(m - m[0])))
+        rel = max(st["finalRel"] for st in R["stages"])
+        print(f"cn fixed patch regrid={regrid}: drift {drift:.3e}, max rel {rel:.3e}", flush=True)
+        assert drift < 2e-10
+        assert f"{rel:.3e}" == "1.157e-12"
 
 
 def test_oracle_reaction_t_exact_in_space():
@@ -180,7 +210,7 @@ def test_device_temporal_orders():
     """BE ~ first and BDF2 ~ second order in time on a fixed uniform mesh
     (Dt-Cauchy differences, runParabolicTests.m:35-62 at 64^2)."""
     rates = {}
-    for scheme in ("be", "bdf2"):
+    for scheme in ("be", "bdf2", "cn"):
         U = []
         for dt in (0.04, 0.02, 0.01, 0.005):
             pp, opts = CASES["mms_bump"](Scheme=scheme, Dt=dt, RegridEvery=0, BaseCells=64, MaxLevels=1,
@@ -193,6 +223,26 @@ def test_device_temporal_orders():
         print(scheme, d, rates[scheme], flush=True)
     assert rates["be"][-1] > 0.9
     assert rates["bdf2"][-1] > 1.8
+    assert rates["cn"][-1] > 1.8
+
+
+@pytest.mark.gpu
+def test_device_cn_published_regressions():
+    """Crank-Nicolson on the device against the published regression log
+    (verification/suite-output.txt:178-181): time-dependent reaction errors at
+    6 digits and the fixed-patch composite mass with regrids (drift below the
+    reference's 2e-10, largest per-step residual 1.157e-12 at 4 digits)."""
+    errs = []
+    for dt in (0.1, 0.05, 0.025, 0.0125):
+        sol, _ = _gpu(reaction_t_cn, Dt=dt)
+        H = sol["H"]
+        errs.append(float(np.max(np.abs(H.L(H.nlev).Q[0][1:-1, 1:-1] - np.exp(-0.4)))))
+    print("cn reaction errors", errs, flush=True)
+    assert [float(f"{e:.6g}") for e in errs] == [0.000359769, 8.98676e-05, 2.24623e-05, 5.61527e-06], errs
+    sol, _ = _gpu(fixed_patch_cn, RegridEvery=2)
+    m = np.array(sol["mass"])
+    assert float(np.max(np.abs(m - m[0]))) < 2e-10
+    assert f"{max(sol['relres']):.3e}" == "1.157e-12"
 
 
 @pytest.mark.gpu
@@ -210,6 +260,11 @@ def test_device_history_roundtrip_and_errors():
     assert "ebp_rhs:history" in str(e.value)
     with pytest.raises(BsamError):
         G.parabolic_rhs(-1.0, "be")
+    with pytest.raises(BsamError) as e:
+        G.parabolic_rhs(0.002, "cn")       # the operator's shift was never given
+    assert "ebp_rhs:shift" in str(e.value)
+    with pytest.raises(BsamError):
+        G.parabolic_set_shift(float("nan"))
     V = [[u * 2.0 for u in lev] for lev in U]
     G.set_history(Uo=V, Uoo=U)
     assert all(np.array_equal(a, b) for la, lb in zip(G.get_history(0), V) for a, b in zip(la, lb))

@@ -150,12 +150,14 @@ def test_realtime_poisson_arrivals():
 
 @pytest.mark.gpu
 def test_parabolic_requests_in_the_mix_match_ebp_solve():
-    """Parabolic requests (one stage per time step, BDF2 history resident)
-    served together with elliptic ones give exactly what ebp_solve gives."""
+    """Parabolic requests (one stage per time step, BDF2 history resident;
+    Crank-Nicolson with time-dependent coefficients and with regrids) served
+    together with elliptic ones give exactly what ebp_solve gives."""
     from bsamgpu import parabolic as par
-    from parabolic_cases import heat_readme, mms_bump
+    from parabolic_cases import diffusion_t_cn, fixed_patch_cn, heat_readme, mms_bump
 
-    cases = {"heat": heat_readme(TFinal=0.02), "bump": mms_bump(TFinal=0.2)}
+    cases = {"heat": heat_readme(TFinal=0.02), "bump": mms_bump(TFinal=0.2), "diff_cn": diffusion_t_cn(),
+             "patch_cn": fixed_patch_cn()}
     ref = {}
     for name, (pp, po) in cases.items():
         ref[name] = par.ebp_solve(pp, po)
