@@ -141,13 +141,14 @@ def bench_large_smoother(n, launches=40):
 def bench_parabolic():
     """Heat BDF2 (BaseCells 64, MaxLevels 4, Dt 0.002, T 0.1, RegridEvery 5,
     RelTol 1e-12 per step): wall time of ebp_solve with the history resident,
-    V-cycles, composite mass drift; FP32-inner against FP64."""
+    V-cycles, composite mass drift; FP32-inner against FP64, and the same
+    problem stepped by Crank-Nicolson."""
     from bsamgpu import parabolic as par
 
-    def run(precision):
+    def run(precision, scheme="bdf2"):
         pp = par.ParabolicProblem([0, 1, 0, 1], 1, 0, 0, ("neumann", 0),
                                   lambda x, y: 1 + 0.2 * np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y))
-        opts = par.parabolic_options(Scheme="bdf2", Dt=0.002, TFinal=0.1, BaseCells=64, MaxLevels=4,
+        opts = par.parabolic_options(Scheme=scheme, Dt=0.002, TFinal=0.1, BaseCells=64, MaxLevels=4,
                                      RegridEvery=5, StepVerbose=0, RelTol=1e-12, Precision=precision)
         t = time.perf_counter()
         sol = par.ebp_solve(pp, opts)
@@ -171,8 +172,9 @@ def bench_parabolic():
                 num += float(((a - b) ** 2).sum())
                 den += float((b ** 2).sum())
         rmx["rel_l2_vs_fp64"] = (num / den) ** 0.5
+    _, rcn = run("fp64", "cn")
     return {"workload": "heat BDF2 64^2 root, MaxLevels 4, 50 steps, 9 regrids, RelTol 1e-12",
-            "fp64": r64, "mixed": rmx}
+            "fp64": r64, "mixed": rmx, "cn_fp64": rcn}
 
 
 def bench_varcoef_refined(steps=5, warmup=2):
