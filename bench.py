@@ -117,11 +117,13 @@ def bench_large_smoother(n, launches=40):
     HBM-bound rather than launch-bound: the colour pass (half a sweep per
     launch) and the streaming sweep (a whole red+black sweep per launch),
     which the solve uses on such levels."""
-    from bsamgpu import Options, Problem, hierarchy
+    from bsamgpu import Options, Problem, _lib, hierarchy
     from bsamgpu.device import GpuHierarchy
 
     prob = Problem([0, 1, 0, 1], 1, 1, 1, ("dirichlet", 0))
-    H = hierarchy.setup(prob, Options(BaseCells=n, MaxLevels=1, Verbose=0))
+    # RelTol 1e-8: FP64 round-off bounds the relative residual near 1e-9 at
+    # this spacing (|L| ~ 8/h^2 = 1.3e8 times |u| ~ 0.07 times 2^-52)
+    H = hierarchy.setup(prob, Options(BaseCells=n, MaxLevels=1, Verbose=0, RelTol=1e-8))
     G = GpuHierarchy(H)
     k = H.nlev
     out = {"cells": n * n}
@@ -134,6 +136,38 @@ def bench_large_smoother(n, launches=40):
             key = f"{name}_colour_pass" if form == "passes" else f"{name}_streaming_sweep"
             out[key] = {"us_per_launch": round(ms * 1e3, 2), "bytes_per_launch": byt,
                         "GBps": round(byt / ms / 1e6, 1), "frac": round(byt / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    # the streaming sweep at each row-block height (0: the automatic choice)
+    from bsamgpu import autotune
+
+    rows = {}
+    for r in (0,) + autotune.STREAM_ROWS:
+        G.solve_cycles(max_vcycles=0, smoother="exact", stream_rows=r)
+        rows[str(r)] = {}
+        for name, mixed, b in (("fp64", False, 8), ("fp32", True, 4)):
+            G.time_smoother(k, 5, mixed=mixed)
+            ms = G.time_smoother(k, launches, mixed=mixed)
+            rows[str(r)][name] = {"us_per_launch": round(ms * 1e3, 2),
+                                  "frac": round(G.smoother_bytes(k, b) / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    out["streaming_sweep_by_rows"] = rows
+    # whole solves from zero to RelTol, the option search over precision and
+    # row-block height (coarse cycle off: one level of 2^24 cells and its
+    # sub-root levels); the mixed form's error against the FP64 solution
+    G.reset()
+    G.solve_cycles(precision="fp64", coarse_cells=-1)
+    q64 = G.get_field(k, _lib.BSAM_Q)[0][1:-1, 1:-1].copy()
+    G.reset()
+    G.solve_cycles(precision="mixed", coarse_cells=-1)
+    qmx = G.get_field(k, _lib.BSAM_Q)[0][1:-1, 1:-1]
+    rel_l2 = float(np.linalg.norm(qmx - q64) / np.linalg.norm(q64))
+    cands = [dict(coarse_cells=-1, smoother="exact", precision=p) for p in ("fp64", "mixed")]
+    best, table = autotune.autotune(G, bit_exact_only=False, repeats=3, candidates=cands)
+    t64 = min(r["seconds"] for r in table if r["precision"] == "fp64")
+    tmx = min(r["seconds"] for r in table if r["precision"] == "mixed")
+    out["solve"] = {"rel_tol": G.opts.RelTol, "autotune_pick": best, "fp64_ms": round(t64 * 1e3, 3),
+                    "mixed_ms": round(tmx * 1e3, 3), "speedup_mixed_vs_fp64": round(t64 / tmx, 3),
+                    "rel_l2_mixed_vs_fp64": rel_l2,
+                    "table": [{k: (float(f"{v:.6g}") if isinstance(v, float) else v) for k, v in r.items()}
+                              for r in table]}
     G.close()
     return out
 

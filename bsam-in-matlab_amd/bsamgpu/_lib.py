@@ -40,7 +40,7 @@ EXPORTS = [
     "bsam_fill_ghosts", "bsam_relax_level", "bsam_transfer", "bsam_coarse_load", "bsam_coarsest_solve",
     "bsam_reset_force", "bsam_init_level_prolong", "bsam_composite_mean", "bsam_level_field_size",
     "bsam_get_level", "bsam_set_level", "bsam_export_size", "bsam_export", "bsam_hier_reset",
-    "bsam_time_smoother", "bsam_smoother_bytes", "bsam_update_coeffs", "bsam_parabolic_init",
+    "bsam_time_smoother", "bsam_smoother_bytes", "bsam_level_streams", "bsam_update_coeffs", "bsam_parabolic_init",
     "bsam_parabolic_set_history", "bsam_parabolic_get_history", "bsam_parabolic_set_source",
     "bsam_parabolic_rhs", "bsam_parabolic_set_shift", "bsam_parabolic_advance", "bsam_parabolic_step", "bsam_batch_create",
     "bsam_batch_destroy", "bsam_batch_step", "bsam_tag_level", "bsam_polygon_fields",
@@ -62,7 +62,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("npre", c_int), ("npost", c_int), ("omega", c_double), ("rel_tol", c_double),
                 ("max_vcycles", c_int), ("ghost_interp", c_int), ("mass_correction", c_int),
                 ("precision", c_int), ("inner_cycles", c_int), ("use_graph", c_int),
-                ("coarse_cells", c_int), ("smoother", c_int)]
+                ("coarse_cells", c_int), ("smoother", c_int), ("stream_rows", c_int)]
 
 
 class StageStats(ctypes.Structure):
@@ -131,6 +131,7 @@ def load(path=None):
         "bsam_hier_reset": (c_int, [vp]),
         "bsam_time_smoother": (c_int, [vp, c_int, c_int, c_int, P_double]),
         "bsam_smoother_bytes": (c_double, [vp, c_int, c_int]),
+        "bsam_level_streams": (c_int, [vp, c_int]),
         "bsam_tag_level": (c_int, [vp, c_int, c_int, c_double, c_int, vp, vp, P_ll]),
         "bsam_hier_inherit": (c_int, [vp, vp]),
         "bsam_cluster_tags": (c_int, [vp, vp, c_int, c_int, c_int, c_double, P_int, c_int, P_int, P_int]),

@@ -8,12 +8,17 @@ The device path has knobs that change speed but not the converged solution:
   fused into one launch each on levels whose 32x32 tiles fit one round of
   workgroups, also bit-identical; ``tiled``: every pass of a relax call in
   one launch on LDS tiles with sibling ghosts frozen);
-- the arithmetic of the inner V-cycles (``fp64`` or ``mixed``).
+- the arithmetic of the inner V-cycles (``fp64`` or ``mixed``);
+- on levels that run streaming sweeps, the row-block height of a sweep
+  launch (``stream_rows``: the tile shape is 124 columns by that many rows,
+  one 64-lane wave each), which changes the number of waves per CU and the
+  share of recomputed halo rows but not the result.
 
 ``autotune`` times complete solves from the hierarchy's uploaded state for
 each candidate and returns the fastest one whose final relative residual
-meets RelTol.  ``bit_exact_only`` restricts the search to the candidates
-that reproduce the reference's FP64 sequence bit for bit.
+meets RelTol; when the hierarchy has a streaming level it then times the
+winner at each row-block height.  ``bit_exact_only`` restricts the search to
+the candidates that reproduce the reference's FP64 sequence bit for bit.
 """
 from __future__ import annotations
 
@@ -24,6 +29,7 @@ import time
 EXACT_CANDIDATES = [
     dict(coarse_cells=c, smoother=s, precision="fp64") for s in ("exact", "sweep") for c in (-1, 300, 1100, 4100)
 ]
+STREAM_ROWS = (8, 16, 32, 64)
 ALL_CANDIDATES = EXACT_CANDIDATES + [
     dict(coarse_cells=c, smoother=s, precision=p)
     for c, s, p in itertools.product((300, 1100), ("exact", "tiled"), ("fp64", "mixed"))
@@ -55,5 +61,14 @@ def autotune(G, rel_tol=None, bit_exact_only=True, repeats=3, candidates=None):
         table.append(dict(kw, seconds=sec, vcycles=st["vcycles"], finalRel=st["finalRel"]))
     ok = [r for r in table if r["finalRel"] <= rel_tol]
     best = min(ok or table, key=lambda r: r["seconds"])
+    keys = ("coarse_cells", "smoother", "precision")
+    best = dict({k: best[k] for k in keys}, stream_rows=best.get("stream_rows", 0), seconds=best["seconds"])
+    if best["smoother"] in ("exact", "sweep") and G.streaming_levels():
+        for r in STREAM_ROWS:
+            kw = dict({k: best[k] for k in keys}, stream_rows=r)
+            sec, st = time_solve(G, repeats=repeats, **kw)
+            table.append(dict(kw, seconds=sec, vcycles=st["vcycles"], finalRel=st["finalRel"]))
+            if st["finalRel"] <= rel_tol and sec < best["seconds"]:
+                best = dict(kw, seconds=sec)
     G.reset()
-    return {k: best[k] for k in ("coarse_cells", "smoother", "precision")}, table
+    return {k: best[k] for k in keys + ("stream_rows",)}, table

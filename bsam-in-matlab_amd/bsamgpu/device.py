@@ -24,7 +24,7 @@ def context(device=0):
 
 
 def solve_opts(opts, precision=None, use_graph=True, inner_cycles=1, max_vcycles=None, coarse_cells=0,
-               smoother=None):
+               smoother=None, stream_rows=0):
     o = _lib.default_opts()
     o.npre = int(opts.NPre)
     o.npost = int(opts.NPost)
@@ -40,6 +40,7 @@ def solve_opts(opts, precision=None, use_graph=True, inner_cycles=1, max_vcycles
     o.coarse_cells = int(coarse_cells)
     sm = smoother if smoother is not None else getattr(opts, "Smoother", "exact")
     o.smoother = {"exact": 0, "tiled": 1, "sweep": 2, "passes": 3}[sm] if isinstance(sm, str) else int(sm)
+    o.stream_rows = int(stream_rows)
     return o
 
 
@@ -241,6 +242,10 @@ class GpuHierarchy:
         check(self.lib.bsam_time_smoother(self.h, k, 1 if mixed else 0, nlaunch, ctypes.byref(ms)),
               "bsam_time_smoother")
         return ms.value
+
+    def streaming_levels(self):
+        """Levels whose sweeps run as streaming sweep launches."""
+        return [k for k in range(1, self.H.nlev + 1) if self.lib.bsam_level_streams(self.h, k) == 1]
 
     def smoother_bytes(self, k, elem_bytes=8):
         return self.lib.bsam_smoother_bytes(self.h, k, elem_bytes)

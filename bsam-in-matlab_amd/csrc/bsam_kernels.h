@@ -26,13 +26,13 @@ void launch_relax_sweep(const LevelDev<T>& L, const LevelDev<T>* parent, const T
 // one red+black sweep of a one-patch level with physical boundaries on every
 // side, Qin -> Qout: 64-lane waves on strips of kStreamCols written columns
 // (two per lane, one halo lane at each end) and blocks of StreamRows<T> rows
-// (the block height: the largest power of two from kStreamRowsMin up to
-// StreamRows<T> that keeps about 8 waves per CU)
+// (the block height: rows_req when positive, else the largest power of two
+// from kStreamRowsMin up to StreamRows<T> that keeps about 8 waves per CU)
 constexpr int kStreamCols = 124, kStreamRowsMin = 8;
 template <typename T> struct StreamRows { static constexpr int value = sizeof(T) == 8 ? 64 : 32; };
 template <typename T>
 void launch_sweep_stream(const LevelDev<T>& L, const PatchInfo& P, const T* Qin, T* Qout, T omega, int homog,
-                         int ncu, hipStream_t s);
+                         int ncu, int rows_req, hipStream_t s);
 // all passes of one relax call, one patch per workgroup staged in LDS
 constexpr long long kTiledLdsBytes = 64 * 1024;
 template <typename T> bool tiled_fits(const LevelDev<T>& L);

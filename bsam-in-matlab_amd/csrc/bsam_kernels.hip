@@ -2068,8 +2068,8 @@ int stream_rows(const PatchInfo& P, int ncu) {
 
 template <typename T>
 void launch_sweep_stream(const LevelDev<T>& L, const PatchInfo& P, const T* Qin, T* Qout, T omega, int homog,
-                         int ncu, hipStream_t s) {
-  const int rows = stream_rows<T>(P, ncu);
+                         int ncu, int rows_req, hipStream_t s) {
+  const int rows = rows_req > 0 ? rows_req : stream_rows<T>(P, ncu);
   const int nstrip = (P.mx + kStreamCols - 1) / kStreamCols;
   const int nblk_rows = (P.my + rows - 1) / rows;
   hipLaunchKernelGGL(k_sweep_stream<T>, dim3(nstrip * nblk_rows), dim3(64), 0, s, L, P, Qin, Qout, omega, homog,
@@ -2287,7 +2287,7 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int);            \
   template void launch_filldown_small<T>(const LevelDev<T>*, int, int, hipStream_t);                       \
   template void launch_reset_force<T>(const LevelDev<T>*, int, int, hipStream_t);                         \
-  template void launch_sweep_stream<T>(const LevelDev<T>&, const PatchInfo&, const T*, T*, T, int, int,          \
+  template void launch_sweep_stream<T>(const LevelDev<T>&, const PatchInfo&, const T*, T*, T, int, int, int,     \
                                        hipStream_t);                                                         \
   template void launch_relax_sweep<T>(const LevelDev<T>&, const LevelDev<T>*, const T*, T*, int, int, T, int,   \
                                       hipStream_t);                                                          \

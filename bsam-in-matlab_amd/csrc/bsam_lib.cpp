@@ -1120,7 +1120,7 @@ struct Seq {
       T* buf[2] = {odd ? Lk.Qs : Lk.Q, odd ? Lk.Q : Lk.Qs};
       for (int it = 0; it < nsweeps; ++it)
         launch_sweep_stream<T>(Lk, h->H[k - 1].pi[0], buf[it & 1], buf[(it + 1) & 1], (T)h->opts.omega, homog,
-                               h->ncu, s);
+                               h->ncu, h->opts.stream_rows, s);
       fill(k);
       return;
     }
@@ -1241,7 +1241,7 @@ void bsam_default_opts(bsam_solve_opts* o) {
   if (!o) return;
   o->npre = 3; o->npost = 3; o->omega = 1.0; o->rel_tol = 1e-9; o->max_vcycles = 60;
   o->ghost_interp = 0; o->mass_correction = 1; o->precision = BSAM_FP64; o->inner_cycles = 1;
-  o->use_graph = 1; o->coarse_cells = 0; o->smoother = 0;
+  o->use_graph = 1; o->coarse_cells = 0; o->smoother = 0; o->stream_rows = 0;
 }
 
 const char* bsam_last_error(void) { return g_err.c_str(); }
@@ -1611,6 +1611,12 @@ static int enqueue_cycle_mixed(bsam_hier* h, hipStream_t s) {
   return 0;
 }
 
+static int check_stream_rows(int r) {
+  if (r != 0 && r != 8 && r != 16 && r != 32 && r != 64)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: stream_rows must be 0, 8, 16, 32 or 64");
+  return 0;
+}
+
 static int run_cycle(bsam_hier* h, bool mixed) {
   hipStream_t s = h->stream;
   if (!h->opts.use_graph) {
@@ -1621,7 +1627,8 @@ static int run_cycle(bsam_hier* h, bool mixed) {
   }
   const int key = (mixed ? 1 : 0) | (h->opts.inner_cycles << 1) | (h->opts.npre << 8) | (h->opts.npost << 12) |
                   (h->opts.mass_correction << 16) | (h->opts.omega == 1.0 ? 0 : (1 << 17)) |
-                  (coarse_cut(h, h->opts.coarse_cells) << 18) | (h->opts.smoother << 26);
+                  (coarse_cut(h, h->opts.coarse_cells) << 18) | (h->opts.smoother << 26) |
+                  ((h->opts.stream_rows ? __builtin_ctz(h->opts.stream_rows) - 2 : 0) << 28);
   if (!h->gexec || h->graph_key != key || h->graph_omega != h->opts.omega) {
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
     if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
@@ -1651,6 +1658,7 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
     return fail(BSAM_E_ARG, "bsamgpu:arg: negative sweep or cycle count");
   if (h->opts.smoother < 0 || h->opts.smoother > 3)
     return fail(BSAM_E_ARG, "bsamgpu:arg: smoother must be 0, 1, 2 or 3");
+  if (int rc = check_stream_rows(h->opts.stream_rows)) return rc;
   const bool mixed = h->opts.precision == BSAM_MIXED;
   if (mixed)
     if (int rc = ensure32(h)) return rc;
@@ -1938,8 +1946,14 @@ double bsam_smoother_bytes(const bsam_hier* h, int k, int elem_bytes) {
   return sweep ? b * elem_bytes : b * elem_bytes / 2.0;
 }
 
+int bsam_level_streams(const bsam_hier* h, int k) {
+  if (!h || k < 1 || k > h->nlev || (int)h->d64.size() < k) return -1;
+  return h->d64[k - 1].stream ? 1 : 0;
+}
+
 int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* ms_per_launch) {
   if (int rc = check_level(h, k, 1)) return rc;
+  if (int rc = check_stream_rows(h->opts.stream_rows)) return rc;
   if (nlaunch < 1 || !ms_per_launch) return fail(BSAM_E_ARG, "bsamgpu:arg: bad smoother timing request");
   if (precision == BSAM_MIXED)
     if (int rc = ensure32(h)) return rc;
@@ -1955,11 +1969,11 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
       if (precision == BSAM_MIXED) {
         LevelDev<float>& D = h->d32[k - 1];
         launch_sweep_stream<float>(D, P, (i & 1) ? D.Qs : D.Q, (i & 1) ? D.Q : D.Qs, (float)h->opts.omega, 1,
-                                   h->ncu, h->stream);
+                                   h->ncu, h->opts.stream_rows, h->stream);
       } else {
         LevelDev<double>& D = h->d64[k - 1];
         launch_sweep_stream<double>(D, P, (i & 1) ? D.Qs : D.Q, (i & 1) ? D.Q : D.Qs, h->opts.omega, 0, h->ncu,
-                                    h->stream);
+                                    h->opts.stream_rows, h->stream);
       }
       continue;
     }

@@ -191,6 +191,7 @@ static void opts_from(const mxArray* o, bsam_solve_opts* so) {
   if ((f = mxGetField(o, 0, "RelTol"))) so->rel_tol = mxGetScalar(f);
   if ((f = mxGetField(o, 0, "MaxVCycles"))) so->max_vcycles = (int)mxGetScalar(f);
   if ((f = mxGetField(o, 0, "MassCorrection"))) so->mass_correction = mxGetScalar(f) != 0;
+  if ((f = mxGetField(o, 0, "StreamRows"))) so->stream_rows = (int)mxGetScalar(f);
   if ((f = mxGetField(o, 0, "GhostInterp")) && mxIsChar(f)) {
     char s[8] = {0};
     mxGetString(f, s, sizeof s);

@@ -109,6 +109,11 @@ typedef struct {
                           with physical sides only and >= 2^20 cells runs pairs of
                           sweeps as streaming sweep launches (one pass over the
                           rows per red+black sweep), also bit-identical */
+  int stream_rows;     /* row-block height of a streaming sweep launch (one 64-lane
+                          wave per block of 124 columns x stream_rows rows):
+                          0 = automatic (about 8 waves per CU, at most 64 rows in
+                          FP64 and 32 in FP32), else 8, 16, 32 or 64; the result
+                          is the same for every value */
 } bsam_solve_opts;
 
 /* One solve stage (EllipticBSAM/src/ebSolveCycles.m:58-70) plus device timing. */
@@ -222,6 +227,11 @@ int bsam_hier_reset(bsam_hier *h);
  * B/2 for a colour pass. */
 int bsam_time_smoother(bsam_hier *h, int k, int precision, int nlaunch, double *ms_per_launch);
 double bsam_smoother_bytes(const bsam_hier *h, int k, int elem_bytes);
+/* 1 when level k runs its sweeps as streaming sweep launches under smoothers
+ * 0 and 2 (one patch of even width, physical sides only, >= 2^20 cells), so
+ * that the stream_rows option applies to it; 0 otherwise; -1 on a bad level.
+ * For option searches (no counterpart in the reference). */
+int bsam_level_streams(const bsam_hier *h, int k);
 
 /* Packed per-level field access (layout as in the descriptor; QC is
  * (mx/2+2)*(my/2+2) per patch, levels >= 2). */

@@ -281,4 +281,14 @@ def test_streaming_sweeps_on_a_uniform_level(omega):
         print(f"omega {omega} {prec}: {len(out['exact'][0]) - 1} cycles", flush=True)
         assert out["exact"][0] == out["passes"][0]
         assert all(np.array_equal(a, b) for a, b in zip(out["exact"][1], out["passes"][1]))
+        # every row-block height of the streaming launch gives the same solve
+        for rows in (8, 16, 32, 64):
+            G.reset()
+            st = G.solve_cycles(smoother="exact", precision=prec, stream_rows=rows)
+            assert st["resHist"] == out["passes"][0], rows
+            assert all(np.array_equal(a, b) for a, b in zip(G.get_field(H.nlev, _lib.BSAM_Q), out["passes"][1]))
+    assert G.streaming_levels() == [H.nlev]
+    with pytest.raises(Exception) as ei:
+        G.solve_cycles(stream_rows=12)
+    assert "stream_rows" in str(ei.value)
     G.close()
