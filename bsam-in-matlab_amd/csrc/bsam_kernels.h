@@ -55,7 +55,8 @@ void launch_tag_finish(const uint8_t* raw, const uint8_t* cover, int n1, int n2,
                        hipStream_t s);
 template <typename T> void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
 // restrict k -> k-1 + fill k-1 for k = kfrom..2 in one workgroup
-template <typename T> void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int homog, hipStream_t s);
+template <typename T>
+void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int homog, hipStream_t s);
 template <typename T> void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s);
 // ebCoarseLoad alone (QC ring as stored) and the fused ring + load + reflux
 template <typename T> void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);

@@ -1226,14 +1226,14 @@ __device__ void blk_vcycle(const LevelDev<T>* __restrict__ LV, int kc, const Coa
   }
 }
 
-// The bottom of the filldown (ebTransfer.m:73-79) for levels kfrom..1 in one
-// workgroup: restrict k -> k-1, fill k-1, for k = kfrom..2, with barriers
+// The bottom of the filldown (ebTransfer.m:73-79) in one workgroup:
+// restrict k -> k-1, fill k-1, for k = kfrom..kto (kto >= 2), with barriers
 // between the dependent steps (the small levels' steps would otherwise each
 // cost a launch).
 template <typename T>
-__global__ void __launch_bounds__(1024) k_filldown_small(const LevelDev<T>* __restrict__ LV, int kfrom,
+__global__ void __launch_bounds__(1024) k_filldown_small(const LevelDev<T>* __restrict__ LV, int kfrom, int kto,
                                                           int homog) {
-  for (int k = kfrom; k >= 2; --k) {
+  for (int k = kfrom; k >= kto; --k) {
     const LevelDev<T>& L = LV[k - 1];
     const LevelDev<T>& P = LV[k - 2];
     for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
@@ -1937,9 +1937,10 @@ void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s)
 }
 
 template <typename T>
-void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int homog, hipStream_t s) {
-  if (kfrom < 2) return;
-  hipLaunchKernelGGL(k_filldown_small<T>, dim3(1), dim3(1024), 0, s, LV_dev, kfrom, homog);
+void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int homog, hipStream_t s) {
+  kto = std::max(kto, 2);
+  if (kfrom < kto) return;
+  hipLaunchKernelGGL(k_filldown_small<T>, dim3(1), dim3(1024), 0, s, LV_dev, kfrom, kto, homog);
 }
 
 template <typename T>
@@ -2285,7 +2286,7 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int);            \
-  template void launch_filldown_small<T>(const LevelDev<T>*, int, int, hipStream_t);                       \
+  template void launch_filldown_small<T>(const LevelDev<T>*, int, int, int, hipStream_t);                  \
   template void launch_reset_force<T>(const LevelDev<T>*, int, int, hipStream_t);                         \
   template void launch_sweep_stream<T>(const LevelDev<T>&, const PatchInfo&, const T*, T*, T, int, int, int,     \
                                        hipStream_t);                                                         \

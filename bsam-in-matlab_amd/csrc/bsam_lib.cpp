@@ -1200,15 +1200,16 @@ struct Seq {
       fill(k);
     }
   }
-  // ebTransfer.m:73-79; the steps of the levels with at most
-  // kFilldownSmallCells cells run as one single-workgroup launch
-  void filldown() {
-    const int kb = filldown_cut(h);
+  // ebTransfer.m:73-79 for the levels kk = nlev..klo (restrict kk into kk-1,
+  // fill kk-1); the steps of the levels with at most kFilldownSmallCells cells
+  // run as one single-workgroup launch
+  void filldown(int klo = 2) {
+    const int kb = std::max(filldown_cut(h), klo - 1);
     for (int kk = h->nlev; kk > kb; --kk) {
       restrict_(kk);
       fill(kk - 1);
     }
-    launch_filldown_small<T>(LVd, kb, homog, s);
+    launch_filldown_small<T>(LVd, kb, klo, homog, s);
   }
 };
 
@@ -1591,7 +1592,10 @@ static int enqueue_cycle64(bsam_hier* h, hipStream_t s) {
 // residual (which also writes the next right side).  With a zero initial
 // guess and homogeneous data every ghost is zero, so no initial fill is needed;
 // covered cells of the correction are recomputed by the FP64 filldown, so the
-// FP32 filldown only runs between inner cycles.
+// FP32 filldown only runs between inner cycles.  Levels below the root carry
+// no visible cells and the next cycle's FP32 correction starts from zero, so
+// the FP64 update, filldown and fill stop at the root; the sub-root levels
+// are brought up to date once, after the last cycle (sub_root_filldown).
 static int enqueue_cycle_mixed(bsam_hier* h, hipStream_t s) {
   Seq<float> q = seq32(h, s);
   const int kc = coarse_cut(h, h->opts.coarse_cells);
@@ -1603,12 +1607,19 @@ static int enqueue_cycle_mixed(bsam_hier* h, hipStream_t s) {
     q.vcycle(h->nlev, kc);
     if (it + 1 < inner) q.filldown();
   }
-  for (int k = 0; k < h->nlev; ++k) launch_add_correction(h->d64[k], h->d32[k], s);
+  for (int k = h->root; k <= h->nlev; ++k) launch_add_correction(h->d64[k - 1], h->d32[k - 1], s);
   Seq<double> d = seq64(h, s);
-  d.filldown();
-  for (int k = 1; k <= h->nlev; ++k) d.fill(k);
+  d.filldown(h->root + 1);
+  for (int k = h->root; k <= h->nlev; ++k) d.fill(k);
   enqueue_residual(h, h->rout32, s);
   return 0;
+}
+
+// the full FP64 filldown after the last mixed cycle: levels >= root are
+// rewritten with the values they hold (restriction of an unchanged child), the
+// sub-root levels receive the restriction of the root, as after FP64 cycles
+static void sub_root_filldown(bsam_hier* h) {
+  if (h->root > 1) seq64(h, h->stream).filldown();
 }
 
 static int check_stream_rows(int r) {
@@ -1688,6 +1699,7 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
     ncell = r.ncell;
     if (!std::isfinite(rel)) break;
   }
+  if (mixed && it > 0) sub_root_filldown(h);
   HIPCHK(hipEventRecord(e1, h->stream));
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0;

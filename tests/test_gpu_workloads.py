@@ -107,6 +107,13 @@ def test_starfish_full_size_bit_exact(starfish):
     err = (num / den) ** 0.5
     print(f"mixed: {sm['vcycles']} cycles, rel {sm['finalRel']:.3e}, rel L2 vs FP64 oracle {err:.2e}", flush=True)
     assert sm["finalRel"] <= 1e-10 and err < 1e-10
+    # the cycles stop their FP64 update at the root; after the last one the
+    # sub-root levels hold the restriction of the root, as after FP64 cycles
+    assert H.rootIdx > 1
+    for k in range(1, H.rootIdx):
+        assert H.L(k).np == 1 and H.L(k + 1).np == 1
+        child = G.get_field(k + 1, _lib.BSAM_Q)[0][1:-1, 1:-1]
+        assert np.array_equal(G.get_field(k, _lib.BSAM_Q)[0][1:-1, 1:-1], O.restrict2(child)), k
     G.close()
 
 
