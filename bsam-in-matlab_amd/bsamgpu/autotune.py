@@ -8,7 +8,8 @@ The device path has knobs that change speed but not the converged solution:
   fused into one launch each on levels whose 32x32 tiles fit one round of
   workgroups, also bit-identical; ``tiled``: every pass of a relax call in
   one launch on LDS tiles with sibling ghosts frozen);
-- the arithmetic of the inner V-cycles (``fp64`` or ``mixed``);
+- the arithmetic of the inner V-cycles (``fp64`` or ``mixed``) and, for
+  ``mixed``, how many FP32 V-cycles run per FP64 residual (``inner_cycles``);
 - on levels that run streaming sweeps, the row-block height of a sweep
   launch (``stream_rows``: the tile shape is 124 columns by that many rows,
   one 64-lane wave each), which changes the number of waves per CU and the
@@ -61,7 +62,7 @@ def autotune(G, rel_tol=None, bit_exact_only=True, repeats=3, candidates=None):
         table.append(dict(kw, seconds=sec, vcycles=st["vcycles"], finalRel=st["finalRel"]))
     ok = [r for r in table if r["finalRel"] <= rel_tol]
     best = min(ok or table, key=lambda r: r["seconds"])
-    keys = ("coarse_cells", "smoother", "precision")
+    keys = tuple(k for k in best if k not in ("seconds", "vcycles", "finalRel", "stream_rows"))
     best = dict({k: best[k] for k in keys}, stream_rows=best.get("stream_rows", 0), seconds=best["seconds"])
     if best["smoother"] in ("exact", "sweep") and G.streaming_levels():
         for r in STREAM_ROWS:

@@ -128,6 +128,7 @@ struct bsam_hier {
   int2* qc_items = nullptr;                // (level, item): ring + QC interior, k >= max(2, root+1)
   int nqc_items = 0;
   float** rout32 = nullptr;                // per level FP32 FTrue (defect right side)
+  float** rout32f = nullptr;               // per level FP32 F (the same, one inner cycle)
   double** rout64 = nullptr;               // per level FP64 FTrue (cn right side)
   float* q32all = nullptr;                 // FP32 Q of all levels (contiguous)
   float* qc32all = nullptr;                // FP32 QC of all levels (contiguous)
@@ -1465,6 +1466,8 @@ static int ensure32(bsam_hier* h) {
   launch_fill_value<float>(h->qc32all, nqc, 0.0f, h->stream);
   if ((rc = upload(h, &h->lv32, h->d32))) return rc;
   if ((rc = upload(h, &h->rout32, rout))) return rc;
+  for (int k = 0; k < h->nlev; ++k) rout[k] = h->d32[k].F;
+  if ((rc = upload(h, &h->rout32f, rout))) return rc;
   h->have32 = true;
   HIPCHK(hipStreamSynchronize(h->stream));
   return 0;
@@ -1586,6 +1589,14 @@ static int enqueue_cycle64(bsam_hier* h, hipStream_t s) {
   return 0;
 }
 
+// where the FP64 residual writes the FP32 right side of the correction
+// problem: straight into F for one inner cycle per step (levels >= root are
+// rewritten every step; the FAS load sets the sub-root levels' F before it is
+// read), into FTrue when F must be reset from it before each inner cycle
+static float* const* mixed_rout(const bsam_hier* h) {
+  return h->opts.inner_cycles <= 1 ? h->rout32f : h->rout32;
+}
+
 // one step of the mixed loop: FP32 V-cycles on the correction problem
 // (homogeneous boundary data, zero initial guess, the current FP64 composite
 // residual as right side), FP64 update, filldown, ghost refresh and composite
@@ -1603,7 +1614,7 @@ static int enqueue_cycle_mixed(bsam_hier* h, hipStream_t s) {
   launch_fill_value<float>(h->qc32all, h->nqc32all, 0.0f, s);
   const int inner = std::max(1, h->opts.inner_cycles);
   for (int it = 0; it < inner; ++it) {
-    enqueue_reset_force<float>(h, h->lv32, s);
+    if (inner > 1) enqueue_reset_force<float>(h, h->lv32, s);  // else the residual wrote F itself
     q.vcycle(h->nlev, kc);
     if (it + 1 < inner) q.filldown();
   }
@@ -1611,7 +1622,7 @@ static int enqueue_cycle_mixed(bsam_hier* h, hipStream_t s) {
   Seq<double> d = seq64(h, s);
   d.filldown(h->root + 1);
   for (int k = h->root; k <= h->nlev; ++k) d.fill(k);
-  enqueue_residual(h, h->rout32, s);
+  enqueue_residual(h, mixed_rout(h), s);
   return 0;
 }
 
@@ -1681,7 +1692,7 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
   // ebSolveCycles.m:11-19: fill every level, baseline residual
   Seq<double> d = seq64(h, h->stream);
   for (int k = 1; k <= h->nlev; ++k) d.fill(k);
-  enqueue_residual(h, mixed ? h->rout32 : nullptr, h->stream);
+  enqueue_residual(h, mixed ? mixed_rout(h) : nullptr, h->stream);
   HIPCHK(hipGetLastError());
   bsam_residual res{};
   if (int rc = read_residual(h, &res)) return rc;
