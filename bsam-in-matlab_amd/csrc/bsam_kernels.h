@@ -57,15 +57,19 @@ template <typename T> void launch_ring(const LevelDev<T>& L, const LevelDev<T>& 
 // restrict k -> k-1 + fill k-1 for k = kfrom..2 in one workgroup
 template <typename T>
 void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int homog, hipStream_t s);
-template <typename T> void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s);
+// dup (may be null): a second parent array that receives the restricted values too
+template <typename T>
+void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s, T* dup = nullptr);
 // ebCoarseLoad alone (QC ring as stored) and the fused ring + load + reflux
 template <typename T> void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
 template <typename T> void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
 // prep != 0 (levels with L.arith): also store the parent parts of the face
 // records and push corrected boundary values into sibling ghosts, for a
-// following smoother pass that assembles its ghosts (kInlArith)
+// following smoother pass that assembles its ghosts (kInlArith); dup (may be
+// null): a second array of the level that receives the corrected values too
 template <typename T>
-void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s, int prep = 0);
+void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s, int prep = 0,
+                    T* dup = nullptr);
 template <typename T> void launch_coarsest(const CoarsestDev& C, const LevelDev<T>& L1, int homog, hipStream_t s);
 // V-cycle of levels 1..kc in one workgroup (LV_dev: device array of levels,
 // LV_host: the same descriptors on the host).  When the levels' fields and

@@ -842,7 +842,7 @@ __device__ __forceinline__ void ring_item(const LevelDev<T>& L, const T* __restr
 
 template <typename T>
 __device__ __forceinline__ void restrict_item(const LevelDev<T>& L, T* __restrict__ Pq, int t,
-                                              int qc_only) {
+                                              int qc_only, T* __restrict__ Pdup = nullptr) {
   const FootRec fr = L.foot[t];
   const FootRec& pi = fr;  // patch offsets carried inline
   const int ldq = pi.mx + 2;
@@ -851,6 +851,7 @@ __device__ __forceinline__ void restrict_item(const LevelDev<T>& L, T* __restric
   const T R = T(0.25) * (((Q[q00] + Q[q00 + 1]) + Q[q00 + ldq]) + Q[q00 + ldq + 1]);
   L.QC[pi.qcoff + (fr.ci + 1) + (fr.cj + 1) * (pi.mx / 2 + 2)] = R;
   if (!qc_only) Pq[fr.pq] = R;
+  if (Pdup) Pdup[fr.pq] = R;  // the parent's second array (streaming sweeps)
 }
 
 // L(u) at one cell of a padded array (ebKernels.m:13-25)
@@ -985,7 +986,7 @@ __device__ __forceinline__ void load_item(const LevelDev<T>& L, const LevelDev<T
 
 template <typename T>
 __device__ __forceinline__ void correct_item(const LevelDev<T>& L, const T* __restrict__ Pq, int tile,
-                                             int lane, int init, int prep = 0) {
+                                             int lane, int init, int prep = 0, T* __restrict__ Qdup = nullptr) {
   const TileRec& tr = L.tiles[tile];
   const PatchInfo& pi = tr.pi;
   const int mx = pi.mx, my = pi.my;
@@ -1028,6 +1029,7 @@ __device__ __forceinline__ void correct_item(const LevelDev<T>& L, const T* __re
   const int qc = pi.qoff + (i + 1) + (j + 1) * (mx + 2);
   const T y = init ? p : L.Q[qc] + p;
   L.Q[qc] = y;
+  if (Qdup) Qdup[qc] = y;  // the level's second array (streaming sweeps)
   if (pw >= 0) L.Q[pw] = y;
   if (pe >= 0) L.Q[pe] = y;
   if (ps >= 0) L.Q[ps] = y;
@@ -1079,9 +1081,10 @@ __global__ void __launch_bounds__(kTile) k_ring(LevelDev<T> L, const T* __restri
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_restrict(LevelDev<T> L, T* __restrict__ Pq, int qc_only) {
+__global__ void __launch_bounds__(kTile) k_restrict(LevelDev<T> L, T* __restrict__ Pq, int qc_only,
+                                                    T* __restrict__ Pdup) {
   const int t = blockIdx.x * kTile + threadIdx.x;
-  if (t < L.nfoot) restrict_item<T>(L, Pq, t, qc_only);
+  if (t < L.nfoot) restrict_item<T>(L, Pq, t, qc_only, Pdup);
 }
 
 template <typename T>
@@ -1103,8 +1106,9 @@ __global__ void __launch_bounds__(kTile) k_load(LevelDev<T> L, LevelDev<T> P, in
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_correct(LevelDev<T> L, const T* __restrict__ Pq, int init, int prep) {
-  correct_item<T>(L, Pq, blockIdx.x, threadIdx.x, init, prep);
+__global__ void __launch_bounds__(kTile) k_correct(LevelDev<T> L, const T* __restrict__ Pq, int init, int prep,
+                                                   T* __restrict__ Qdup) {
+  correct_item<T>(L, Pq, blockIdx.x, threadIdx.x, init, prep, Qdup);
 }
 
 // ---------------------------------------------------------------- coarsest
@@ -1944,9 +1948,9 @@ void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int ho
 }
 
 template <typename T>
-void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s) {
+void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s, T* dup) {
   if (!L.nfoot) return;
-  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(L.nfoot)), dim3(kTile), 0, s, L, parent.Q, qc_only);
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(L.nfoot)), dim3(kTile), 0, s, L, parent.Q, qc_only, dup);
 }
 
 template <typename T>
@@ -1964,10 +1968,10 @@ void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr,
 }
 
 template <typename T>
-void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s, int prep) {
+void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, hipStream_t s, int prep, T* dup) {
   if (!L.ntile) return;
   hipLaunchKernelGGL(k_correct<T>, dim3(L.ntile), dim3(kTile), 0, s, L, (const T*)parent.Q, init,
-                     prep && L.arith && L.aux ? 1 : 0);
+                     prep && L.arith && L.aux ? 1 : 0, dup);
 }
 
 template <typename T>
@@ -2282,10 +2286,10 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_relax_tiled<T>(const LevelDev<T>&, const LevelDev<T>*, int, T, int, hipStream_t); \
   template void launch_fill<T>(const LevelDev<T>&, const LevelDev<T>*, int, hipStream_t);               \
   template void launch_ring<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                    \
-  template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);           \
+  template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, T*);       \
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
-  template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int);            \
+  template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, T*);        \
   template void launch_filldown_small<T>(const LevelDev<T>*, int, int, int, hipStream_t);                  \
   template void launch_reset_force<T>(const LevelDev<T>*, int, int, hipStream_t);                         \
   template void launch_sweep_stream<T>(const LevelDev<T>&, const PatchInfo&, const T*, T*, T, int, int, int,     \

@@ -1090,6 +1090,8 @@ struct Seq {
   // cf_current: the level was filled after its parent last changed, so the
   // parent parts that fill stored are those of the current parent
   void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false) {
+    const bool qs_current = qs_ready == k;
+    qs_ready = 0;
     if (nsweeps > 0 && h->opts.smoother == 1 && tiled_fits<T>(lev(k))) {
       // tiled smoother: pass 1 reads the materialised ghost frame
       if (first_inline) fill(k);
@@ -1117,7 +1119,8 @@ struct Seq {
       const LevelDev<T>& Lk = lev(k);
       const int odd = nsweeps & 1;
       // (a failure is reported by the sequence's hipGetLastError check)
-      if (odd) (void)hipMemcpyAsync(Lk.Qs, Lk.Q, (size_t)Lk.nq * sizeof(T), hipMemcpyDeviceToDevice, s);
+      if (odd && !qs_current)
+        (void)hipMemcpyAsync(Lk.Qs, Lk.Q, (size_t)Lk.nq * sizeof(T), hipMemcpyDeviceToDevice, s);
       T* buf[2] = {odd ? Lk.Qs : Lk.Q, odd ? Lk.Q : Lk.Qs};
       for (int it = 0; it < nsweeps; ++it)
         launch_sweep_stream<T>(Lk, h->H[k - 1].pi[0], buf[it & 1], buf[(it + 1) & 1], (T)h->opts.omega, homog,
@@ -1155,6 +1158,15 @@ struct Seq {
     if (nsweeps > 0) fill(k);
   }
   void restrict_(int k) { launch_restrict<T>(lev(k), lev(k - 1), 0, s); }
+  // A level whose odd sweep count starts from a copy of Q in Qs gets the copy
+  // from the launch that last wrote its interior instead: the restriction
+  // into a level the child covers entirely (below the root), or the
+  // correction (a one-patch level, every cell).  qs_ready names that level.
+  int qs_ready = 0;
+  bool odd_stream(int k, int nsweeps) const {  // relax(k, nsweeps) takes the streaming path and copies
+    return (nsweeps & 1) && L[k - 1].stream && h->opts.smoother != 3 &&
+           !(h->opts.smoother == 1 && tiled_fits<T>(L[k - 1]));
+  }
   void ring(int k) { launch_ring<T>(lev(k), lev(k - 1), s); }
   void coarse_load(int k) { launch_coarse_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s); }
   void correct(int k, bool prep = false) { launch_correct<T>(lev(k), lev(k - 1), 0, s, prep ? 1 : 0); }
@@ -1189,12 +1201,22 @@ struct Seq {
     }
     // below the top level the fill in the previous step is current
     relax(k, h->opts.npre, 0, k < h->nlev);
-    restrict_(k);
+    if (k <= h->root && k - 1 > kc && odd_stream(k - 1, h->opts.npre)) {
+      launch_restrict<T>(lev(k), lev(k - 1), 0, s, lev(k - 1).Qs);
+      qs_ready = k - 1;
+    } else {
+      restrict_(k);
+    }
     fill(k - 1);
     launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s);
     vcycle(k - 1, kc);
     const bool prep = h->opts.npost > 0 && lev(k).arith && lev(k).aux && h->opts.smoother != 1;
-    correct(k, prep);
+    if (odd_stream(k, h->opts.npost)) {
+      launch_correct<T>(lev(k), lev(k - 1), 0, s, prep ? 1 : 0, lev(k).Qs);
+      qs_ready = k;
+    } else {
+      correct(k, prep);
+    }
     if (h->opts.npost > 0) {
       relax(k, h->opts.npost, prep ? 2 : 1);
     } else {

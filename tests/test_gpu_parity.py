@@ -253,13 +253,15 @@ def test_errors_are_reported():
     assert "eb_make_level:D" in str(ei.value)
 
 
-@pytest.mark.parametrize("omega", [1.0, 1.3])
-def test_streaming_sweeps_on_a_uniform_level(omega):
+@pytest.mark.parametrize("omega,cells", [(1.0, [2048, 1024]), (1.3, [2048, 1024]), (1.0, [4096, 2048])])
+def test_streaming_sweeps_on_a_uniform_level(omega, cells):
     """A one-patch level with physical sides on every face and at least 2^20
     cells runs whole red+black sweeps as streaming launches by default: the
     solve equals the colour-pass solve ('passes') bit for bit (inhomogeneous
     Dirichlet and Neumann sides, variable D and C, a 2048 x 1024 finest level:
-    16 full strips and a partial one, several row blocks), in FP64 and in the
+    16 full strips and a partial one, several row blocks; at 4096 x 2048 the
+    first sub-root level streams too, and its odd sweep counts take their
+    copy from the restriction and the correction), in FP64 and in the
     FP32-inner form."""
     from bsamgpu import Options, Problem, hierarchy
 
@@ -269,9 +271,10 @@ def test_streaming_sweeps_on_a_uniform_level(omega):
     bc = {"west": ("dirichlet", lambda x, y: 1 + y), "east": ("neumann", lambda x, y: 0.5 * y),
           "south": ("dirichlet", lambda x, y: x * x), "north": ("neumann", lambda x, y: 0 * x - 1)}
     H = hierarchy.setup(Problem([0, 1, 0, 1], D, C, f, bc),
-                        Options(BaseCells=[2048, 1024], MaxLevels=1, Omega=omega, RelTol=1e-10, MaxVCycles=4,
+                        Options(BaseCells=cells, MaxLevels=1, Omega=omega, RelTol=1e-10, MaxVCycles=4,
                                 Verbose=0))
     G = GpuHierarchy(H)
+    streams = [H.nlev] if cells[0] == 2048 else [H.nlev - 1, H.nlev]
     for prec in ("fp64", "mixed"):
         out = {}
         for sm in ("passes", "exact"):
@@ -287,7 +290,7 @@ def test_streaming_sweeps_on_a_uniform_level(omega):
             st = G.solve_cycles(smoother="exact", precision=prec, stream_rows=rows)
             assert st["resHist"] == out["passes"][0], rows
             assert all(np.array_equal(a, b) for a, b in zip(G.get_field(H.nlev, _lib.BSAM_Q), out["passes"][1]))
-    assert G.streaming_levels() == [H.nlev]
+    assert G.streaming_levels() == streams
     with pytest.raises(Exception) as ei:
         G.solve_cycles(stream_rows=12)
     assert "stream_rows" in str(ei.value)
