@@ -177,6 +177,31 @@ def bench_large_smoother(n, launches=40):
     return out
 
 
+def bench_uniform_solves(n, repeats=3):
+    """Only the whole solves of the uniform n x n level (the problem of
+    bench_large_smoother): FP64 and FP32-inner with two FP32 cycles per FP64
+    residual, `repeats` of each after a warm-up (for kernel traces)."""
+    from bsamgpu import Options, Problem, hierarchy
+    from bsamgpu.device import GpuHierarchy
+
+    prob = Problem([0, 1, 0, 1], 1, 1, 1, ("dirichlet", 0))
+    H = hierarchy.setup(prob, Options(BaseCells=n, MaxLevels=1, Verbose=0, RelTol=1e-8))
+    G = GpuHierarchy(H)
+    out = {"cells": n * n}
+    for name, kw in (("fp64", dict(precision="fp64")), ("mixed", dict(precision="mixed", inner_cycles=2))):
+        G.reset()
+        G.solve_cycles(coarse_cells=-1, **kw)
+        ts = []
+        for _ in range(repeats):
+            G.reset()
+            t = time.perf_counter()
+            st = G.solve_cycles(coarse_cells=-1, **kw)
+            ts.append(time.perf_counter() - t)
+        out[name] = {"ms": round(min(ts) * 1e3, 3), "vcycles": st["vcycles"], "final_rel": st["finalRel"]}
+    G.close()
+    return out
+
+
 def bench_parabolic():
     """Heat BDF2 (BaseCells 64, MaxLevels 4, Dt 0.002, T 0.1, RegridEvery 5,
     RelTol 1e-12 per step): wall time of ebp_solve with the history resident,
@@ -341,7 +366,7 @@ def main():
     ap.add_argument("--no-varcoef", action="store_true", help="skip the refined variable-coefficient workload")
     ap.add_argument("--serve-requests", type=int, default=128)
     ap.add_argument("--serve-rate", type=float, default=100.0)
-    ap.add_argument("--only", choices=["parabolic", "serving", "varcoef", "uniform"],
+    ap.add_argument("--only", choices=["parabolic", "serving", "varcoef", "uniform", "uniform_solve"],
                     help="run one secondary workload and print it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain device copy of MIB MiB 20 times (scale for the PMC byte counters)")
@@ -378,6 +403,9 @@ def main():
         return
     if args.only == "uniform":
         print(json.dumps(bench_large_smoother(args.large_smoother or 4096)), flush=True)
+        return
+    if args.only == "uniform_solve":
+        print(json.dumps(bench_uniform_solves(args.large_smoother or 4096)), flush=True)
         return
 
     log("building starfish hierarchy on the host")
