@@ -146,8 +146,7 @@ struct bsam_hier {
   long long ncell_all = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  int graph_key = -1;
-  double graph_omega = 1.0;
+  std::vector<double> graph_key;          // the options the captured cycle was built with
   ~bsam_hier() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1669,11 +1668,18 @@ static int run_cycle(bsam_hier* h, bool mixed) {
     HIPCHK(hipGetLastError());
     return 0;
   }
-  const int key = (mixed ? 1 : 0) | (h->opts.inner_cycles << 1) | (h->opts.npre << 8) | (h->opts.npost << 12) |
-                  (h->opts.mass_correction << 16) | (h->opts.omega == 1.0 ? 0 : (1 << 17)) |
-                  (coarse_cut(h, h->opts.coarse_cells) << 18) | (h->opts.smoother << 26) |
-                  ((h->opts.stream_rows ? __builtin_ctz(h->opts.stream_rows) - 2 : 0) << 28);
-  if (!h->gexec || h->graph_key != key || h->graph_omega != h->opts.omega) {
+  // every option the enqueued sequence depends on (omega is baked into the
+  // kernel arguments)
+  const std::vector<double> key = {mixed ? 1.0 : 0.0,
+                                   (double)h->opts.inner_cycles,
+                                   (double)h->opts.npre,
+                                   (double)h->opts.npost,
+                                   (double)h->opts.mass_correction,
+                                   h->opts.omega,
+                                   (double)coarse_cut(h, h->opts.coarse_cells),
+                                   (double)h->opts.smoother,
+                                   (double)h->opts.stream_rows};
+  if (!h->gexec || h->graph_key != key) {
     if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
     if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -1685,7 +1691,6 @@ static int run_cycle(bsam_hier* h, bool mixed) {
     h->graph = g;
     HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
     h->graph_key = key;
-    h->graph_omega = h->opts.omega;
   }
   HIPCHK(hipGraphLaunch(h->gexec, s));
   return 0;
@@ -2142,7 +2147,7 @@ int bsam_update_coeffs(bsam_hier* h, const bsam_hier_desc* d) {
   // captured cycles hold the coarsest order and buffers by value
   if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
   if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
-  h->graph_key = -1;
+  h->graph_key.clear();
   if (M != h->cst.M) {
     if (int rc = upload(h, &h->cst.Ainv, Ainv)) return rc;
     h->cst.M = M;
