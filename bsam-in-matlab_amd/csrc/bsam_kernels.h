@@ -29,6 +29,7 @@ void launch_relax_sweep(const LevelDev<T>& L, const LevelDev<T>* parent, const T
 // (the block height: rows_req when positive, else the largest power of two
 // from kStreamRowsMin up to StreamRows<T> that keeps about 8 waves per CU)
 constexpr int kStreamCols = 124, kStreamRowsMin = 8;
+constexpr int kXcds = 8;  // MI355X: workgroups are dealt to the 8 XCDs in turn
 template <typename T> struct StreamRows { static constexpr int value = sizeof(T) == 8 ? 64 : 32; };
 template <typename T>
 void launch_sweep_stream(const LevelDev<T>& L, const PatchInfo& P, const T* Qin, T* Qout, T omega, int homog,

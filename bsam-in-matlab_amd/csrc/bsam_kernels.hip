@@ -662,7 +662,13 @@ __global__ void __launch_bounds__(64) k_sweep_stream(LevelDev<T> L, PatchInfo P,
                                                      T* __restrict__ Qout, T omega, int homog, int kRows) {
   const int mx = P.mx, my = P.my, ldq = mx + 2;
   const int nstrip = (mx + kStreamCols - 1) / kStreamCols;
-  const int strip = blockIdx.x % nstrip, blk = blockIdx.x / nstrip;
+  // workgroups go to the 8 XCDs in turn: XCD x takes the x-th contiguous
+  // range of (row block, strip) items, so the strips of a row block run side
+  // by side on one XCD and share the cache lines that straddle their edges
+  const int total = nstrip * ((my + kRows - 1) / kRows), per = (total + kXcds - 1) / kXcds;
+  const int item = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if (item >= total) return;
+  const int strip = item % nstrip, blk = item / nstrip;
   const int j0 = blk * kRows, j1 = min(j0 + kRows, my);
   const int own0 = strip * kStreamCols, own1 = min(own0 + kStreamCols, mx);
   const int A = own0 - 2 + 2 * (int)threadIdx.x;  // the lane's columns A, A+1
@@ -2077,7 +2083,8 @@ void launch_sweep_stream(const LevelDev<T>& L, const PatchInfo& P, const T* Qin,
   const int rows = rows_req > 0 ? rows_req : stream_rows<T>(P, ncu);
   const int nstrip = (P.mx + kStreamCols - 1) / kStreamCols;
   const int nblk_rows = (P.my + rows - 1) / rows;
-  hipLaunchKernelGGL(k_sweep_stream<T>, dim3(nstrip * nblk_rows), dim3(64), 0, s, L, P, Qin, Qout, omega, homog,
+  const int per = (nstrip * nblk_rows + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL(k_sweep_stream<T>, dim3(per * kXcds), dim3(64), 0, s, L, P, Qin, Qout, omega, homog,
                      rows);
 }
 
