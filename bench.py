@@ -112,7 +112,7 @@ def cpu_baseline(H):
             "seconds_per_solve": dt, "seconds_per_vcycle": dt / max(st["vcycles"], 1)}
 
 
-def bench_large_smoother(n, launches=40):
+def bench_large_smoother(n, launches=40, solves=True):
     """The smoother on one uniform n x n level (D = C = 1), large enough to be
     HBM-bound rather than launch-bound: the colour pass (half a sweep per
     launch) and the streaming sweep (a whole red+black sweep per launch),
@@ -149,6 +149,9 @@ def bench_large_smoother(n, launches=40):
             rows[str(r)][name] = {"us_per_launch": round(ms * 1e3, 2),
                                   "frac": round(G.smoother_bytes(k, b) / ms / 1e6 / HBM_PEAK_GBS, 4)}
     out["streaming_sweep_by_rows"] = rows
+    if not solves:
+        G.close()
+        return out
     # whole solves from zero to RelTol, the option search over precision,
     # FP32 cycles per FP64 residual and row-block height (coarse cycle off:
     # one level of 2^24 cells and its sub-root levels); the fastest mixed
@@ -402,7 +405,9 @@ def main():
         print(json.dumps(bench_varcoef_refined()), flush=True)
         return
     if args.only == "uniform":
-        print(json.dumps(bench_large_smoother(args.large_smoother or 4096)), flush=True)
+        # --no-autotune: the smoother launches only, no whole solves
+        print(json.dumps(bench_large_smoother(args.large_smoother or 4096, solves=not args.no_autotune)),
+              flush=True)
         return
     if args.only == "uniform_solve":
         print(json.dumps(bench_uniform_solves(args.large_smoother or 4096)), flush=True)

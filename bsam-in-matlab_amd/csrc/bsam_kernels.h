@@ -27,7 +27,8 @@ void launch_relax_sweep(const LevelDev<T>& L, const LevelDev<T>* parent, const T
 // side, Qin -> Qout: 64-lane waves on strips of kStreamCols written columns
 // (two per lane, one halo lane at each end) and blocks of StreamRows<T> rows
 // (the block height: rows_req when positive, else the largest power of two
-// from kStreamRowsMin up to StreamRows<T> that keeps about 8 waves per CU)
+// from kStreamRowsMin up to StreamRows<T> that keeps about 32 waves per CU in
+// FP64, 8 in FP32)
 constexpr int kStreamCols = 124, kStreamRowsMin = 8;
 constexpr int kXcds = 8;  // MI355X: workgroups are dealt to the 8 XCDs in turn
 template <typename T> struct StreamRows { static constexpr int value = sizeof(T) == 8 ? 64 : 32; };

@@ -654,7 +654,8 @@ __device__ __forceinline__ T lane_down(T v) {  // value of lane l+1
 // in the step that loads it; a row's two columns are stored together.  The
 // strip's first and last lanes are a halo (their black values would need
 // lanes outside the wave and are not stored) and the red rows just outside
-// the block are recomputed.  Every update is the colour pass's expression
+// the block are recomputed.  Even row blocks march up and odd ones down, so
+// that neighbouring blocks read their shared edge rows at about the same time.  Every update is the colour pass's expression
 // (relax_combine) with the partial physical ghosts of ebKernels.m:51-70, so
 // the sweep equals two colour passes bit for bit.
 template <typename T>
@@ -675,8 +676,54 @@ __global__ void __launch_bounds__(64) k_sweep_stream(LevelDev<T> L, PatchInfo P,
   const bool owned = A >= own0 && A < own1;       // both columns or neither (own0, own1, A even)
   const T* __restrict__ Q = Qin + P.qoff;
   T* __restrict__ Qo = Qout + P.qoff;
-  // rows jr-1, jr, jr+1 (loaded) around the red row jr = j-2, and row j (loading)
   StreamRow<T> r0, r1, r2, r3;
+  if (blk & 1) {
+    // odd row blocks march down: they meet the even blocks above and below at
+    // the start and the end of the launch, when both read the shared edge rows
+    // rows jr+1, jr, jr-1 (loaded) around the red row jr = j+2, and row j (loading)
+    stream_row<T>(L, P, Q, A, j1 + 1, r0);
+    stream_row<T>(L, P, Q, A, j1, r1);
+    stream_row<T>(L, P, Q, A, j1 - 1, r2);
+    T rd1 = T(0), rd2 = T(0);     // red values of this sweep, one and two rows above the red row
+    T pds0 = T(0), pds1 = T(0);   // south faces of the row two above the red row
+    for (int j = j1 - 2; j >= j0 - 3; --j) {
+      stream_row<T>(L, P, Q, A, j, r3);
+      const int jr = j + 2;
+      const int k = ((A + jr + P.par0) & 1) == 0 ? 0 : 1;  // red column of row jr (uniform)
+      const int c = A + k;
+      const bool col = c >= 0 && c < mx;
+      const T qo = k ? r1.q0 : r1.q1;
+      const T from_up = lane_up(r1.q1), from_down = lane_down(r1.q0);
+      T red = T(0);
+      if (col && jr >= max(j0 - 1, 0) && jr <= min(j1, my - 1)) {
+        const T e = k ? from_down : qo, w = k ? qo : from_up;
+        const T n = k ? r0.q1 : r0.q0, s = k ? r2.q1 : r2.q0, self = k ? r1.q1 : r1.q0;
+        red = stream_cell<T>(L, P, c, jr, r1, k, k ? r0.ds1 : r0.ds0, e, w, n, s, self, omega, homog);
+      }
+      // black cell (c, jr+1): red of row jr+1 along the row, red of rows jr+2 and jr
+      const T rup = lane_up(rd1), rdown = lane_down(rd1);
+      const int jb = jr + 1;
+      if (col && jb >= j0 && jb < j1) {
+        const T e = k ? rdown : rd1, w = k ? rd1 : rup;
+        const T self = k ? r0.q1 : r0.q0;
+        const T v = stream_cell<T>(L, P, c, jb, r0, k, k ? pds1 : pds0, e, w, rd2, red, self, omega, homog);
+        if (owned) {
+          const int qa = (A + 1) + (jb + 1) * ldq;
+          Qo[qa + k] = v;
+          Qo[qa + 1 - k] = rd1;
+        }
+      }
+      rd2 = rd1;
+      rd1 = red;
+      pds0 = r0.ds0;
+      pds1 = r0.ds1;
+      r0 = r1;
+      r1 = r2;
+      r2 = r3;
+    }
+    return;
+  }
+  // rows jr-1, jr, jr+1 (loaded) around the red row jr = j-2, and row j (loading)
   stream_row<T>(L, P, Q, A, j0 - 2, r0);
   stream_row<T>(L, P, Q, A, j0 - 1, r1);
   stream_row<T>(L, P, Q, A, j0, r2);
@@ -2068,10 +2115,13 @@ void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rota
   hipLaunchKernelGGL(k_history, dim3(L.ntile), dim3(kTile), 0, s, L, Uo, Uoo, rotate);
 }
 
+// about 32 waves per CU in FP64 and 8 in FP32 (the fastest heights measured
+// on a 4096 x 4096 level once neighbouring row blocks share their edge rows)
 template <typename T>
 int stream_rows(const PatchInfo& P, int ncu) {
   const int nstrip = (P.mx + kStreamCols - 1) / kStreamCols;
-  const long long want = (long long)P.my * nstrip / (8LL * std::max(ncu, 1));  // about 8 waves per CU
+  const long long waves = sizeof(T) == 8 ? 32 : 8;
+  const long long want = (long long)P.my * nstrip / (waves * std::max(ncu, 1));
   int rows = kStreamRowsMin;
   while (rows * 2 <= want && rows * 2 <= StreamRows<T>::value) rows *= 2;
   return rows;

@@ -111,9 +111,9 @@ typedef struct {
                           rows per red+black sweep), also bit-identical */
   int stream_rows;     /* row-block height of a streaming sweep launch (one 64-lane
                           wave per block of 124 columns x stream_rows rows):
-                          0 = automatic (about 8 waves per CU, at most 64 rows in
-                          FP64 and 32 in FP32), else 8, 16, 32 or 64; the result
-                          is the same for every value */
+                          0 = automatic (about 32 waves per CU in FP64 and 8 in
+                          FP32, at most 64 and 32 rows), else 8, 16, 32 or 64;
+                          the result is the same for every value */
 } bsam_solve_opts;
 
 /* One solve stage (EllipticBSAM/src/ebSolveCycles.m:58-70) plus device timing. */
