@@ -389,3 +389,52 @@ def cluster_native(T, allowed, min_width, fill_ratio):
     for _ in range(nd.value):
         warnings.warn("eb_tag_cluster:dropthin: dropping an ungrowable sliver cluster box")
     return buf[:4 * nb.value].reshape(-1, 4).tolist()
+
+
+def cluster_per_parent(T, allowed, parents, min_width, fill_ratio):
+    """ebtTagCluster.m:93-128 (tree variant): br_cluster -> clip_allowed ->
+    grow_min -> merge_or_drop_thin inside every parent box [ilo ihi jlo jhi]
+    (1-based) on that parent's own tags and allowed region; rows
+    [parent, ilo, ihi, jlo, jhi] in level coordinates."""
+    out = []
+    for q, (ilo, ihi, jlo, jhi) in enumerate(parents, 1):
+        Tq = T[ilo - 1:ihi, jlo - 1:jhi]
+        if not Tq.any():
+            continue
+        Aq = allowed[ilo - 1:ihi, jlo - 1:jhi]
+        b = br_cluster(Tq, min_width, fill_ratio)
+        b = clip_allowed(b, Tq, Aq)
+        b = grow_min(b, Aq, min_width, Tq.shape)
+        b = merge_or_drop_thin(b, Aq)
+        out += [[q, int(bb[0]) + ilo - 1, int(bb[1]) + ilo - 1, int(bb[2]) + jlo - 1, int(bb[3]) + jlo - 1]
+                for bb in b]
+    return out
+
+
+def cluster_per_parent_native(T, allowed, parents, min_width, fill_ratio):
+    """The library's per-parent clustering (bsam_cluster_tags_per_parent)."""
+    import ctypes
+
+    from . import _lib
+
+    lib = _lib.load()
+    t = np.asfortranarray(T, dtype=np.uint8)
+    a = np.asfortranarray(allowed, dtype=np.uint8)
+    par = np.ascontiguousarray(np.asarray(parents, dtype=np.int32).reshape(-1))
+    n1, n2 = t.shape
+    nb, nd = ctypes.c_int(), ctypes.c_int()
+    cap = 64
+    while True:
+        buf = np.zeros(5 * cap, dtype=np.int32)
+        rc = lib.bsam_cluster_tags_per_parent(t.ctypes.data, a.ctypes.data, n1, n2, par.ctypes.data_as(_lib.P_int),
+                                              len(parents), int(min_width), float(fill_ratio),
+                                              buf.ctypes.data_as(_lib.P_int), cap, ctypes.byref(nb),
+                                              ctypes.byref(nd))
+        if rc != 0 and nb.value > cap:
+            cap = nb.value
+            continue
+        _lib.check(rc, "bsam_cluster_tags_per_parent")
+        break
+    for _ in range(nd.value):
+        warnings.warn("ebt_tag_cluster:dropthin: dropping an ungrowable sliver cluster box")
+    return buf[:5 * nb.value].reshape(-1, 5).tolist()

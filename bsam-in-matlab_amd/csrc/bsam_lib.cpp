@@ -1878,6 +1878,53 @@ int bsam_cluster_tags(const unsigned char* tags, const unsigned char* allowed, i
   return 0;
 }
 
+int bsam_cluster_tags_per_parent(const unsigned char* tags, const unsigned char* allowed, int n1, int n2,
+                                 const int* parents, int np, int min_width, double fill_ratio, int* boxes, int cap,
+                                 int* nboxes, int* ndropped) {
+  if (!tags || !allowed || n1 < 1 || n2 < 1 || np < 0 || (np > 0 && !parents) || cap < 0 ||
+      (cap > 0 && !boxes) || !nboxes)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: bad tag map, parent list or box buffer");
+  if (min_width < 1) return fail(BSAM_E_ARG, "eb_options:minbox: MinBoxWidth must be a positive integer");
+  int total = 0, dropped = 0;
+  std::vector<unsigned char> tq, aq;
+  std::vector<int> bq;
+  for (int q = 0; q < np; ++q) {
+    const int ilo = parents[4 * q], ihi = parents[4 * q + 1], jlo = parents[4 * q + 2], jhi = parents[4 * q + 3];
+    if (ilo < 1 || ihi > n1 || jlo < 1 || jhi > n2 || ilo > ihi || jlo > jhi)
+      return fail(BSAM_E_ARG, "bsamgpu:arg: parent box %d lies outside the %d x %d level", q + 1, n1, n2);
+    const int m1 = ihi - ilo + 1, m2 = jhi - jlo + 1;
+    // this parent's own tags and allowed region, in its local index space
+    tq.assign((size_t)m1 * m2, 0);
+    aq.assign((size_t)m1 * m2, 0);
+    bool any = false;
+    for (int j = 0; j < m2; ++j)
+      for (int i = 0; i < m1; ++i) {
+        const size_t g = (size_t)(ilo - 1 + i) + (size_t)(jlo - 1 + j) * n1, l = (size_t)i + (size_t)j * m1;
+        tq[l] = tags[g] != 0;
+        aq[l] = allowed[g] != 0;
+        any = any || tq[l];
+      }
+    if (!any) continue;
+    bq.assign(4 * tq.size(), 0);  // every box holds a tag: at most m1*m2 boxes
+    int dq = 0;
+    const int nq = bsam_cluster_impl(tq.data(), aq.data(), m1, m2, min_width, fill_ratio, bq.data(),
+                                     (int)tq.size(), &dq);
+    dropped += dq;
+    for (int b = 0; b < nq; ++b, ++total)
+      if (total < cap) {
+        boxes[5 * total] = q + 1;
+        boxes[5 * total + 1] = bq[4 * b] + ilo - 1;
+        boxes[5 * total + 2] = bq[4 * b + 1] + ilo - 1;
+        boxes[5 * total + 3] = bq[4 * b + 2] + jlo - 1;
+        boxes[5 * total + 4] = bq[4 * b + 3] + jlo - 1;
+      }
+  }
+  *nboxes = total;
+  if (ndropped) *ndropped = dropped;
+  if (total > cap) return fail(BSAM_E_ARG, "bsamgpu:arg: %d boxes do not fit the buffer of %d", total, cap);
+  return 0;
+}
+
 long long bsam_level_field_size(const bsam_hier* h, int k, int field) {
   if (!h || k < 1 || k > h->nlev) return -1;
   const HLevel& L = h->H[k - 1];

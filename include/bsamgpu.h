@@ -200,6 +200,17 @@ int bsam_tag_level(bsam_hier *h, int k, int indicator, double threshold, int buf
    *ndropped (may be NULL): slivers dropped.  Host code: needs no device. */
 int bsam_cluster_tags(const unsigned char *tags, const unsigned char *allowed, int n1, int n2, int min_width,
                       double fill_ratio, int *boxes, int cap, int *nboxes, int *ndropped);
+/* The tree variant's clustering (EllipticBSAMTree/src/ebtTagCluster.m:93-128):
+ * the same steps as bsam_cluster_tags, run inside every parent patch on that
+ * parent's own tags and allowed region, so every box lies in exactly one
+ * parent.  tags/allowed: the level-wide maps (n1 x n2, column-major, after the
+ * global buffer and nesting clip); parents: np boxes [ilo ihi jlo jhi] (1-based,
+ * level index space).  Writes min(count, cap) rows [parent ilo ihi jlo jhi]
+ * (parent 1-based, box in level coordinates), parent by parent; *nboxes =
+ * count, *ndropped = slivers dropped over all parents. */
+int bsam_cluster_tags_per_parent(const unsigned char *tags, const unsigned char *allowed, int n1, int n2,
+                                 const int *parents, int np, int min_width, double fill_ratio, int *boxes,
+                                 int cap, int *nboxes, int *ndropped);
 
 /* Diffuse-domain geometry on device `device`: for npts points (x, y), the
    signed distance d (positive inside) to the closed, radially ordered polygon
