@@ -21,6 +21,7 @@
  *                                                         before ebpSetTime moves to t1)
  *   mass = bsamgpu_mex('parabolic_advance', h)   history rotation and compMass after a 'solve'
  *   U = bsamgpu_mex('get_history', h, which)     0: Uo, 1: Uoo
+ *   boxes = bsamgpu_mex('cluster_per_parent', T, allowed, parentBoxes, minw, fill)   tree variant
  *   bsamgpu_mex('destroy', h)
  *
  * Library errors are raised as MATLAB errors with identifier
@@ -351,6 +352,42 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
     double* out = mxGetPr(plhs[0]);
     for (int b = 0; b < nb; ++b)
       for (int c = 0; c < 4; ++c) out[b + (size_t)c * nb] = buf[4 * b + c];
+    mxFree(buf);
+    return;
+  }
+  if (!strcmp(cmd, "cluster_per_parent")) {
+    /* boxes = bsamgpu_mex('cluster_per_parent', T, allowed, parentBoxes, MinBoxWidth, MinFillRatio):
+       the per-parent clustering of EllipticBSAMTree/src/ebtTagCluster.m:93-128;
+       parentBoxes np x 4 [ilo ihi jlo jhi], result nb x 5 [parentIdx ilo ihi jlo jhi] */
+    if (nrhs < 6 || !mxIsLogical(prhs[1]) || !mxIsLogical(prhs[2]) || !mxIsDouble(prhs[3]) ||
+        (mxGetNumberOfElements(prhs[3]) > 0 && mxGetN(prhs[3]) != 4))
+      mexErrMsgIdAndTxt("bsamgpu:arg", "cluster_per_parent(T, allowed, parentBoxes (np x 4), minw, fill)");
+    const int n1 = (int)mxGetM(prhs[1]), n2 = (int)mxGetN(prhs[1]);
+    if ((int)mxGetM(prhs[2]) != n1 || (int)mxGetN(prhs[2]) != n2)
+      mexErrMsgIdAndTxt("bsamgpu:arg", "cluster_per_parent: T and allowed differ in size");
+    const int np = (int)mxGetM(prhs[3]);
+    const double* pb = mxGetPr(prhs[3]);
+    int* par = (int*)mxCalloc((size_t)(np > 0 ? np : 1) * 4, sizeof(int));
+    for (int q = 0; q < np; ++q)
+      for (int c = 0; c < 4; ++c) par[4 * q + c] = (int)pb[q + (size_t)c * np];
+    int cap = 64, nb = 0, nd = 0, rc;
+    int* buf = NULL;
+    for (;;) {
+      buf = (int*)mxCalloc((size_t)cap * 5, sizeof(int));
+      rc = bsam_cluster_tags_per_parent((const unsigned char*)mxGetLogicals(prhs[1]),
+                                        (const unsigned char*)mxGetLogicals(prhs[2]), n1, n2, par, np,
+                                        (int)mxGetScalar(prhs[4]), mxGetScalar(prhs[5]), buf, cap, &nb, &nd);
+      if (rc == 0 || nb <= cap) break;
+      mxFree(buf);
+      cap = nb;
+    }
+    mxFree(par);
+    chk(rc);
+    if (nd > 0) mexWarnMsgIdAndTxt("ebt_tag_cluster:dropthin", "dropping %d ungrowable sliver cluster box(es)", nd);
+    plhs[0] = mxCreateDoubleMatrix((mwSize)nb, 5, mxREAL);
+    double* out = mxGetPr(plhs[0]);
+    for (int b = 0; b < nb; ++b)
+      for (int c = 0; c < 5; ++c) out[b + (size_t)c * nb] = buf[5 * b + c];
     mxFree(buf);
     return;
   }

@@ -81,5 +81,5 @@ def test_gateway_maps_every_scheme():
     src = open(GATEWAY).read()
     for name, code in (("be", "BSAM_BE"), ("bdf2", "BSAM_BDF2"), ("cn", "BSAM_CN")):
         assert f'"{name}") ? {code}' in src
-    for cmd in ("parabolic_rhs", "parabolic_advance", "parabolic_step", "tag", "cluster"):
+    for cmd in ("parabolic_rhs", "parabolic_advance", "parabolic_step", "tag", "cluster", "cluster_per_parent"):
         assert f'"{cmd}"' in src
