@@ -246,6 +246,26 @@ def test_device_cn_published_regressions():
 
 
 @pytest.mark.gpu
+def test_device_cn_mixed_precision():
+    """Crank-Nicolson with FP32 inner cycles (the right side stays FP64): the
+    fixed-patch problem with regrids ends within 1e-10 relative L2 of the FP64
+    run, with the same V-cycle counts and composite mass drift below 2e-10."""
+    out = {}
+    for prec in ("fp64", "mixed"):
+        sol, _ = _gpu(fixed_patch_cn, RegridEvery=2, Precision=prec)
+        H = sol["H"]
+        out[prec] = (sol, np.concatenate([H.L(k).Q[p][1:-1, 1:-1].ravel() for k in range(H.rootIdx, H.nlev + 1)
+                                          for p in range(H.L(k).np)]))
+    (s64, q64), (smx, qmx) = out["fp64"], out["mixed"]
+    err = float(np.linalg.norm(qmx - q64) / np.linalg.norm(q64))
+    m = np.array(smx["mass"])
+    print(f"cn mixed: rel L2 vs fp64 {err:.2e}, cycles {smx['cycles']} vs {s64['cycles']}", flush=True)
+    assert err < 1e-10
+    assert smx["cycles"] == s64["cycles"]
+    assert float(np.max(np.abs(m - m[0]))) < 2e-10
+
+
+@pytest.mark.gpu
 def test_device_history_roundtrip_and_errors():
     pp, opts = heat_readme(TFinal=0.004)
     S = par.ebp_init(pp, opts, 1.0 / 0.002)
