@@ -294,9 +294,27 @@ def bench_varcoef_refined(steps=5, warmup=2):
                                                 "GBps": round(b32 / ms32 / 1e6, 1)}}}
 
 
-def bench_serving(nreq, rate):
+def _served_rel_l2(a, b):
+    """Visible-cell relative L2 difference of two served results on the same
+    hierarchy (None when the hierarchies differ)."""
+    if [list(map(list, a.H.L(k).box)) for k in range(1, a.H.nlev + 1)] != \
+            [list(map(list, b.H.L(k).box)) for k in range(1, b.H.nlev + 1)]:
+        return None
+    num = den = 0.0
+    for k in range(a.H.rootIdx, a.H.nlev + 1):
+        L = a.H.L(k)
+        for p in range(L.np):
+            vis = ~L.covered[p]
+            x = a.Q[k - 1][p][1:-1, 1:-1][vis]
+            y = b.Q[k - 1][p][1:-1, 1:-1][vis]
+            num += float(((x - y) ** 2).sum())
+            den += float((y ** 2).sum())
+    return (num / den) ** 0.5
+
+
+def bench_serving(nreq, rates):
     """Served stream: nreq requests of the seeded fixture with Poisson
-    arrivals at `rate` req/s, continuous batching (FP32-inner and FP64)
+    arrivals at each rate (req/s), continuous batching (FP32-inner and FP64)
     against one-at-a-time plain FP64 ebSolve on the same arrivals."""
     import json as _json
 
@@ -305,47 +323,63 @@ def bench_serving(nreq, rate):
     with open(os.path.join(ROOT, "tests", "golden", "config5_requests.json")) as f:
         fx = _json.load(f)
 
-    def reqs():
+    def reqs(rate):
         return [serving.Request(i, configs.served_problem(fx["requests"][i]),
                                 configs.served_options(fx["requests"][i]), fx["arrivals_at_rate_1"][i] / rate)
                 for i in range(nreq)]
 
-    out = {"requests": nreq, "rate_per_s": rate, "tolerance": 1e-8}
+    out = {"requests": nreq, "tolerance": 1e-8, "rates": {}}
     # warm-up of the batched kernels and the host pool
     srv = serving.Server(slots=256, mem_budget=32 << 30, precision="mixed")
-    srv.run(reqs()[:8], realtime=False)
+    srv.run(reqs(1.0)[:8], realtime=False)
     srv.close()
-    for prec in ("mixed", "fp64"):
-        srv = serving.Server(slots=256, mem_budget=32 << 30, precision=prec)
-        t = time.perf_counter()
-        done = srv.run(reqs(), realtime=True)
-        wall = time.perf_counter() - t
-        srv.close()
-        bad = [r.id for r in done if r.error is not None or r.stages[-1]["finalRel"] > r.opts.RelTol]
-        st = serving.latency_stats(done)
-        out[f"batched_{prec}"] = {"p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
-                                  "throughput_per_s": round(nreq / max(r.t_done for r in done), 2),
-                                  "wall_s": round(wall, 3), "ticks": srv.ticks,
-                                  "mean_batch": round(srv.tick_items / max(srv.ticks, 1), 1),
-                                  "peak_budget_bytes": srv.peak_bytes, "unconverged": bad}
-    # one at a time, FIFO on the same arrivals
+    # one at a time: service times of plain FP64 solves, FIFO on the arrivals
     service = []
     t0 = time.perf_counter()
-    for r in reqs():
+    for r in reqs(1.0):
         t = time.perf_counter()
         serving.serial_solve(r, "fp64")
         service.append(time.perf_counter() - t)
     total = time.perf_counter() - t0
-    fin = 0.0
-    lat = []
-    for r, sv in zip(reqs(), service):
-        fin = max(fin, r.arrival) + sv
-        lat.append(fin - r.arrival)
-    out["one_at_a_time_fp64"] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
-                                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
-                                 "throughput_per_s": round(nreq / total, 2),
-                                 "mean_service_ms": round(1e3 * total / nreq, 2)}
+    log(f"serving: one-at-a-time mean service {1e3 * total / nreq:.2f} ms")
+    for rate in rates:
+        res = {}
+        done = {}
+        for prec in ("mixed", "fp64"):
+            srv = serving.Server(slots=256, mem_budget=32 << 30, precision=prec)
+            t = time.perf_counter()
+            d = srv.run(reqs(rate), realtime=True)
+            wall = time.perf_counter() - t
+            srv.close()
+            done[prec] = {r.id: r for r in d}
+            bad = [r.id for r in d if r.error is not None or r.stages[-1]["finalRel"] > r.opts.RelTol]
+            st = serving.latency_stats(d)
+            res[f"batched_{prec}"] = {"p50_ms": round(st["p50_ms"], 2), "p99_ms": round(st["p99_ms"], 2),
+                                      "throughput_per_s": round(nreq / max(r.t_done for r in d), 2),
+                                      "wall_s": round(wall, 3), "ticks": srv.ticks,
+                                      "mean_batch": round(srv.tick_items / max(srv.ticks, 1), 1),
+                                      "peak_budget_bytes": srv.peak_bytes, "unconverged": bad}
+            log(f"serving at {rate}/s {prec}: {res[f'batched_{prec}']}")
+        errs = [e for e in (_served_rel_l2(done["mixed"][i], done["fp64"][i]) for i in done["fp64"]) if e is not None]
+        res["batched_mixed"]["rel_l2_vs_fp64_max"] = float(max(errs)) if errs else None
+        res["batched_mixed"]["rel_l2_vs_fp64_median"] = float(np.median(errs)) if errs else None
+        res["batched_mixed"]["same_hierarchy_requests"] = len(errs)
+        fin = 0.0
+        lat = []
+        for r, sv in zip(reqs(rate), service):
+            fin = max(fin, r.arrival) + sv
+            lat.append(fin - r.arrival)
+        res["one_at_a_time_fp64"] = {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 2),
+                                     "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                                     "throughput_per_s": round(nreq / fin, 2),
+                                     "mean_service_ms": round(1e3 * total / nreq, 2),
+                                     "utilisation": round(rate * total / nreq, 2)}
+        out["rates"][str(rate)] = res
     return out
+
+
+def _rates(text):
+    return [float(x) for x in str(text).split(",") if x.strip()]
 
 
 def main():
@@ -367,8 +401,8 @@ def main():
                     help="also time the smoother on one uniform N x N level (0: skip)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-varcoef", action="store_true", help="skip the refined variable-coefficient workload")
-    ap.add_argument("--serve-requests", type=int, default=128)
-    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-rates", default="50,400", help="comma-separated Poisson arrival rates (req/s)")
     ap.add_argument("--only", choices=["parabolic", "serving", "varcoef", "uniform", "uniform_solve"],
                     help="run one secondary workload and print it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -399,7 +433,7 @@ def main():
         print(json.dumps(bench_parabolic()), flush=True)
         return
     if args.only == "serving":
-        print(json.dumps(bench_serving(args.serve_requests, args.serve_rate)), flush=True)
+        print(json.dumps(bench_serving(args.serve_requests, _rates(args.serve_rates))), flush=True)
         return
     if args.only == "varcoef":
         print(json.dumps(bench_varcoef_refined()), flush=True)
@@ -504,8 +538,8 @@ def main():
         log(f"parabolic: {parab['fp64']}")
     serv = None
     if not args.no_serving:
-        log(f"serving {args.serve_requests} requests at {args.serve_rate}/s")
-        serv = bench_serving(args.serve_requests, args.serve_rate)
+        log(f"serving {args.serve_requests} requests at {args.serve_rates} req/s")
+        serv = bench_serving(args.serve_requests, _rates(args.serve_rates))
         log(f"serving: {serv}")
 
     base = None

@@ -43,7 +43,7 @@ EXPORTS = [
     "bsam_time_smoother", "bsam_smoother_bytes", "bsam_level_streams", "bsam_update_coeffs", "bsam_parabolic_init",
     "bsam_parabolic_set_history", "bsam_parabolic_get_history", "bsam_parabolic_set_source",
     "bsam_parabolic_rhs", "bsam_parabolic_set_shift", "bsam_parabolic_advance", "bsam_parabolic_step", "bsam_batch_create",
-    "bsam_batch_destroy", "bsam_batch_step", "bsam_tag_level", "bsam_polygon_fields",
+    "bsam_batch_destroy", "bsam_batch_step", "bsam_tag_level", "bsam_tag_level_static", "bsam_polygon_fields",
     "bsam_hier_inherit", "bsam_regrid_level", "bsam_parabolic_uo_from_q", "bsam_cluster_tags", "bsam_cluster_tags_per_parent",
 ]
 
@@ -133,6 +133,7 @@ def load(path=None):
         "bsam_smoother_bytes": (c_double, [vp, c_int, c_int]),
         "bsam_level_streams": (c_int, [vp, c_int]),
         "bsam_tag_level": (c_int, [vp, c_int, c_int, c_double, c_int, vp, vp, P_ll]),
+        "bsam_tag_level_static": (c_int, [vp, c_int, c_int, c_double, c_int, vp, vp, P_ll]),
         "bsam_hier_inherit": (c_int, [vp, vp]),
         "bsam_cluster_tags": (c_int, [vp, vp, c_int, c_int, c_int, c_double, P_int, c_int, P_int, P_int]),
         "bsam_cluster_tags_per_parent": (c_int, [vp, vp, c_int, c_int, P_int, c_int, c_int, c_double, P_int,

@@ -17,7 +17,7 @@ The device path has knobs that change speed but not the converged solution:
 
 ``autotune`` times complete solves from the hierarchy's uploaded state for
 each candidate and returns the fastest one whose final relative residual
-meets RelTol; when the hierarchy has a streaming level it then times the
+meets RelTol (an error when none does); when the hierarchy has a streaming level it then times the
 winner at each row-block height.  ``bit_exact_only`` restricts the search to
 the candidates that reproduce the reference's FP64 sequence bit for bit.
 """
@@ -61,7 +61,10 @@ def autotune(G, rel_tol=None, bit_exact_only=True, repeats=3, candidates=None):
         sec, st = time_solve(G, repeats=repeats, **kw)
         table.append(dict(kw, seconds=sec, vcycles=st["vcycles"], finalRel=st["finalRel"]))
     ok = [r for r in table if r["finalRel"] <= rel_tol]
-    best = min(ok or table, key=lambda r: r["seconds"])
+    if not ok:
+        raise RuntimeError(f"autotune: no candidate met RelTol {rel_tol:g} "
+                           f"(best final rel {min(r['finalRel'] for r in table):.3e})")
+    best = min(ok, key=lambda r: r["seconds"])
     keys = tuple(k for k in best if k not in ("seconds", "vcycles", "finalRel", "stream_rows"))
     best = dict({k: best[k] for k in keys}, stream_rows=best.get("stream_rows", 0), seconds=best["seconds"])
     if best["smoother"] in ("exact", "sweep") and G.streaming_levels():

@@ -29,7 +29,7 @@ def ndgrid(x, y):
 
 
 def _evalgrid(fh, X, Y):
-    # ebMakeLevel.m:163-174
+    # ebMakeLevel.m:95-105
     v = fh(X, Y)
     if np.isscalar(v) or np.ndim(v) == 0:
         v = float(v) * np.ones(X.shape)
@@ -45,7 +45,7 @@ def _evalgrid(fh, X, Y):
 
 
 def _srcquad(fh, XC, YC, hx, hy, nq):
-    # ebMakeLevel.m:176-200 (midpoint, or tensor Gauss-Legendre cell average)
+    # ebMakeLevel.m:108-143 (midpoint, or tensor Gauss-Legendre cell average)
     if nq <= 1:
         return _evalgrid(fh, XC, YC)
     xi, w = np.polynomial.legendre.leggauss(nq)
@@ -58,7 +58,7 @@ def _srcquad(fh, XC, YC, hx, hy, nq):
 
 
 def _bc_side(prob: Problem, side: int, xv, yv, isvert: bool):
-    # ebMakeLevel.m:146-161
+    # ebMakeLevel.m:78-92
     bcs = prob.bc[side]
     xv = np.asarray(xv, float)
     yv = np.asarray(yv, float)

@@ -13,7 +13,7 @@ evaluates from user functions or decides structurally:
   * ``parabolic_options``  <- ParabolicBSAM/src/ebpOptions.m:1-57
   * ``eprob``              <- ParabolicBSAM/src/ebpEprob.m:1-19
   * ``ebp_init``           <- ParabolicBSAM/src/ebpInit.m:1-61
-  * ``set_time``           <- ParabolicBSAM/src/ebpSetTime.m:1-60
+  * ``set_time``           <- ParabolicBSAM/src/ebpSetTime.m:1-59
   * ``ebp_regrid``         <- ParabolicBSAM/src/ebpRegrid.m:1-162
   * ``ebp_solve``          <- ParabolicBSAM/src/ebpSolve.m:1-107
 

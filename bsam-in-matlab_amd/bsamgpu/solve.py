@@ -18,12 +18,15 @@ class Solution(dict):
 
 def _add_level(H, boxes, dev):
     """ebAddLevel.m:10-40: host level build, then the device prolongation of
-    the parent window as the new level's initial guess."""
-    if dev is not None:
-        dev.pull_q(H)
-        dev.close()
+    the parent window as the new level's initial guess.  The existing levels
+    keep their device state (ebAddLevel leaves them unchanged): the new
+    hierarchy takes their Q device to device from the old one."""
     kNew = hierarchy.add_level(H, boxes)
-    dev = GpuHierarchy(H)
+    new = GpuHierarchy(H)
+    if dev is not None:
+        new.inherit(dev)
+        dev.close()
+    dev = new
     dev.init_level_prolong(kNew)
     dev.pull_q(H)
     if H.opts.Verbose >= 1:

@@ -19,10 +19,10 @@
 //   ghost records    EllipticBSAM/src/ebFillGhosts.m:18-92, ebInterpCf.m:41-153
 //   restrict         EllipticBSAM/src/ebTransfer.m:10-33, ebKernels.m:98-102
 //   coarse load      EllipticBSAM/src/ebCoarseLoad.m:17-32, ebKernels.m:13-25
-//   reflux           EllipticBSAM/src/ebFaceCorrections.m:107-133, ebCoarseLoad.m:37-45
+//   reflux           EllipticBSAM/src/ebFaceCorrections.m:29-55, ebCoarseLoad.m:37-45
 //   correct          EllipticBSAM/src/ebTransfer.m:57-71, ebKernels.m:105-123
 //   coarsest         EllipticBSAM/src/ebCoarsest.m:79-91
-//   residual         EllipticBSAM/src/ebCompResidual.m:271-332
+//   residual         EllipticBSAM/src/ebCompResidual.m:23-84
 //   composite mean   EllipticBSAM/src/ebCompositeMean.m:4-23
 #include <algorithm>
 
@@ -956,7 +956,7 @@ __device__ __forceinline__ T restricted_residual(const LevelDev<T>& L, const Foo
   return T(0.25) * (((rf[0] + rf[1]) + rf[2]) + rf[3]);
 }
 
-// one flux-balance contribution (ebFaceCorrections.m:111-133), in FP64;
+// one flux-balance contribution (ebFaceCorrections.m:29-55), in FP64;
 // `ring` is the QC ring value just outside the coarse face
 // hc: the coarse (parent) level's h^2 in FP64
 template <typename T>
@@ -1498,7 +1498,7 @@ __device__ __forceinline__ void block_reduce4(double a, double b, double c, long
   }
 }
 
-// QC refresh of every level k >= kmin in one launch (ebCompResidual.m:271-287):
+// QC refresh of every level k >= kmin in one launch (ebCompResidual.m:23-39):
 // ring gather and QC interior = R(u) (no parent scatter).  Items of the
 // combined table: (level, item) with item < nring_all + nfoot.
 template <typename T>
@@ -1516,7 +1516,7 @@ __global__ void __launch_bounds__(kTile) k_qc_refresh_all(const LevelDev<T>* __r
 }
 
 // one cell of the composite residual: r = FTrue - L(Q) + reflux chunks
-// (ebCompResidual.m:302-332).  Accumulates max|r|, sum r^2 da, max|FTrue| and
+// (ebCompResidual.m:54-78).  Accumulates max|r|, sum r^2 da, max|FTrue| and
 // the visible count; optionally writes r to rout[level].
 template <typename T, typename TO>
 __device__ __forceinline__ void residual_item(const LevelDev<T>* __restrict__ LV, int nlev, int lvl, int tile,
@@ -1559,7 +1559,7 @@ __device__ __forceinline__ void residual_item(const LevelDev<T>* __restrict__ LV
   }
 }
 
-// composite residual of every level in [k0, nlev] (ebCompResidual.m:302-332);
+// composite residual of every level in [k0, nlev] (ebCompResidual.m:54-78);
 // optionally writes r (the defect-correction right side).  The tile table is
 // (level, tile) pairs; a workgroup takes every gridDim.x-th tile and writes one
 // part, so the final reduction reads at most kResidualParts parts.
@@ -2368,6 +2368,7 @@ template void launch_residual_all<double, double>(const LevelDev<double>*, int, 
                                                  double* const*, int, hipStream_t);
 template void launch_residual_all<float, float>(const LevelDev<float>*, int, const int2*, int, int, ResidualOut*,
                                                 float* const*, int, hipStream_t);
+template void launch_fill_value<uint8_t>(uint8_t*, long long, uint8_t, hipStream_t);
 template void launch_convert<float, double>(float*, const double*, long long, hipStream_t);
 template void launch_convert<double, float>(double*, const float*, long long, hipStream_t);
 

@@ -3,7 +3,7 @@
 //
 // The descriptor tables are derived from the patch boxes the way the reference
 // derives its gather/scatter descriptors (EllipticBSAM/src/ebBuildOps.m:17-199,
-// ebGatherParentCoeffs.m:286-337), but flattened into per-level arrays of
+// ebGatherParentCoeffs.m:12-63), but flattened into per-level arrays of
 // absolute element indices so that every kernel sweeps all patches of a level
 // in one launch.  The solve loop replays the reference's call sequence
 // (ebSolveCycles.m / ebVcycle.m / ebRelaxLevel.m / ebTransfer.m) as a stream
@@ -17,6 +17,7 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -53,6 +54,22 @@ int fail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess)                                                              \
       return fail(BSAM_E_HIP, "bsamgpu:hip: %s failed: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
+
+// two HIP events released on every return path
+struct EventPair {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  hipError_t create() {
+    for (hipEvent_t& x : e) {
+      const hipError_t r = hipEventCreate(&x);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  }
+  ~EventPair() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
 
 struct Box {
   int ilo, ihi, jlo, jhi;
@@ -515,7 +532,7 @@ void build_phys_level(HLevel& L) {
 }
 
 // Tables of level k (F) against its parent level k-1 (C): ebBuildOps.m:17-199
-// and ebGatherParentCoeffs.m:286-337.
+// and ebGatherParentCoeffs.m:12-63.
 int build_level_tables(HLevel& F, HLevel& C, bool qif) {
   OwnerMap OF, OC;
   OF.build(F);
@@ -658,7 +675,7 @@ int build_level_tables(HLevel& F, HLevel& C, bool qif) {
         if (F.pi[p].cf) F.ring_cf.push_back(rr);
       }
     // ---- footprint: parent cell, F index, parent face coefficients
-    //      (ebBuildOps.m:144-156, ebGatherParentCoeffs.m:294-324; the last
+    //      (ebBuildOps.m:144-156, ebGatherParentCoeffs.m:20-46; the last
     //      parent patch in order wins on shared faces, as in the reference)
     std::vector<int> pdew((size_t)(cmx + 1) * cmy, -1), pdns((size_t)cmx * (cmy + 1), -1);
     for (int q = 0; q < C.np; ++q) {
@@ -692,7 +709,7 @@ int build_level_tables(HLevel& F, HLevel& C, bool qif) {
         fr.dns_n = pdns[ci + (size_t)(cj + 1) * cmx];
         F.foot.push_back(fr);
       }
-    // ---- reflux entries (ebBuildOps.m:158-166, ebFaceCorrections.m:107-133)
+    // ---- reflux entries (ebBuildOps.m:158-166, ebFaceCorrections.m:29-55)
     const int ko = F.klass_off.back();
     for (int side = 0; side < 4; ++side) {
       const int clen = side < 2 ? cmy : cmx;
@@ -1572,7 +1589,7 @@ int bsam_init_level_prolong(bsam_hier* h, int k) {
   return sync(h);
 }
 
-// |r|_inf / max(|FTrue|_inf, realmin) (ebCompResidual.m:328-332); NaN when the
+// |r|_inf / max(|FTrue|_inf, realmin) (ebCompResidual.m:80-84); NaN when the
 // residual holds non-finite values (the max-reductions skip NaN, the sum does not)
 static double residual_rel(const ResidualOut& r) {
   if (!std::isfinite(r.l2sq)) return NAN;
@@ -1686,8 +1703,10 @@ static int run_cycle(bsam_hier* h, bool mixed) {
     int rc = mixed ? enqueue_cycle_mixed(h, s) : enqueue_cycle64(h, s);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
-    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess) return fail(BSAM_E_HIP, "bsamgpu:hip: graph capture failed: %s", hipGetErrorString(e));
+    if (rc || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc ? rc : fail(BSAM_E_HIP, "bsamgpu:hip: graph capture failed: %s", hipGetErrorString(e));
+    }
     h->graph = g;
     HIPCHK(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
     h->graph_key = key;
@@ -1712,9 +1731,9 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
   if (mixed)
     if (int rc = ensure32(h)) return rc;
   HIPCHK(hipSetDevice(h->ctx->device));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  EventPair ev;
+  HIPCHK(ev.create());
+  hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   HIPCHK(hipEventRecord(e0, h->stream));
   // ebSolveCycles.m:11-19: fill every level, baseline residual
   Seq<double> d = seq64(h, h->stream);
@@ -1742,8 +1761,6 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (out) {
     out->amr_levels = h->nlev - h->root + 1;
     out->nlev = h->nlev;
@@ -1820,6 +1837,92 @@ int bsam_tag_level(bsam_hier* h, int k, int indicator, double threshold, int buf
   if (ntag) {
     long long c = 0;
     for (long long i = 0; i < n; ++i) c += dst[i];
+    *ntag = c;
+  }
+  return 0;
+}
+
+// ---- tagging without allocation
+// The tag maps and the patch corner table of bsam_tag_level_static live in
+// one file-scope device array.  In host code `scratch` names a host-side
+// stand-in, not device memory: every use goes through the device address
+// that hipGetSymbolAddress returns, looked up once per device.
+constexpr long long kTagStaticMaxCells = 1LL << 22;    // n1*n2 of the largest level map (2048 x 2048)
+constexpr int kTagStaticMaxPatches = 16384;
+constexpr size_t kTagStaticLoBytes = (size_t)kTagStaticMaxPatches * sizeof(int2);
+constexpr size_t kTagStaticBytes = kTagStaticLoBytes + 3 * (size_t)kTagStaticMaxCells;
+static __device__ __align__(256) unsigned char scratch[kTagStaticBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// the calling thread's current device must be `dev`
+int tag_scratch(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(BSAM_E_ARG, "bsamgpu:arg: device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(BSAM_E_HIP, "bsamgpu:hip: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+int bsam_tag_level_static(bsam_hier* h, int k, int indicator, double threshold, int buffer, unsigned char* tags,
+                          unsigned char* raw, long long* ntag) {
+  if (int rc = check_level(h, k, 1)) return rc;
+  if (indicator != BSAM_TAG_ULAP && indicator != BSAM_TAG_RTE)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: indicator must be BSAM_TAG_ULAP or BSAM_TAG_RTE");
+  const HLevel& L = h->H[k - 1];
+  if (indicator == BSAM_TAG_RTE)
+    for (const PatchInfo& P : L.pi)
+      if ((P.mx | P.my) & 1) return fail(BSAM_E_ARG, "bsamgpu:arg: the rte indicator needs even patch widths");
+  const long long n = (long long)L.n[0] * L.n[1];
+  if (n > kTagStaticMaxCells)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: level %d map has %lld cells, above the %lld of bsam_tag_level_static", k, n,
+                kTagStaticMaxCells);
+  if (L.np > kTagStaticMaxPatches)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: level %d has %d patches, above the %d of bsam_tag_level_static", k, L.np,
+                kTagStaticMaxPatches);
+  HIPCHK(hipSetDevice(h->ctx->device));
+  unsigned char* base = nullptr;
+  if (int rc = tag_scratch(h->ctx->device, &base)) return rc;
+  int2* dlo = (int2*)base;
+  uint8_t* d = base + kTagStaticLoBytes;
+  // patch corners through a fixed host buffer (synchronous copies)
+  int2 chunk[256];
+  for (int p0 = 0; p0 < L.np; p0 += 256) {
+    const int m = std::min(256, L.np - p0);
+    for (int q = 0; q < m; ++q) chunk[q] = make_int2(L.box[p0 + q].ilo - 1, L.box[p0 + q].jlo - 1);
+    HIPCHK(hipMemcpyAsync(dlo + p0, chunk, (size_t)m * sizeof(int2), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+  }
+  // raw and coverage maps start at zero: the array keeps the last call's bytes
+  launch_fill_value<uint8_t>(d, 2 * n, 0, h->stream);
+  launch_tag_cells(h->d64[k - 1], dlo, indicator, threshold, d, d + n, h->stream);
+  launch_tag_finish(d, d + n, L.n[0], L.n[1], buffer, d + 2 * n, h->stream);
+  HIPCHK(hipGetLastError());
+  if (tags) HIPCHK(hipMemcpyAsync(tags, d + 2 * n, (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  if (raw) HIPCHK(hipMemcpyAsync(raw, d, (size_t)n, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (ntag) {
+    long long c = 0;
+    if (tags) {
+      for (long long i = 0; i < n; ++i) c += tags[i];
+    } else {  // count through a fixed host buffer
+      unsigned char buf[65536];
+      for (long long i0 = 0; i0 < n; i0 += (long long)sizeof buf) {
+        const size_t m = (size_t)std::min<long long>((long long)sizeof buf, n - i0);
+        HIPCHK(hipMemcpy(buf, d + 2 * n + i0, m, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < m; ++i) c += buf[i];
+      }
+    }
     *ntag = c;
   }
   return 0;
@@ -2054,9 +2157,9 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
   if (nlaunch < 1 || !ms_per_launch) return fail(BSAM_E_ARG, "bsamgpu:arg: bad smoother timing request");
   if (precision == BSAM_MIXED)
     if (int rc = ensure32(h)) return rc;
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  EventPair ev;
+  HIPCHK(ev.create());
+  hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   HIPCHK(hipEventRecord(e0, h->stream));
   const bool stream = h->d64[k - 1].stream && h->opts.smoother != 3;
   const bool sweep = stream || (h->opts.smoother == 2 && h->d64[k - 1].nsweep > 0);
@@ -2102,8 +2205,6 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   *ms_per_launch = ms / nlaunch;
   return sync(h);
 }

@@ -44,10 +44,10 @@ extern "C" {
 #define BSAM_E_RING (-4)       /* eb_build_ops:ring         ebBuildOps.m:233-236 */
 #define BSAM_E_OVERHANG (-5)   /* eb_build_ops:overhang     ebBuildOps.m:240-242 */
 #define BSAM_E_CORR (-6)       /* eb_build_ops:corr         ebBuildOps.m:269-271 */
-#define BSAM_E_COVER (-7)      /* eb_gather_parent_coeffs:cover  ebGatherParentCoeffs.m:327-330 */
-#define BSAM_E_BOX (-8)        /* eb_make_level:box         ebMakeLevel.m:86-89 */
-#define BSAM_E_D (-9)          /* eb_make_level:D           ebMakeLevel.m:112-114 */
-#define BSAM_E_NONFINITE (-10) /* eb_make_level:eval        ebMakeLevel.m:171-173 */
+#define BSAM_E_COVER (-7)      /* eb_gather_parent_coeffs:cover  ebGatherParentCoeffs.m:53-56 */
+#define BSAM_E_BOX (-8)        /* eb_make_level:box         ebMakeLevel.m:17-20 */
+#define BSAM_E_D (-9)          /* eb_make_level:D           ebMakeLevel.m:43-45 */
+#define BSAM_E_NONFINITE (-10) /* eb_make_level:eval        ebMakeLevel.m:99-104 */
 #define BSAM_E_SINGULAR (-11)  /* coarsest matrix singular */
 #define BSAM_E_HIP (-20)       /* HIP runtime failure */
 #define BSAM_E_NOMEM (-21)
@@ -131,7 +131,7 @@ typedef struct {
   double gpu_ms;       /* device time of the cycle loop (HIP events) */
 } bsam_stage_stats;
 
-/* EllipticBSAM/src/ebCompResidual.m:328-332 */
+/* EllipticBSAM/src/ebCompResidual.m:80-84 */
 typedef struct {
   double linf, l2, fnorm, rel;
   long long ncell;
@@ -144,7 +144,7 @@ const char *bsam_version(void);
 int bsam_ctx_create(int hip_device, bsam_ctx **out);
 void bsam_ctx_destroy(bsam_ctx *ctx);
 
-/* Upload H and derive its tables: ebBuildOps.m:17-199, ebGatherParentCoeffs.m:286-337,
+/* Upload H and derive its tables: ebBuildOps.m:17-199, ebGatherParentCoeffs.m:12-63,
  * ebCoarsest.m:14-77 ('assemble'). */
 int bsam_hier_create(bsam_ctx *ctx, const bsam_hier_desc *desc, const bsam_solve_opts *opts,
                      bsam_hier **out);
@@ -168,7 +168,7 @@ int bsam_coarse_load(bsam_hier *h, int k);
 int bsam_coarsest_solve(bsam_hier *h);
 /* F = FTrue on every level (ebSolveCycles.m:30-34) */
 int bsam_reset_force(bsam_hier *h);
-/* ebAddLevel.m:368-375: fill level k-1 ghosts, prolong its window into level k, fill k */
+/* ebAddLevel.m:25-32: fill level k-1 ghosts, prolong its window into level k, fill k */
 int bsam_init_level_prolong(bsam_hier *h, int k);
 /* == [m, area] = ebCompositeMean(H)   EllipticBSAM/src/ebCompositeMean.m:1-24 */
 int bsam_composite_mean(bsam_hier *h, double *mean, double *area);
@@ -188,6 +188,19 @@ int bsam_composite_mean(bsam_hier *h, double *mean, double *area);
 #define BSAM_TAG_RTE 1
 int bsam_tag_level(bsam_hier *h, int k, int indicator, double threshold, int buffer, unsigned char *tags,
                    unsigned char *raw, long long *ntag);
+
+/* bsam_tag_level without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as bsam_tag_level, except that the tag
+ * maps and the patch-corner table live in one library-owned device array of
+ * 12.125 MiB per device (held until the process ends): a level map may have
+ * at most 2^22 cells (n1*n2, e.g. 2048 x 2048) and a level at most 16384
+ * patches, else BSAM_E_ARG.  Every call on a device shares that device's
+ * array, so calls that overlap on one device (from several threads or
+ * streams) race; each call clears what must start at zero. */
+int bsam_tag_level_static(bsam_hier *h, int k, int indicator, double threshold, int buffer, unsigned char *tags,
+                          unsigned char *raw, long long *ntag);
 
 /* Clustering of a tag map into patch boxes (EllipticBSAM/src/ebTagCluster.m:92-380):
    Berger-Rigoutsos signature splitting, bisection of boxes that leave the

@@ -812,7 +812,7 @@ class OracleHierarchy:
         return stage
 
     def init_level_prolong(self, k):
-        """ebAddLevel.m:368-375: fill parent ghosts, prolong the window, fill."""
+        """ebAddLevel.m:25-32: fill parent ghosts, prolong the window, fill."""
         self.fill_ghosts(k - 1)
         lev = self.L(k)
         for p in range(lev.np):

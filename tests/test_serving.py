@@ -1,16 +1,25 @@
 """Served workload: seeded request fixture, and (GPU) the continuous-batching
-loop against one-at-a-time solves.
+loop against the CPU oracle and against one-at-a-time solves.
 
-Stated tolerance: bit-identical.  A request's stage records (V-cycles,
-residual histories) and its final Q are the same whether it is served alone
-or in a mix, and in FP64 they equal the plain multi-launch ebSolve path.
+Stated tolerances:
+- FP64: bit-identical.  All 512 requests of the served workload, served
+  together, give the oracle's boxes, V-cycle counts, residual histories and
+  Q (per-level SHA-256 digests of the padded arrays, tests/golden/
+  config5_oracle.json, written by tests/golden/make_config5_oracle.py); a
+  request's result is the same whether it is served alone or in the mix.
+- FP32-inner: every request converges to its RelTol 1e-8; the visible-cell
+  relative L2 difference from the FP64 (= oracle) solution is reported and
+  must be below 1e-6 wherever both forms built the same hierarchy; alone and
+  in the mix the result is bit-identical.
 """
 import json
 import os
+import time
 
 import numpy as np
 import pytest
 
+import config5
 from bsamgpu import configs, serving
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
@@ -178,3 +187,102 @@ def test_parabolic_requests_in_the_mix_match_ebp_solve():
         for k in range(1, H.nlev + 1):
             for p in range(H.L(k).np):
                 assert np.array_equal(r.Q[k - 1][p], H.L(k).Q[p]), (name, k, p)
+
+
+def test_oracle_fixture_matches_oracle():
+    """A sample of the committed oracle records, recomputed by the oracle:
+    one request of every (BaseCells, MaxLevels, boundary mix) kind."""
+    d = config5.fixture()["requests"]
+    gold = config5.oracle_fixture()["requests"]
+    assert len(gold) == len(d) == 512
+    seen = set()
+    for par in d:
+        kind = (par["base"], par["levels"], tuple(par["bc"]).count("dirichlet") in (0, 4))
+        if kind in seen or len(seen) >= 12:
+            continue
+        seen.add(kind)
+        assert config5.oracle_record(par) == gold[par["id"]], par["id"]
+    # the workload refines: some requests run 2 and 3 stages
+    assert {len(g["vcycles"]) for g in gold} == {1, 2, 3}
+    assert max(g["finalRel"][-1] for g in gold) <= 1e-8
+
+
+def _served_record(r):
+    return config5.record([[b for b in r.H.L(k).box] for k in range(1, r.H.nlev + 1)], r.stages, r.Q)
+
+
+def _serve_all(precision, slots=128):
+    reqs = _requests(range(512))
+    srv = serving.Server(slots=slots, precision=precision)
+    t0 = time.perf_counter()
+    done = {r.id: r for r in srv.run(reqs, realtime=False)}
+    wall = time.perf_counter() - t0
+    print(f"{precision}: 512 requests in {wall:.2f} s, {srv.ticks} ticks, "
+          f"mean batch {srv.tick_items / max(srv.ticks, 1):.1f}", flush=True)
+    srv.close()
+    assert len(done) == 512
+    for r in done.values():
+        assert r.error is None, (r.id, r.error)
+    return done
+
+
+@pytest.mark.gpu
+def test_served_workload_fp64_equals_oracle():
+    """All 512 requests served together in FP64 equal the oracle's ebSolve
+    bit for bit (EllipticBSAM/src/ebSolve.m:22-77 per request)."""
+    gold = config5.oracle_fixture()["requests"]
+    done = _serve_all("fp64")
+    bad = [i for i, r in done.items() if _served_record(r) != gold[i]]
+    assert not bad, bad[:10]
+    assert all(r.stages[-1]["finalRel"] <= 1e-8 for r in done.values())
+
+
+@pytest.mark.gpu
+def test_served_workload_mixed_converges_with_stated_error():
+    gold = config5.oracle_fixture()["requests"]
+    done64 = _serve_all("fp64")
+    done = _serve_all("mixed")
+    errs = []
+    for i, r in done.items():
+        assert r.stages[-1]["finalRel"] <= 1e-8, (i, r.stages[-1]["finalRel"])
+        a = done64[i]
+        if [list(map(list, a.H.L(k).box)) for k in range(1, a.H.nlev + 1)] != \
+                [list(map(list, r.H.L(k).box)) for k in range(1, r.H.nlev + 1)]:
+            continue  # tagging on the mixed stage solution chose another hierarchy
+        num = den = 0.0
+        for k in range(a.H.rootIdx, a.H.nlev + 1):
+            L = a.H.L(k)
+            for p in range(L.np):
+                vis = ~L.covered[p]
+                x = r.Q[k - 1][p][1:-1, 1:-1][vis]
+                y = a.Q[k - 1][p][1:-1, 1:-1][vis]
+                num += float(((x - y) ** 2).sum())
+                den += float((y ** 2).sum())
+        errs.append(np.sqrt(num / den))
+    errs = np.array(errs)
+    print(f"mixed vs fp64 (= oracle): {errs.size} requests on the same hierarchy, rel L2 max {errs.max():.2e}, "
+          f"median {np.median(errs):.2e}", flush=True)
+    assert errs.size >= 400 and errs.max() < 1e-6
+    assert sum(_served_record(done64[i]) == gold[i] for i in done64) == 512
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_served_alone_equals_in_the_full_mix(precision):
+    """A request of every stage count and size, served alone, gives exactly
+    what it gives inside the full 512-request mix."""
+    gold = config5.oracle_fixture()["requests"]
+    pick = []
+    for nst in (1, 2, 3):
+        for base in (32, 64, 128):
+            ids = [i for i, g in enumerate(gold) if len(g["vcycles"]) == nst and
+                   config5.fixture()["requests"][i]["base"] == base]
+            pick += ids[:1]
+    assert len(pick) >= 7
+    mix = _serve_all(precision)
+    for r in _requests(pick):
+        srv = serving.Server(slots=4, precision=precision)
+        alone = srv.run([r], realtime=False)[0]
+        srv.close()
+        print(f"request {r.id}: stages {[s['vcycles'] for s in alone.stages]}", flush=True)
+        _same(alone, mix[r.id])
