@@ -39,7 +39,7 @@ def solve_opts(opts, precision=None, use_graph=True, inner_cycles=1, max_vcycles
     o.use_graph = 1 if use_graph else 0
     o.coarse_cells = int(coarse_cells)
     sm = smoother if smoother is not None else getattr(opts, "Smoother", "exact")
-    o.smoother = {"exact": 0, "tiled": 1, "sweep": 2, "passes": 3}[sm] if isinstance(sm, str) else int(sm)
+    o.smoother = {"exact": 0, "tiled": 1, "sweep": 2, "passes": 3, "classic": 4}[sm] if isinstance(sm, str) else int(sm)
     o.stream_rows = int(stream_rows)
     return o
 

@@ -121,6 +121,39 @@ struct alignas(16) SweepTile {
   SweepHalo halo[kSweepMaxHalo];
 };
 
+// Whole relax calls in one launch (levels with L.arith): a region tile owns
+// the cells of a rectangle of one patch and stages, in an LDS canvas laid out
+// in the level's index space, every cell whose value after some pass the
+// owned cells' final values depend on (same-patch neighbours, sibling cells
+// across copy faces, coarse-fine interpolation sources).  Each such cell has a
+// depth d: its value is exact after passes 1..d of a call of `rdepth` passes,
+// the tile's own cells have depth rdepth, and a pass p updates the cells of
+// its colour with depth >= p (a cell of depth 0 is only read).
+// Cell entries: patch slot (bits 0-3) | i (4-15) | j (16-27) | depth (28-31),
+// (i, j) 0-based in the patch; the lists of a tile are its depth-0 cells,
+// then its colour-1 cells, then its colour-2 cells.
+constexpr int kRegionMaxPatches = 16;
+constexpr int kRegionMaxSlots = 4096;
+constexpr int kRegionMaxDepth = 14;
+constexpr int kRegionThreads = 1024;  // lanes of a region workgroup
+constexpr int kRegionK = 1;           // updated cells per lane and colour
+constexpr int kRegionKL = 1;          // depth-0 cells per lane
+// entries per tile: [colour 1: NT*K][colour 2: NT*K][depth 0: NT*KL], the
+// unused ones kRegionNone; the patches of a tile take kRegionMaxPatches slots
+constexpr int kRegionStride = kRegionThreads * (2 * kRegionK + kRegionKL);
+constexpr unsigned kRegionNone = 0xffffffffu;
+struct alignas(16) RegionTile {
+  int cx0, cy0, cw, ch;   // canvas origin (0-based level index) and size
+  int np, nload, nred, nblack;
+};
+struct RegionPatch {
+  PatchInfo pi;  // the patch record
+  int ox, oy;    // canvas position of the patch's cell (0, 0)
+};
+__host__ __device__ inline unsigned region_code(int ps, int i, int j, int d) {
+  return (unsigned)ps | ((unsigned)i << 4) | ((unsigned)j << 16) | ((unsigned)d << 28);
+}
+
 // device tables and buffers of one level
 template <typename T>
 struct LevelDev {
@@ -141,6 +174,12 @@ struct LevelDev {
   int nsweep;             // fused sweep tiles (0: the level runs colour passes)
   SweepTile* sweep;
   T* Qs;                  // the sweep's second copy of Q (same layout)
+  int nregion;            // region tiles (0: the level's relax calls take other forms)
+  int rdepth;             // passes per relax call the region tables cover
+  int rslots;             // largest canvas of the level's region tiles
+  RegionTile* region;
+  RegionPatch* rpatch;    // kRegionMaxPatches per tile
+  unsigned* rcell;        // kRegionStride per tile
   PatchInfo* P;
   TileRec* tiles;    // patch record + first cell per 256-cell tile
   int2* ctiles;      // (patch, first coarse cell) per tile of the footprint

@@ -35,6 +35,19 @@ template <typename T> struct StreamRows { static constexpr int value = sizeof(T)
 template <typename T>
 void launch_sweep_stream(const LevelDev<T>& L, const PatchInfo& P, const T* Qin, T* Qout, T omega, int homog,
                          int ncu, int rows_req, hipStream_t s);
+// a whole relax call of nsweeps red+black sweeps on the level's region tiles
+// (levels with L.nregion, 2 nsweeps <= L.rdepth), own cells into L.Qs; frame:
+// pass 1 reads internal-face ghosts from the ghost frame; the parent parts
+// in L.aux must be those of the current parent.  launch_fill_copy
+// (Qsrc = L.Qs) must follow: it copies the interior into Q and fills the
+// ghosts from it
+template <typename T>
+void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, int frame, T omega, int homog,
+                         hipStream_t s);
+template <typename T>
+void launch_fill_copy(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, const T* Qsrc, hipStream_t s);
+// aux parent parts of every coarse-fine record of the level from the current parent
+template <typename T> void launch_refresh_cf(const LevelDev<T>& L, const LevelDev<T>* parent, hipStream_t s);
 // all passes of one relax call, one patch per workgroup staged in LDS
 constexpr long long kTiledLdsBytes = 64 * 1024;
 template <typename T> bool tiled_fits(const LevelDev<T>& L);

@@ -107,19 +107,22 @@ __device__ __forceinline__ T ghost_value(const GhostRec& r, const T* __restrict_
 // has a parent) one QC ring refresh of a patch with coarse-fine faces.  A
 // coarse-fine record also stores its parent part in L.aux for the smoother
 // passes that assemble ghosts from neighbours.
+// Qsrc: where the interior values are read (L.Q, or a second copy that
+// holds the level's new interior values)
 template <typename T>
 __device__ __forceinline__ void fill_item(const LevelDev<T>& L, const T* __restrict__ Pq, int t,
-                                          int homog) {
+                                          int homog, const T* __restrict__ Qsrc = nullptr) {
   if (t < L.nghost) {
+    const T* __restrict__ Qr = Qsrc ? Qsrc : L.Q;
     const GhostRec r = L.ghost[t];
     T v;
     if (r.kind == G_PCORNER) {
       // physical corner from the two adjacent side ghosts (ebFillGhosts.m:85-89)
-      const T a = ghost_value<T>(L.ghost[r.s[0]], L.Q, Pq, homog);
-      const T b = ghost_value<T>(L.ghost[r.s[1]], L.Q, Pq, homog);
-      v = (a + b) - L.Q[r.s[2]];
+      const T a = ghost_value<T>(L.ghost[r.s[0]], Qr, Pq, homog);
+      const T b = ghost_value<T>(L.ghost[r.s[1]], Qr, Pq, homog);
+      v = (a + b) - Qr[r.s[2]];
     } else {
-      v = ghost_value<T>(r, L.Q, Pq, homog);
+      v = ghost_value<T>(r, Qr, Pq, homog);
       if (L.aux && is_cf_kind(r.kind)) L.aux[t].cf = cf_part<T>(r, Pq);
     }
     L.Q[r.dst] = v;
@@ -576,6 +579,270 @@ __global__ void __launch_bounds__(256) k_relax_sweep(LevelDev<T> L, const SweepT
     if ((i == 0 || ee) && !P.bc[ee ? 1 : 0] && rax[k].push >= 0) Qout[rax[k].push] = y;
     if ((j == 0 || nn) && !P.bc[nn ? 3 : 2] && ray[k].push >= 0) Qout[ray[k].push] = y;
   }
+}
+
+// ---------------------------------------------------------------- region relax
+// A whole relax call (ebRelaxLevel.m:28-38: nsweeps x two colour passes, the
+// ghosts refreshed after each) in one launch per level.  A workgroup stages
+// its region tile in an LDS canvas in the level's index space: neighbours in
+// the same patch and sibling cells across copy faces are then the adjacent
+// canvas slots, and a coarse-fine ghost is assembled from the canvas and the
+// parent part exactly as ebInterpCf's record would after ebFillGhosts.  Every
+// cell the workgroup updates keeps its coefficients in registers for the
+// whole call; pass p updates, in place, the cells of its colour whose depth
+// is at least p (they read only cells of the other colour and themselves, so
+// the in-place update is race free), with a barrier between passes.  The
+// tile's own cells go to Qout, a second copy of Q that no workgroup reads in
+// this launch; launch_fill_copy then moves them into Q while it refreshes the
+// ghosts, as the fill after the last pass.
+// Pass 1 of a call whose ghost frame is what the reference's first pass reads
+// (frame = 1) takes the internal-face ghosts from the frame; otherwise, and in
+// every later pass, they come from the canvas and the parent.
+enum RegionFace : int { RF_NONE = 0, RF_DIR = 1, RF_NEU = 2, RF_QIF = 3, RF_QIF_END = 4, RF_LIF = 5, RF_COPY = 6 };
+
+template <typename T>
+struct RegionCell {
+  T f, id, de, dw, dn, ds;
+  T bx, by;   // x / y face: boundary data (physical side) or parent part (coarse-fine)
+  int slot;   // canvas slot, -1: no cell
+  int fl;     // x face: kind (bits 0-2), east (3), even position (4); y face: kind (5-7), north (8),
+              // even (9); depth (bits 12-15)
+  int dxy;    // canvas offsets of the diagonal interpolation sources: x face low 16 bits, y face high 16
+};
+
+__device__ __forceinline__ int region_cf_kind(int kind) {
+  return kind == G_COPY ? RF_COPY : kind == G_QIF ? RF_QIF : kind == G_QIF_END ? RF_QIF_END : RF_LIF;
+}
+__device__ __forceinline__ int region_ghost_kind(int rf) {
+  return rf == RF_QIF ? G_QIF : rf == RF_QIF_END ? G_QIF_END : G_LIF;
+}
+
+// Issues the loads of one updated cell straight into its state: the value
+// (into q), the coefficients and, for a cell on a face, the boundary datum or
+// the face record's kind and parent part and (with frame) the frame's ghost
+// value.  Nothing loaded is used here, so the loads of all of a lane's cells
+// are in flight together.  kx / ky: the internal faces' record kinds (-1: a
+// physical side or no face), resolved by region_kinds once the loads land.
+template <typename T>
+__device__ __forceinline__ void region_load(const LevelDev<T>& L, const RegionPatch* sp, int cw, unsigned code,
+                                            int homog, bool frame, RegionCell<T>& c, T& q, int& qc, T& frx,
+                                            T& fry, int& kx, int& ky) {
+  const unsigned cd = code == kRegionNone ? 0u : code;  // a stand-in cell for an empty entry
+  const int ps = cd & 15, i = (cd >> 4) & 4095, j = (cd >> 16) & 4095;
+  const PatchInfo& P = sp[ps].pi;
+  const int mx = P.mx, my = P.my, ldq = mx + 2;
+  qc = P.qoff + (i + 1) + (j + 1) * ldq;
+  const int cc = P.coff + i + j * mx, fx = P.xoff + i + j * (mx + 1), fy = P.yoff + i + j * mx;
+  c.slot = code == kRegionNone ? -1 : (sp[ps].ox + i) + (sp[ps].oy + j) * cw;
+  q = L.Q[qc];
+  c.f = L.F[cc];
+  c.id = L.ID[cc];
+  c.dw = L.DeW[fx];
+  c.de = L.DeW[fx + 1];
+  c.ds = L.DnS[fy];
+  c.dn = L.DnS[fy + mx];
+  const bool e = i == mx - 1, n = j == my - 1;
+  int fl = (int)((cd >> 28) << 12), dx = 0, dy = 0;
+  kx = ky = -1;
+  c.bx = c.by = T(0);
+  frx = fry = T(0);
+  if (i == 0 || e) {
+    const int side = e ? 1 : 0, bc = P.bc[side];
+    if (bc) {
+      fl |= bc == 1 ? RF_DIR : RF_NEU;
+      if (!homog) c.bx = L.G[P.goff + (e ? my : 0) + j];
+    } else {
+      const FaceAux<T>* ax = L.aux + P.gbase + (e ? my : 0) + j;
+      kx = ax->kind;
+      c.bx = ax->cf;
+      if (frame) frx = L.Q[qc + (e ? 1 : -1)];
+      dx = (e ? -1 : 1) + cf_diag_step(j + 1, my) * cw;
+    }
+    fl |= (e ? 8 : 0) | (((j + 1) & 1) == 0 ? 16 : 0);
+  }
+  if (j == 0 || n) {
+    const int side = n ? 3 : 2, bc = P.bc[side];
+    if (bc) {
+      fl |= (bc == 1 ? RF_DIR : RF_NEU) << 5;
+      if (!homog) c.by = L.G[P.goff + 2 * my + (n ? mx : 0) + i];
+    } else {
+      const FaceAux<T>* ay = L.aux + P.gbase + 2 * my + (n ? mx : 0) + i;
+      ky = ay->kind;
+      c.by = ay->cf;
+      if (frame) fry = L.Q[qc + (n ? ldq : -ldq)];
+      dy = (n ? -cw : cw) + cf_diag_step(i + 1, mx);
+    }
+    fl |= (n ? 256 : 0) | (((i + 1) & 1) == 0 ? 512 : 0);
+  }
+  c.fl = fl;
+  c.dxy = (dx & 0xffff) | (dy << 16);
+}
+
+// the internal faces' kinds, after the loads
+template <typename T>
+__device__ __forceinline__ void region_kinds(RegionCell<T>& c, int kx, int ky) {
+  if (kx >= 0) c.fl |= region_cf_kind(kx);
+  if (ky >= 0) c.fl |= region_cf_kind(ky) << 5;
+}
+
+__device__ __forceinline__ int region_depth(int fl) { return (fl >> 12) & 15; }
+__device__ __forceinline__ bool region_face(int fl) { return (fl & 0x3ff) != 0; }
+
+// one cell's update from the canvas: the expressions of resolve_ghosts and
+// relax_combine, in the same operation order; fr* are the frame's ghost
+// values (pass 1 of a call that reads the frame)
+template <typename T>
+__device__ __forceinline__ T region_update(const LevelDev<T>& L, const T* cv, int cw, const RegionCell<T>& c,
+                                           bool frame, T frx, T fry, T omega) {
+  const int s = c.slot;
+  const T self = cv[s], re = cv[s + 1], rw = cv[s - 1], rn = cv[s + cw], rs = cv[s - cw];
+  if (!region_face(c.fl))  // off every face (the host lists these cells first, so whole waves take this path)
+    return relax_combine<T>(L, re, rw, rn, rs, c.f, c.de, c.dw, c.dn, c.ds, c.id, self, omega);
+  T qe = re, qw = rw, qn = rn, qs = rs;
+  const int xk = c.fl & 7, yk = (c.fl >> 5) & 7;
+  if (xk) {
+    const bool e = (c.fl >> 3) & 1;
+    const T in = e ? rw : re;
+    T v = e ? re : rw;
+    if (xk == RF_DIR) {
+      v = T(C83_) * c.bx + T(C13_) * in;
+    } else if (xk == RF_NEU) {
+      v = (T)L.h[0] * c.bx;
+    } else if (frame) {
+      v = frx;
+    } else if (xk != RF_COPY) {
+      const T diag = cv[s + (int)(short)(c.dxy & 0xffff)];
+      v = cf_face_ghost<T>(region_ghost_kind(xk), ((c.fl >> 4) & 1) ? 2 : 1, self, in, rs, rn, diag, c.bx);
+    }
+    if (e) qe = v; else qw = v;
+  }
+  if (yk) {
+    const bool n = (c.fl >> 8) & 1;
+    const T in = n ? rs : rn;
+    T v = n ? rn : rs;
+    if (yk == RF_DIR) {
+      v = T(C83_) * c.by + T(C13_) * in;
+    } else if (yk == RF_NEU) {
+      v = (T)L.h[1] * c.by;
+    } else if (frame) {
+      v = fry;
+    } else if (yk != RF_COPY) {
+      const T diag = cv[s + (int)(short)(c.dxy >> 16)];
+      v = cf_face_ghost<T>(region_ghost_kind(yk), ((c.fl >> 9) & 1) ? 2 : 1, self, in, rw, re, diag, c.by);
+    }
+    if (n) qn = v; else qs = v;
+  }
+  return relax_combine<T>(L, qe, qw, qn, qs, c.f, c.de, c.dw, c.dn, c.ds, c.id, self, omega);
+}
+
+// npass = 2 x sweeps (even, <= L.rdepth).  Workgroup h takes tile
+// (h mod 8) * per + h / 8 (workgroups are dealt to the 8 XCDs in turn, so
+// each XCD runs a contiguous, spatially ordered range of tiles and shares
+// their halo lines in its L2); the grid is padded to 8 * per.  The header,
+// the patch records and a lane's entries sit at fixed offsets of the tile, so
+// they arrive in one round trip, and the cells' data in a second: the parent
+// parts of coarse-fine faces come from the face records' aux entries, which
+// must hold those of the current parent (a fill, a preparing correction or
+// launch_refresh_cf wrote them after the parent last changed).
+template <typename T, int NT, int K, int KL>
+__global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __restrict__ Pq, T* __restrict__ Qout,
+                                                     int npass, int frame, T omega, int homog, int per) {
+  extern __shared__ __align__(16) unsigned char region_lds[];
+  T* cv = reinterpret_cast<T*>(region_lds);
+  __shared__ RegionPatch sp[kRegionMaxPatches];
+  const int h = blockIdx.x;
+  const int t = (h % kXcds) * per + h / kXcds;
+  if (t >= L.nregion) return;
+  const int tid = threadIdx.x;
+  const RegionTile tr = L.region[t];
+  const unsigned* __restrict__ cl = L.rcell + (size_t)t * kRegionStride;
+  unsigned e1[K], e2[K], e0[KL];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    e1[k] = cl[tid + k * NT];
+    e2[k] = cl[NT * K + tid + k * NT];
+  }
+#pragma unroll
+  for (int k = 0; k < KL; ++k) e0[k] = cl[2 * NT * K + tid + k * NT];
+  if (tid < kRegionMaxPatches) sp[tid] = L.rpatch[(size_t)t * kRegionMaxPatches + tid];
+  const int cw = tr.cw;
+  for (int x = tid; x < cw * tr.ch; x += NT) cv[x] = T(0);
+  __syncthreads();
+  // every cell's loads, then their uses
+  RegionCell<T> c1[K], c2[K];
+  T q1[K], q2[K], fx1[K], fy1[K], fdx, fdy, q0[KL];
+  int qc1[K], qc2[K], kx1[K], ky1[K], kx2[K], ky2[K], s0[KL];
+#pragma unroll
+  for (int k = 0; k < KL; ++k) {
+    const unsigned cd = e0[k] == kRegionNone ? 0u : e0[k];
+    const int ps = cd & 15, i = (cd >> 4) & 4095, j = (cd >> 16) & 4095;
+    const PatchInfo& P = sp[ps].pi;
+    s0[k] = e0[k] == kRegionNone ? -1 : (sp[ps].ox + i) + (sp[ps].oy + j) * cw;
+    q0[k] = L.Q[P.qoff + (i + 1) + (j + 1) * (P.mx + 2)];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    region_load<T>(L, sp, cw, e1[k], homog, frame != 0, c1[k], q1[k], qc1[k], fx1[k], fy1[k], kx1[k], ky1[k]);
+    region_load<T>(L, sp, cw, e2[k], homog, false, c2[k], q2[k], qc2[k], fdx, fdy, kx2[k], ky2[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < KL; ++k)
+    if (s0[k] >= 0) cv[s0[k]] = q0[k];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    region_kinds<T>(c1[k], kx1[k], ky1[k]);
+    region_kinds<T>(c2[k], kx2[k], ky2[k]);
+    if (c1[k].slot >= 0) cv[c1[k].slot] = q1[k];
+    if (c2[k].slot >= 0) cv[c2[k].slot] = q2[k];
+  }
+  __syncthreads();
+  const int dshift = L.rdepth - npass;
+  for (int p = 1; p <= npass; p += 2) {
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c1[k].slot >= 0 && region_depth(c1[k].fl) >= p + dshift)
+        cv[c1[k].slot] = region_update<T>(L, cv, cw, c1[k], frame && p == 1, fx1[k], fy1[k], omega);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if (c2[k].slot >= 0 && region_depth(c2[k].fl) >= p + 1 + dshift)
+        cv[c2[k].slot] = region_update<T>(L, cv, cw, c2[k], false, T(0), T(0), omega);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (c1[k].slot >= 0 && region_depth(c1[k].fl) == L.rdepth) Qout[qc1[k]] = cv[c1[k].slot];
+    if (c2[k].slot >= 0 && region_depth(c2[k].fl) == L.rdepth) Qout[qc2[k]] = cv[c2[k].slot];
+  }
+}
+
+// ebFillGhosts after a region relax: the ghosts from the new interior values
+// in Qsrc (fill items, blocks [0, nfb)) and the interior copied from Qsrc into
+// Q (one block per 256-cell tile); nothing in the launch reads Q
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_fill_copy(LevelDev<T> L, const T* __restrict__ Pq, int homog,
+                                                     const T* __restrict__ Qsrc, int nfb) {
+  if ((int)blockIdx.x < nfb) {
+    const int t = blockIdx.x * kTile + threadIdx.x;
+    if (t < fill_items<T>(L, Pq)) fill_item<T>(L, Pq, t, homog, Qsrc);
+    return;
+  }
+  const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane((int)blockIdx.x - nfb));
+  const PatchInfo& pi = tr.pi;
+  const int c = tr.start + threadIdx.x;
+  if (c >= pi.mx * pi.my) return;
+  const int j = cell_row(c, pi.mx, pi.magic);
+  const int i = c - j * pi.mx;
+  const int qc = pi.qoff + (i + 1) + (j + 1) * (pi.mx + 2);
+  L.Q[qc] = Qsrc[qc];
+}
+
+// the parent part of every coarse-fine record of a level from the current
+// parent, into aux (what a fill stores, without touching Q)
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_refresh_cf(LevelDev<T> L, const T* __restrict__ Pq) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t < L.nghost) keep_face_cf<T>(L, Pq, t);
 }
 
 // ---------------------------------------------------------------- streaming sweep
@@ -1988,6 +2255,33 @@ void launch_fill(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, hip
 }
 
 template <typename T>
+void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, int frame, T omega,
+                         int homog, hipStream_t s) {
+  if (!L.nregion || nsweeps <= 0) return;
+  const int per = (L.nregion + kXcds - 1) / kXcds;
+  const size_t lds = (size_t)L.rslots * sizeof(T);
+  hipLaunchKernelGGL((k_relax_region<T, kRegionThreads, kRegionK, kRegionKL>), dim3(per * kXcds), dim3(kRegionThreads),
+                     lds, s,
+                     L, parent ? (const T*)parent->Q : (const T*)nullptr, L.Qs, 2 * nsweeps, frame, omega, homog,
+                     per);
+}
+
+template <typename T>
+void launch_refresh_cf(const LevelDev<T>& L, const LevelDev<T>* parent, hipStream_t s) {
+  if (!L.nghost || !parent || !L.aux) return;
+  hipLaunchKernelGGL(k_refresh_cf<T>, dim3(nblk(L.nghost)), dim3(kTile), 0, s, L, (const T*)parent->Q);
+}
+
+template <typename T>
+void launch_fill_copy(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, const T* Qsrc, hipStream_t s) {
+  const long long n = (long long)L.nghost + (parent ? L.nring_cf : 0);
+  const int nfb = (int)nblk(n);
+  if (nfb + L.ntile == 0) return;
+  hipLaunchKernelGGL(k_fill_copy<T>, dim3(nfb + L.ntile), dim3(kTile), 0, s, L,
+                     parent ? (const T*)parent->Q : (const T*)nullptr, homog, Qsrc, nfb);
+}
+
+template <typename T>
 void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s) {
   if (!L.nring_all) return;
   hipLaunchKernelGGL(k_ring<T>, dim3(nblk(L.nring_all)), dim3(kTile), 0, s, L, (const T*)parent.Q);
@@ -2342,6 +2636,9 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template bool tiled_fits<T>(const LevelDev<T>&);                                                      \
   template void launch_relax_tiled<T>(const LevelDev<T>&, const LevelDev<T>*, int, T, int, hipStream_t); \
   template void launch_fill<T>(const LevelDev<T>&, const LevelDev<T>*, int, hipStream_t);               \
+  template void launch_relax_region<T>(const LevelDev<T>&, const LevelDev<T>*, int, int, T, int, hipStream_t); \
+  template void launch_fill_copy<T>(const LevelDev<T>&, const LevelDev<T>*, int, const T*, hipStream_t); \
+  template void launch_refresh_cf<T>(const LevelDev<T>&, const LevelDev<T>*, hipStream_t);              \
   template void launch_ring<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                    \
   template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, T*);       \
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \

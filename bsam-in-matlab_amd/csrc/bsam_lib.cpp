@@ -10,6 +10,7 @@
 // of level-wide launches, optionally captured once into a HIP graph and
 // replayed per cycle.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -500,6 +501,259 @@ bool build_sweep_tiles(const HLevel& L, std::vector<SweepTile>& out) {
 
 // Ghost records of a level without a parent (level 1): all faces physical,
 // corners extrapolated (ebFillGhosts.m:53-89).
+// ------------------------------------------------------------ region tiles
+// Tables of the whole-relax-call kernel (k_relax_region).  For every cell of
+// the level, the cells its colour-pass update reads (ebKernels.m:37-95 with
+// the ghosts of ebFillGhosts / ebInterpCf): the same-patch neighbours, the
+// sibling cell a copy face ghost copies, the inward neighbour of a Dirichlet
+// side, and the interpolation sources of a coarse-fine face (the record's
+// cells, which face_records_arith has checked to follow the fixed stencil),
+// plus the cell itself.  A tile owns a rectangle of one patch; walking the
+// passes of a call backwards from its own cells gives each needed cell its
+// depth (region_code), and the canvas is the bounding box of those cells
+// with one slot of margin, so that every read of a pass stays in it.
+struct RegionBuild {
+  std::vector<RegionTile> tiles;
+  std::vector<RegionPatch> patches;  // kRegionMaxPatches per tile
+  std::vector<int> pid;              // their patch indices (-1: unused slot)
+  std::vector<unsigned> cells;       // kRegionStride per tile
+  int max_slots = 0;
+};
+
+bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
+  out = RegionBuild();
+  if (L.np == 0 || rdepth < 2 || rdepth > kRegionMaxDepth || rdepth % 2) return false;
+  for (const PatchInfo& P : L.pi)
+    if (P.mx < 2 || P.my < 2 || P.mx > 4095 || P.my > 4095) return false;
+  OwnerMap own;
+  own.build(L);
+  const int n0 = L.n[0];
+  // cell id = global index (0-based) gi + gj * n0
+  auto cell_of = [&](int p, int i, int j) -> long long {
+    return (long long)(L.box[p].ilo - 1 + i) + (long long)(L.box[p].jlo - 1 + j) * n0;
+  };
+  // dependencies of cell (p, i, j); false when a source is not where the
+  // fixed stencil puts it
+  std::vector<long long> dep;  // the last deps() result, global ids
+  auto deps = [&](int p, int i, int j) -> bool {
+    dep.clear();
+    const PatchInfo& P = L.pi[p];
+    const Box& b = L.box[p];
+    const int mx = P.mx, my = P.my;
+    dep.push_back(cell_of(p, i, j));
+    const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+    for (int d = 0; d < 4; ++d) {
+      const int ni = i + di[d], nj = j + dj[d];
+      if (ni >= 0 && ni < mx && nj >= 0 && nj < my) {
+        dep.push_back(cell_of(p, ni, nj));
+        continue;
+      }
+      const int ii = i - di[d], ij = j - dj[d];  // inward neighbour
+      if (P.bc[d]) {
+        if (P.bc[d] == 1) dep.push_back(cell_of(p, ii, ij));
+        continue;
+      }
+      const int t = d < 2 ? j : i, len = d < 2 ? my : mx, u = t + 1;
+      const int off = d == 0 ? 0 : d == 1 ? my : d == 2 ? 2 * my : 2 * my + mx;
+      const GhostRec& r = L.ghost[P.gbase + off + t];
+      const int ai = d < 2 ? 0 : 1, aj = d < 2 ? 1 : 0;  // along-face step
+      auto inpatch = [&](int a, int c) { return a >= 0 && a < mx && c >= 0 && c < my; };
+      switch (r.kind) {
+        case G_COPY: {
+          const int q = own.at(b.ilo + ni, b.jlo + nj) - 1;
+          if (q < 0 || q == p) return false;
+          const int si = b.ilo + ni - L.box[q].ilo, sj = b.jlo + nj - L.box[q].jlo;
+          if (r.s[0] != qidx(L.pi[q], si + 1, sj + 1)) return false;
+          dep.push_back(cell_of(q, si, sj));
+          break;
+        }
+        case G_QIF: {
+          if (u == 1 || u == len) return false;
+          const int s = (u & 1) == 0 ? 1 : -1;
+          if (!inpatch(i + s * ai, j + s * aj) || !inpatch(ii + 2 * s * ai, ij + 2 * s * aj)) return false;
+          dep.push_back(cell_of(p, i + s * ai, j + s * aj));
+          dep.push_back(cell_of(p, ii + 2 * s * ai, ij + 2 * s * aj));
+          break;
+        }
+        case G_QIF_END:
+          dep.push_back(cell_of(p, ii, ij));
+          break;
+        case G_LIF: {
+          const int s = (u & 1) == 0 ? -1 : 1;
+          if (!inpatch(i + s * ai, j + s * aj)) return false;
+          dep.push_back(cell_of(p, i + s * ai, j + s * aj));
+          break;
+        }
+        default:
+          return false;
+      }
+    }
+    return true;
+  };
+  // owner patch and local position of a cell id
+  auto locate = [&](long long c, int& p, int& i, int& j) {
+    const int gi = (int)(c % n0), gj = (int)(c / n0);
+    p = own.at(gi + 1, gj + 1) - 1;
+    i = gi - (L.box[p].ilo - 1);
+    j = gj - (L.box[p].jlo - 1);
+  };
+  // the dependencies of every cell, once (CSR over the level's cells in
+  // patch order, entries as global ids)
+  const long long nglob = (long long)n0 * L.n[1];
+  std::vector<int> cell_ix((size_t)nglob, -1);  // global id -> level cell index (patch order)
+  std::vector<long long> gid;                    // level cell index -> global id
+  std::vector<unsigned char> colour;
+  gid.reserve((size_t)L.ncell);
+  for (int p = 0; p < L.np; ++p)
+    for (int j = 0; j < L.pi[p].my; ++j)
+      for (int i = 0; i < L.pi[p].mx; ++i) {
+        const long long g = cell_of(p, i, j);
+        cell_ix[(size_t)g] = (int)gid.size();
+        gid.push_back(g);
+        colour.push_back((unsigned char)((i + j + L.pi[p].par0) & 1));
+      }
+  const int ncell = (int)gid.size();
+  std::vector<int> dptr(ncell + 1, 0), dlist;
+  dlist.reserve((size_t)ncell * 6);
+  {
+    int x = 0;
+    for (int p = 0; p < L.np; ++p)
+      for (int j = 0; j < L.pi[p].my; ++j)
+        for (int i = 0; i < L.pi[p].mx; ++i, ++x) {
+          if (!deps(p, i, j)) return false;
+          for (long long y : dep) dlist.push_back(cell_ix[(size_t)y]);
+          dptr[x + 1] = (int)dlist.size();
+        }
+  }
+  std::vector<int> depth_of(ncell, -1);  // reset after each tile through `order`
+  // one tile: own rectangle [i0, i1) x [j0, j1) of patch p
+  auto make_tile = [&](int p, int i0, int i1, int j0, int j1, bool& fits) -> bool {
+    fits = false;
+    std::vector<int> order;  // level cell indices of the region, in discovery order
+    for (int j = j0; j < j1; ++j)
+      for (int i = i0; i < i1; ++i) {
+        const int c = cell_ix[(size_t)cell_of(p, i, j)];
+        depth_of[c] = rdepth;
+        order.push_back(c);
+      }
+    // pass q updates colour (q - 1) & 1 (0: colour 1)
+    for (int q = rdepth; q >= 1; --q) {
+      const int col = (q - 1) & 1;
+      const size_t n = order.size();
+      for (size_t x = 0; x < n; ++x) {
+        const int c = order[x];
+        if (depth_of[c] < q || colour[c] != col) continue;
+        for (int e = dptr[c]; e < dptr[c + 1]; ++e) {
+          const int y = dlist[e];
+          if (depth_of[y] < 0) {
+            depth_of[y] = q - 1;
+            order.push_back(y);
+          } else if (depth_of[y] < q - 1) {
+            depth_of[y] = q - 1;
+          }
+        }
+      }
+    }
+    // canvas: bounding box of the region plus one slot of margin
+    int gx0 = 1 << 30, gx1 = -1, gy0 = 1 << 30, gy1 = -1;
+    for (int c : order) {
+      const int gi = (int)(gid[c] % n0), gj = (int)(gid[c] / n0);
+      gx0 = std::min(gx0, gi); gx1 = std::max(gx1, gi);
+      gy0 = std::min(gy0, gj); gy1 = std::max(gy1, gj);
+    }
+    const int cx0 = gx0 - 1, cy0 = gy0 - 1, cw = gx1 - gx0 + 3, ch = gy1 - gy0 + 3;
+    // level cell indices run in patch order, row by row: sorting them keeps
+    // each list in (patch, j, i) order, the cells off every face of their
+    // patch first (their updates take the kernel's short path, wave by wave)
+    std::vector<int> sorted(order);
+    auto on_face = [&](int c) {
+      int cp, ci, cj;
+      locate(gid[c], cp, ci, cj);
+      return ci == 0 || cj == 0 || ci == L.pi[cp].mx - 1 || cj == L.pi[cp].my - 1;
+    };
+    std::sort(sorted.begin(), sorted.end());
+    std::stable_partition(sorted.begin(), sorted.end(), [&](int c) { return !on_face(c); });
+    std::vector<int> pl;  // region patches
+    std::vector<unsigned> load, red, black;
+    bool fit = true;
+    for (int c : sorted) {
+      int cp, ci, cj;
+      locate(gid[c], cp, ci, cj);
+      int ps = -1;
+      for (size_t k = 0; k < pl.size(); ++k)
+        if (pl[k] == cp) ps = (int)k;
+      if (ps < 0) {
+        if ((int)pl.size() >= kRegionMaxPatches) { fit = false; break; }
+        pl.push_back(cp);
+        ps = (int)pl.size() - 1;
+      }
+      const int d = depth_of[c];
+      const unsigned code = region_code(ps, ci, cj, d);
+      if (d == 0) load.push_back(code);
+      else if (colour[c] == 0) red.push_back(code);
+      else black.push_back(code);
+    }
+    for (int c : order) depth_of[c] = -1;
+    const int per_colour = kRegionThreads * kRegionK, per_load = kRegionThreads * kRegionKL;
+    if (!fit || (long long)cw * ch > kRegionMaxSlots || (int)red.size() > per_colour ||
+        (int)black.size() > per_colour || (int)load.size() > per_load)
+      return true;  // does not fit: split
+    RegionTile rt;
+    std::memset(&rt, 0, sizeof rt);
+    rt.cx0 = cx0; rt.cy0 = cy0; rt.cw = cw; rt.ch = ch;
+    rt.np = (int)pl.size();
+    rt.nload = (int)load.size(); rt.nred = (int)red.size(); rt.nblack = (int)black.size();
+    for (int k = 0; k < kRegionMaxPatches; ++k) {
+      RegionPatch rp;
+      std::memset(&rp, 0, sizeof rp);
+      if (k < (int)pl.size()) {
+        rp.pi = L.pi[pl[k]];
+        rp.ox = L.box[pl[k]].ilo - 1 - cx0;
+        rp.oy = L.box[pl[k]].jlo - 1 - cy0;
+      }
+      out.patches.push_back(rp);
+      out.pid.push_back(k < (int)pl.size() ? pl[k] : -1);
+    }
+    const size_t base = out.cells.size();
+    out.cells.resize(base + kRegionStride, kRegionNone);
+    std::copy(red.begin(), red.end(), out.cells.begin() + base);
+    std::copy(black.begin(), black.end(), out.cells.begin() + base + per_colour);
+    std::copy(load.begin(), load.end(), out.cells.begin() + base + 2 * per_colour);
+    out.tiles.push_back(rt);
+    out.max_slots = std::max(out.max_slots, cw * ch);
+    fits = true;
+    return true;
+  };
+  // patches in order, each halved along its longer side until the region of
+  // every piece fits a workgroup
+  std::vector<std::array<int, 5>> work;
+  for (int p = 0; p < L.np; ++p) work.push_back({p, 0, L.pi[p].mx, 0, L.pi[p].my});
+  for (size_t w = 0; w < work.size(); ++w) {
+    std::array<int, 5> t = work[w];
+    std::vector<std::array<int, 5>> stack{t};
+    while (!stack.empty()) {
+      const std::array<int, 5> r = stack.back();
+      stack.pop_back();
+      bool fits = false;
+      if (!make_tile(r[0], r[1], r[2], r[3], r[4], fits)) return false;
+      if (fits) continue;
+      const int w0 = r[2] - r[1], h0 = r[4] - r[3];
+      if (w0 <= 2 && h0 <= 2) return false;
+      // the second half is pushed first so that tiles stay in spatial order
+      if (w0 >= h0) {
+        const int m = r[1] + w0 / 2;
+        stack.push_back({r[0], m, r[2], r[3], r[4]});
+        stack.push_back({r[0], r[1], m, r[3], r[4]});
+      } else {
+        const int m = r[3] + h0 / 2;
+        stack.push_back({r[0], r[1], r[2], m, r[4]});
+        stack.push_back({r[0], r[1], r[2], r[3], m});
+      }
+    }
+  }
+  return !out.tiles.empty();
+}
+
 void build_phys_level(HLevel& L) {
   L.ghost.clear();
   L.klass.clear();
@@ -1047,7 +1301,28 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
       }
     }
   }
-  if (D.nsweep || D.stream) {  // the sweeps' second copy of Q
+  // whole relax calls per launch (region tiles) on levels with the standard
+  // face layout that do not stream; the FP32 copy shares the tables
+  D.nregion = 0; D.rdepth = 0; D.rslots = 0;
+  D.region = nullptr; D.rpatch = nullptr; D.rcell = nullptr;
+  if (D.arith && !D.stream && L.np > 0) {
+    if (tables_from64) {
+      D.nregion = d64->nregion; D.rdepth = d64->rdepth; D.rslots = d64->rslots;
+      D.region = d64->region; D.rpatch = d64->rpatch; D.rcell = d64->rcell;
+    } else {
+      const int rdepth = 2 * std::max(3, std::max(h->opts.npre, h->opts.npost));
+      RegionBuild rb;
+      if (rdepth <= kRegionMaxDepth && build_region_tiles(L, rdepth, rb)) {
+        if ((rc = upload(h, &D.region, rb.tiles))) return rc;
+        if ((rc = upload(h, &D.rpatch, rb.patches))) return rc;
+        if ((rc = upload(h, &D.rcell, rb.cells))) return rc;
+        D.nregion = (int)rb.tiles.size();
+        D.rdepth = rdepth;
+        D.rslots = rb.max_slots;
+      }
+    }
+  }
+  if (D.nsweep || D.stream || D.nregion) {  // the sweeps' and region relax's second copy of Q
     std::vector<double> zq(L.nq, 0.0);
     if ((rc = upload_cast(h, &D.Qs, zq))) return rc;
   }
@@ -1105,9 +1380,25 @@ struct Seq {
   // correction prepared the level for assembled ghosts (correct(k, true));
   // cf_current: the level was filled after its parent last changed, so the
   // parent parts that fill stored are those of the current parent
+  // the whole relax call runs as one region launch (plus the fill that
+  // follows every relax call)
+  bool region(int k, int nsweeps) const {
+    const LevelDev<T>& Lk = L[k - 1];
+    return nsweeps > 0 && Lk.nregion > 0 && 2 * nsweeps <= Lk.rdepth && h->opts.smoother == 0;
+  }
   void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false) {
     const bool qs_current = qs_ready == k;
     qs_ready = 0;
+    if (region(k, nsweeps)) {
+      // pass 1 reads the frame when the reference's first pass does (the
+      // preceding fill ran), else assembles its ghosts like every later pass.
+      // The parent parts in aux are current after a fill that followed the
+      // parent's last change or a preparing correction; else refresh them
+      if (first_inline == 1 || (first_inline == 0 && !cf_current)) launch_refresh_cf<T>(lev(k), par(k), s);
+      launch_relax_region<T>(lev(k), par(k), nsweeps, first_inline == 0 ? 1 : 0, (T)h->opts.omega, homog, s);
+      launch_fill_copy<T>(lev(k), par(k), homog, lev(k).Qs, s);
+      return;
+    }
     if (nsweeps > 0 && h->opts.smoother == 1 && tiled_fits<T>(lev(k))) {
       // tiled smoother: pass 1 reads the materialised ghost frame
       if (first_inline) fill(k);
@@ -1273,6 +1564,35 @@ void enqueue_residual(bsam_hier* h, float* const* rout, hipStream_t s) {
 }
 
 }  // namespace
+
+// the region tables of a level as bsam_export(k, 5) returns them:
+// [ntile, rdepth, then per tile cx0 cy0 cw ch np nload nred nblack,
+//  (pid ox oy) x np, the nload + nred + nblack cell entries]; [0, 0] when the
+// level takes other smoother forms
+static void region_export(const bsam_hier* h, const HLevel& L, std::vector<long long>& v) {
+  v.assign(2, 0);
+  const int rdepth = 2 * std::max(3, std::max(h->opts.npre, h->opts.npost));
+  const bool stream = L.np == 1 && L.pi[0].mx % 2 == 0 && L.pi[0].bc[0] && L.pi[0].bc[1] && L.pi[0].bc[2] &&
+                      L.pi[0].bc[3] && (long long)L.pi[0].mx * L.pi[0].my >= kStreamMinCells;
+  RegionBuild rb;
+  if (stream || !face_records_arith(L) || rdepth > kRegionMaxDepth || !build_region_tiles(L, rdepth, rb)) return;
+  v[0] = (long long)rb.tiles.size();
+  v[1] = rdepth;
+  const int per_colour = kRegionThreads * kRegionK;
+  for (size_t x = 0; x < rb.tiles.size(); ++x) {
+    const RegionTile& t = rb.tiles[x];
+    const long long hd[8] = {t.cx0, t.cy0, t.cw, t.ch, t.np, t.nload, t.nred, t.nblack};
+    v.insert(v.end(), hd, hd + 8);
+    for (int q = 0; q < t.np; ++q) {
+      const RegionPatch& rp = rb.patches[x * kRegionMaxPatches + q];
+      v.push_back(rb.pid[x * kRegionMaxPatches + q]); v.push_back(rp.ox); v.push_back(rp.oy);
+    }
+    const unsigned* c = rb.cells.data() + x * kRegionStride;
+    for (int e = 0; e < t.nload; ++e) v.push_back(c[2 * per_colour + e]);
+    for (int e = 0; e < t.nred; ++e) v.push_back(c[e]);
+    for (int e = 0; e < t.nblack; ++e) v.push_back(c[per_colour + e]);
+  }
+}
 
 // ============================================================== C ABI
 extern "C" {
@@ -1724,8 +2044,8 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
   }
   if (h->opts.npre < 0 || h->opts.npost < 0 || h->opts.max_vcycles < 0)
     return fail(BSAM_E_ARG, "bsamgpu:arg: negative sweep or cycle count");
-  if (h->opts.smoother < 0 || h->opts.smoother > 3)
-    return fail(BSAM_E_ARG, "bsamgpu:arg: smoother must be 0, 1, 2 or 3");
+  if (h->opts.smoother < 0 || h->opts.smoother > 4)
+    return fail(BSAM_E_ARG, "bsamgpu:arg: smoother must be 0, 1, 2, 3 or 4");
   if (int rc = check_stream_rows(h->opts.stream_rows)) return rc;
   const bool mixed = h->opts.precision == BSAM_MIXED;
   if (mixed)
@@ -2084,6 +2404,11 @@ long long bsam_export_size(const bsam_hier* h, int k, int what) {
     case 2: return 2 * (long long)L.corr.size();
     case 3: return 1;
     case 4: return 1;
+    case 5: {
+      std::vector<long long> v;
+      region_export(h, L, v);
+      return (long long)v.size();
+    }
     default: return -1;
   }
 }
@@ -2118,6 +2443,12 @@ int bsam_export(const bsam_hier* h, int k, int what, long long* out) {
       out[0] = (face_records_arith(L) && even && L.np > 0 && build_sweep_tiles(L, st)) ? (long long)st.size() : 0;
       break;
     }
+    case 5: {
+      std::vector<long long> v;
+      region_export(h, L, v);
+      for (long long i = 0; i < n; ++i) out[i] = v[(size_t)i];
+      break;
+    }
   }
   return 0;
 }
@@ -2133,6 +2464,13 @@ int bsam_hier_reset(bsam_hier* h) {
   return sync(h);
 }
 
+// level k runs its relax calls as region launches under the current options
+static bool region_level(const bsam_hier* h, int k) {
+  if ((int)h->d64.size() < k || k < 2) return false;
+  const LevelDev<double>& D = h->d64[k - 1];
+  return h->opts.smoother == 0 && D.nregion > 0 && 2 * std::max(h->opts.npre, h->opts.npost) <= D.rdepth;
+}
+
 double bsam_smoother_bytes(const bsam_hier* h, int k, int elem_bytes) {
   if (!h || k < 1 || k > h->nlev) return -1;
   double b = 0;
@@ -2140,9 +2478,11 @@ double bsam_smoother_bytes(const bsam_hier* h, int k, int elem_bytes) {
     const double mx = P.mx, my = P.my;
     b += (mx + 2) * (my + 2) + 3 * mx * my + (mx + 1) * my + mx * (my + 1);
   }
-  // a fused sweep launch does the whole red+black sweep, a colour pass half
+  // a fused sweep launch does the whole red+black sweep, a colour pass half;
+  // a region launch (a whole relax call) reads and writes the level once
   const bool sweep = (int)h->d64.size() >= k && ((h->d64[k - 1].stream && h->opts.smoother != 3) ||
-                                                  (h->opts.smoother == 2 && h->d64[k - 1].nsweep > 0));
+                                                  (h->opts.smoother == 2 && h->d64[k - 1].nsweep > 0) ||
+                                                  region_level(h, k));
   return sweep ? b * elem_bytes : b * elem_bytes / 2.0;
 }
 
@@ -2163,7 +2503,17 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
   HIPCHK(hipEventRecord(e0, h->stream));
   const bool stream = h->d64[k - 1].stream && h->opts.smoother != 3;
   const bool sweep = stream || (h->opts.smoother == 2 && h->d64[k - 1].nsweep > 0);
+  const bool region = region_level(h, k);
   for (int i = 0; i < nlaunch; ++i) {
+    if (region) {  // one relax call of npre sweeps per launch, Q -> Qs
+      if (precision == BSAM_MIXED)
+        launch_relax_region<float>(h->d32[k - 1], k > 1 ? &h->d32[k - 2] : nullptr, h->opts.npre, 0,
+                                   (float)h->opts.omega, 1, h->stream);
+      else
+        launch_relax_region<double>(h->d64[k - 1], k > 1 ? &h->d64[k - 2] : nullptr, h->opts.npre, 0,
+                                    h->opts.omega, 0, h->stream);
+      continue;
+    }
     if (stream) {  // streaming sweeps, Q and Qs in turn
       const PatchInfo& P = h->H[k - 1].pi[0];
       if (precision == BSAM_MIXED) {

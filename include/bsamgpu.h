@@ -94,8 +94,12 @@ typedef struct {
   int use_graph;       /* replay each cycle as a captured HIP graph */
   int coarse_cells;    /* levels 1..kc with at most this many cells each run their
                           V-cycle inside one workgroup (0: default 1100, -1: off) */
-  int smoother;        /* 0: red-black Gauss-Seidel exactly as ebRelaxLevel (one launch
-                          per colour, bit-identical to the reference sequence);
+  int smoother;        /* 0: red-black Gauss-Seidel exactly as ebRelaxLevel, bit-identical
+                          to the reference sequence: on levels with region tiles a
+                          whole relax call in one launch (LDS canvas per tile with
+                          the halo its passes depend on, coefficients in registers,
+                          a barrier between colour passes), elsewhere one launch per
+                          colour;
                           1: LDS-tiled -- every colour pass of a relax call in one
                           launch, one patch per workgroup, sibling ghosts frozen
                           for the call (same converged solution);
@@ -104,8 +108,9 @@ typedef struct {
                           pair of red+black sweeps runs as fused sweep launches
                           (red half in LDS, black half from it), alternating Q
                           and a second copy of Q;
-                          3: colour passes on every level.
-                          Under 0 and 2, a level that is one patch of even width
+                          3: colour passes on every level;
+                          4: as 0 without region launches (colour passes).
+                          Under 0, 2 and 4, a level that is one patch of even width
                           with physical sides only and >= 2^20 cells runs pairs of
                           sweeps as streaming sweep launches (one pass over the
                           rows per red+black sweep), also bit-identical */
@@ -275,7 +280,12 @@ int bsam_set_level(bsam_hier *h, int k, int field, const double *in);
  *           standard interpolation stencil and sibling push target, so that
  *           smoother passes after the first assemble ghosts from neighbours
  * what = 4: one value, the number of fused red+black sweep tiles of level k
- *           (0: the level's relax calls run single-colour passes) */
+ *           (0: the level's relax calls run single-colour passes)
+ * what = 5: the region tiles of level k (whole relax calls per launch):
+ *           [ntile, passes covered, then per tile: canvas x0 y0 width height,
+ *           patches np, depth-0 cells, colour-1 cells, colour-2 cells,
+ *           (patch ox oy) x np, the cell entries: patch slot (bits 0-3) |
+ *           i (4-15) | j (16-27) | depth (28-31)]; [0, 0] when none */
 long long bsam_export_size(const bsam_hier *h, int k, int what);
 int bsam_export(const bsam_hier *h, int k, int what, long long *out);
 
