@@ -506,7 +506,15 @@ def main():
     log(f"mixed: {smx['vcycles']} V-cycles, rel {smx['finalRel']:.3e}, {totmx / args.steps * 1e3:.2f} ms/solve, "
         f"rel L2 vs fp64 {err_mx:.2e}")
 
-    # dominant kernel: finest-level smoother, HIP events on the library stream
+    # dominant kernel: the finest level's smoother launch in the form the
+    # solve uses, HIP events on the library stream.  A region launch runs a
+    # whole relax call (NPre red+black sweeps) over one pass of the level's
+    # data; a colour pass is half a sweep.
+    G.reset()
+    G.solve_cycles(smoother=args.smoother, coarse_cells=args.coarse_cells)  # the options the timing uses
+    sm_bytes = G.smoother_bytes(kf, 8)
+    region = args.smoother == "exact" and int(G.export(kf, 5)[0]) > 0
+    sweeps = opts.NPre if region else 0.5
     G.time_smoother(kf, 20)
     ms64 = G.time_smoother(kf, args.smoother_launches)
     G.time_smoother(kf, 20, mixed=True)
@@ -514,10 +522,20 @@ def main():
     gbs64 = sm_bytes / ms64 / 1e6
     gbs32 = G.smoother_bytes(kf, 4) / ms32 / 1e6
     per_level = {}
+    per_level_us = {}
     for k in range(2, H.nlev + 1):
         msk = G.time_smoother(k, 100)
         per_level[str(k)] = round(G.smoother_bytes(k, 8) / msk / 1e6, 1)
-    log(f"smoother fp64 {ms64 * 1e3:.2f} us/launch {gbs64:.0f} GB/s; fp32 {ms32 * 1e3:.2f} us {gbs32:.0f} GB/s")
+        per_level_us[str(k)] = round(msk * 1e3, 2)
+    # the colour-pass form on the same level, for comparison
+    G.reset()
+    G.solve_cycles(smoother="classic", coarse_cells=args.coarse_cells)
+    G.time_smoother(kf, 20)
+    ms_pass = G.time_smoother(kf, args.smoother_launches)
+    pass_bytes = G.smoother_bytes(kf, 8)
+    relax_call_pass_us = ms_pass * 1e3 * 2 * opts.NPre
+    log(f"smoother fp64 {ms64 * 1e3:.2f} us/launch {gbs64:.0f} GB/s; fp32 {ms32 * 1e3:.2f} us {gbs32:.0f} GB/s; "
+        f"colour pass {ms_pass * 1e3:.2f} us ({pass_bytes / ms_pass / 1e6:.0f} GB/s)")
 
     large = None
     if args.large_smoother:
@@ -576,6 +594,7 @@ def main():
         "p99_ms": round(pct(lat64, 99), 3),
         "smoother_GBps": round(gbs64, 1),
         "smoother_GBps_per_level": per_level,
+        "smoother_us_per_level": per_level_us,
         "smoother_uniform_level": large,
         "mixed": {"value": round(args.steps / totmx, 3), "ms_per_step": round(totmx / args.steps * 1e3, 3),
                   "vcycles": smx["vcycles"], "final_rel": smx["finalRel"],
@@ -584,11 +603,20 @@ def main():
                   "smoother_fp32_GBps": round(gbs32, 1)},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs64 / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic("k_relax_c_starfish_level7"),
-                     "kernel": f"k_relax_c<double,10> (colour pass, neighbour-assembled ghosts + sibling push: "
-                               f"passes 2..2n-1 of a relax call) level {kf} ({H.L(kf).np} patches, "
+                     "traffic": pmc_traffic("k_relax_region_starfish_level7" if region
+                                            else "k_relax_c_starfish_level7"),
+                     "kernel": (f"k_relax_region<double,1024,1,1> (a whole relax call: {opts.NPre} red+black sweeps "
+                                f"per launch on LDS canvases)" if region else
+                                "k_relax_c<double,10> (colour pass, neighbour-assembled ghosts + sibling push)")
+                               + f" level {kf} ({H.L(kf).np} patches, "
                                f"{sum(H.L(kf).dims(p)[0] * H.L(kf).dims(p)[1] for p in range(H.L(kf).np))} cells)",
-                     "bytes_per_launch": sm_bytes, "us_per_launch": round(ms64 * 1e3, 3)},
+                     "bytes_per_launch": sm_bytes, "us_per_launch": round(ms64 * 1e3, 3),
+                     "sweeps_per_launch": sweeps,
+                     # the algorithmic bytes of the sweeps a launch performs, per second
+                     "sweep_bytes_rate_GBps": round(sweeps * gbs64 if region else gbs64, 1),
+                     "colour_pass_us": round(ms_pass * 1e3, 3),
+                     "relax_call_us": {"launch_form": round(ms64 * 1e3 * (1 if region else 2 * opts.NPre), 2),
+                                       "colour_passes": round(relax_call_pass_us, 2)}},
         "cpu_baseline": base,
         "varcoef_refined": varc,
         "parabolic": parab,

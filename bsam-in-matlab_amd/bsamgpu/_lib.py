@@ -45,10 +45,25 @@ EXPORTS = [
     "bsam_parabolic_rhs", "bsam_parabolic_set_shift", "bsam_parabolic_advance", "bsam_parabolic_step", "bsam_batch_create",
     "bsam_batch_destroy", "bsam_batch_step", "bsam_tag_level", "bsam_tag_level_static", "bsam_polygon_fields",
     "bsam_hier_inherit", "bsam_regrid_level", "bsam_parabolic_uo_from_q", "bsam_cluster_tags", "bsam_cluster_tags_per_parent",
+    "bsam_build_create", "bsam_build_destroy", "bsam_build_add_level", "bsam_build_desc", "bsam_build_info",
+    "bsam_build_allowed",
 ]
 
 BSAM_BE, BSAM_BDF2, BSAM_CN = 0, 1, 2
 BSAM_TAG_ULAP, BSAM_TAG_RTE = 0, 1
+
+
+FIELD_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                            ctypes.c_longlong, ctypes.POINTER(ctypes.c_double))
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("domain", c_double * 4), ("D", FIELD_FN), ("C", FIELD_FN), ("f", FIELD_FN),
+                ("bc_type", c_int * 4), ("g", FIELD_FN * 4), ("u0", FIELD_FN), ("user", ctypes.c_void_p)]
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [("base_cells", c_int * 2), ("coarsest_cells", c_int), ("source_quad", c_int)]
 
 
 class HierDesc(ctypes.Structure):
@@ -142,6 +157,12 @@ def load(path=None):
         "bsam_parabolic_uo_from_q": (c_int, [vp]),
         "bsam_polygon_fields": (c_int, [c_int, P_double, P_double, c_int, P_double, P_double, c_ll, c_double,
                                         P_double, P_double, P_double]),
+        "bsam_build_create": (c_int, [ctypes.POINTER(Problem), ctypes.POINTER(BuildOpts), ctypes.POINTER(vp)]),
+        "bsam_build_destroy": (None, [vp]),
+        "bsam_build_add_level": (c_int, [vp, P_int, c_int]),
+        "bsam_build_desc": (c_int, [vp, ctypes.POINTER(HierDesc)]),
+        "bsam_build_info": (c_int, [vp, P_int, P_int, P_int]),
+        "bsam_build_allowed": (c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -1594,6 +1594,9 @@ static void region_export(const bsam_hier* h, const HLevel& L, std::vector<long 
   }
 }
 
+// the thread's last error message, for the host-only sources of the library
+int bsam_set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
 // ============================================================== C ABI
 extern "C" {
 

@@ -230,6 +230,47 @@ int bsam_cluster_tags_per_parent(const unsigned char *tags, const unsigned char 
                                  const int *parents, int np, int min_width, double fill_ratio, int *boxes,
                                  int cap, int *nboxes, int *ndropped);
 
+/* Native hierarchy construction (host code, needs no device): the sub-root
+ * ladder and root of EllipticBSAM/src/ebSetup.m:10-68, level sampling
+ * (ebMakeLevel.m:1-105: D at the x- and y-face midpoints, C and f at the cell
+ * centres, the boundary data at the face midpoints of the domain sides), the
+ * coefficient fill-down (ebCoeffFilldown.m:1-62), the folded inverse
+ * diagonal (ebKernels.m:28-34, ebBcFold.m:12-42), the source projection of
+ * a singular all-Neumann, C = 0 problem (ebCompatProject.m:1-111) and the
+ * refinement of parent-space boxes into a new finest level
+ * (ebAddLevel.m:10-24).  bsam_build_desc gives the hierarchy value that
+ * bsam_hier_create takes (arrays owned by the builder, valid until the next
+ * call on it); the new level's prolonged initial guess is
+ * bsam_init_level_prolong on the device hierarchy (ebAddLevel.m:25-32), after
+ * bsam_hier_inherit from the previous one.  The coefficient callbacks receive
+ * n points (x[i], y[i]) and write n values; non-finite values give
+ * BSAM_E_NONFINITE, D <= 0 gives BSAM_E_D, a bad box BSAM_E_BOX.  Only the
+ * midpoint source (SourceQuad 1) is built natively. */
+typedef void (*bsam_field_fn)(void *user, const double *x, const double *y, long long n, double *out);
+typedef struct {
+  double domain[4];    /* [xmin xmax ymin ymax] */
+  bsam_field_fn D, C, f;
+  int bc_type[4];      /* W E S N: 1 Dirichlet, 2 Neumann */
+  bsam_field_fn g[4];  /* boundary data per side (NULL: 0) */
+  bsam_field_fn u0;    /* InitialGuess at the cell centres of the ladder and root (NULL: 0) */
+  void *user;          /* passed to every callback */
+} bsam_problem;
+typedef struct {
+  int base_cells[2];   /* BaseCells */
+  int coarsest_cells;  /* CoarsestCells (16) */
+  int source_quad;     /* SourceQuad: 1 (midpoint) */
+} bsam_build_opts;
+typedef struct bsam_build bsam_build;
+int bsam_build_create(const bsam_problem *prob, const bsam_build_opts *opts, bsam_build **out);
+void bsam_build_destroy(bsam_build *b);
+/* boxes: nboxes rows [ilo ihi jlo jhi] in the finest level's index space (1-based) */
+int bsam_build_add_level(bsam_build *b, const int *boxes, int nboxes);
+int bsam_build_desc(bsam_build *b, bsam_hier_desc *out);
+int bsam_build_info(const bsam_build *b, int *nlev, int *root_idx, int *singular);
+/* the finest level's allowed map for clustering (erode8 of its coverage,
+ * EllipticBSAM/src/ebTagCluster.m:130-141): n1*n2 bytes, column-major */
+int bsam_build_allowed(const bsam_build *b, unsigned char *out);
+
 /* Diffuse-domain geometry on device `device`: for npts points (x, y), the
    signed distance d (positive inside) to the closed, radially ordered polygon
    of nv vertices (vx, vy; the last repeats the first), as
