@@ -12,7 +12,8 @@ Reported: solves/s (value), seconds per V-cycle, p50/p99 solve latency, the
 smoother's achieved bandwidth against 8 TB/s (roofline of the dominant kernel,
 the finest-level smoother, timed with HIP events on the library stream), the
 FP32-inner / FP64-defect-correction form with its error against the FP64
-solution, and the CPU oracle timed on this host as the baseline.
+solution, and the C++ restatement of the solve stage timed on this host (one
+thread and every core) as the baseline.
 
   python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
   python bench.py --kernel-only      # only the timed smoother launches (for rocprofv3)
@@ -99,17 +100,40 @@ def visible_rel_l2(H, Qa, Qb):
     return (num / den) ** 0.5
 
 
-def cpu_baseline(H):
+def cpu_baseline(H, cores, gpu_hist=None, seconds=6.0):
+    """The C++ restatement of the solve stage (oracle/bsam_cpu.cpp, -O3,
+    OpenMP over patches) timed on this host: whole starfish solves repeated
+    for about ``seconds`` each at one thread and at ``cores`` threads.  Its
+    residual history is bit-identical to the NumPy oracle's and is checked
+    against the device's FP64 history."""
     from oracle import bsam_oracle as O
+    from oracle import bsam_cpu
 
-    Orc = O.OracleHierarchy(H)
-    t = time.perf_counter()
-    st = Orc.solve_cycles()
-    dt = time.perf_counter() - t
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"one full starfish solve by the numpy oracle ({st['vcycles']} V-cycles to rel "
-                      f"{st['finalRel']:.2e}) = {dt:.1f} s, single thread, tables prebuilt",
-            "seconds_per_solve": dt, "seconds_per_vcycle": dt / max(st["vcycles"], 1)}
+    c = bsam_cpu.CpuSolve(O.OracleHierarchy(H))
+    out = {}
+    for label, th in (("single", 1), ("all", cores)):
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            c.reset()
+            hist, it, rel = c.solve(threads=th)
+            n += 1
+            dt = time.perf_counter() - t0
+            if dt >= seconds:
+                break
+        out[label] = {"solves": n, "seconds_per_solve": dt / n, "vcycles": it, "final_rel": rel}
+    same = None if gpu_hist is None else bool(np.array_equal(np.asarray(gpu_hist), hist))
+    a, s1 = out["all"], out["single"]
+    return {"value": round(1.0 / a["seconds_per_solve"], 3), "unit": "solves/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{a['solves']} whole starfish solves ({a['vcycles']} V-cycles to rel {a['final_rel']:.2e}) "
+                      f"by the C++ restatement of the solve stage on {cores} threads, "
+                      f"{a['seconds_per_solve'] * 1e3:.1f} ms each; tables prebuilt",
+            "seconds_per_solve": a["seconds_per_solve"],
+            "seconds_per_vcycle": a["seconds_per_solve"] / max(a["vcycles"], 1),
+            "single_thread": {"value": round(1.0 / s1["seconds_per_solve"], 3), "cores": 1,
+                              "solves": s1["solves"], "seconds_per_solve": s1["seconds_per_solve"]},
+            "residual_history_equals_device_fp64": same}
 
 
 def bench_large_smoother(n, launches=40, solves=True):
@@ -412,6 +436,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    # host threads for the CPU baseline: the process's CPU share (16 per GPU
+    # on a shared node) unless OMP_NUM_THREADS says otherwise
+    host_threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or min(16, len(os.sched_getaffinity(0)))
     os.environ.setdefault("OMP_NUM_THREADS", "1")
 
     if args.copy_calibration:
@@ -562,9 +589,10 @@ def main():
 
     base = None
     if not args.no_cpu_baseline:
-        log("CPU baseline (numpy oracle, one solve)")
-        base = cpu_baseline(H)
-        log(f"cpu: {base['seconds_per_solve']:.1f} s/solve")
+        log("CPU baseline (C++ restatement of the solve stage, 1 thread and all cores)")
+        base = cpu_baseline(H, host_threads, s64["resHist"])
+        log(f"cpu: {base['seconds_per_solve'] * 1e3:.1f} ms/solve on {base['cores']} threads, "
+            f"{base['single_thread']['seconds_per_solve'] * 1e3:.1f} ms on one")
 
     ms_step = tot64 / args.steps * 1e3
     out = {
