@@ -46,6 +46,12 @@ void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int ns
                          hipStream_t s);
 template <typename T>
 void launch_fill_copy(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, const T* Qsrc, hipStream_t s);
+// the fill after a region relax (parent parts from aux, no QC ring refresh)
+// and the restriction into the parent (also into dup when given), reading the
+// new interior from Qsrc, in one launch
+template <typename T>
+void launch_fill_copy_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int homog, const T* Qsrc, T* dup,
+                               hipStream_t s);
 // aux parent parts of every coarse-fine record of the level from the current parent
 template <typename T> void launch_refresh_cf(const LevelDev<T>& L, const LevelDev<T>* parent, hipStream_t s);
 // all passes of one relax call, one patch per workgroup staged in LDS

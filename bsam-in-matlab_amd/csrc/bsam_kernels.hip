@@ -103,6 +103,33 @@ __device__ __forceinline__ T ghost_value(const GhostRec& r, const T* __restrict_
   }
 }
 
+// ghost_value with the parent part of a coarse-fine record taken from its
+// stored copy (the same number cf_part computes from the same parent)
+template <typename T>
+__device__ __forceinline__ T ghost_value_cf(const GhostRec& r, const T* __restrict__ Q, T cf, int homog) {
+  switch (r.kind) {
+    case G_COPY:
+      return Q[r.s[0]];
+    case G_QIF:
+    case G_QIF_END:
+      return (T(A1_) * Q[r.s[0]] + T(A2_) * Q[r.s[1]]) + cf;
+    case G_LIF:
+      return (cf + T(C23_) * Q[r.s[0]]) - T(C13_) * Q[r.s[1]];
+    case G_LIF_CORNER:
+      return cf + T(C13_) * Q[r.s[0]];
+    case G_DIR: {
+      const T g = homog ? T(0) : T(r.c);
+      return (T(C83_) * g - T(2) * Q[r.s[0]]) + T(C13_) * Q[r.s[1]];
+    }
+    case G_NEU: {
+      const T hg = homog ? T(0) : T(r.c);
+      return Q[r.s[0]] + hg;
+    }
+    default:
+      return T(0);
+  }
+}
+
 // item t of ebFillGhosts: t < nghost -> one ghost cell; then (when the level
 // has a parent) one QC ring refresh of a patch with coarse-fine faces.  A
 // coarse-fine record also stores its parent part in L.aux for the smoother
@@ -837,6 +864,50 @@ __global__ void __launch_bounds__(kTile) k_fill_copy(LevelDev<T> L, const T* __r
   L.Q[qc] = Qsrc[qc];
 }
 
+// The fill that follows a region relax and the restriction that follows the
+// pre-smoothing (ebVcycle.m:21-22) as one launch.  Nothing in it reads what
+// it writes: the ghosts read the new interior from Qsrc and the parent parts
+// of coarse-fine records from aux (current: the region launch read them and
+// the parent has not changed since), the restriction reads Qsrc and writes
+// the parent and QC's interior, and the interior copy writes Q.  The fill's
+// QC ring refresh is left out: the load that follows the restriction gathers
+// the ring again from the restricted parent before anything reads it.
+// Blocks: [0, nfb) ghosts, then one per 256-cell tile (copy), then the
+// restriction's footprint cells.
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_fill_copy_restrict(LevelDev<T> L, int homog, const T* __restrict__ Qsrc,
+                                                              int nfb, T* __restrict__ Pq, T* __restrict__ Pdup) {
+  const int b = blockIdx.x;
+  if (b < nfb) {
+    const int t = b * kTile + threadIdx.x;
+    if (t >= L.nghost) return;
+    const GhostRec r = L.ghost[t];
+    T v;
+    if (r.kind == G_PCORNER) {
+      const T a = ghost_value_cf<T>(L.ghost[r.s[0]], Qsrc, L.aux[r.s[0]].cf, homog);
+      const T c = ghost_value_cf<T>(L.ghost[r.s[1]], Qsrc, L.aux[r.s[1]].cf, homog);
+      v = (a + c) - Qsrc[r.s[2]];
+    } else {
+      v = ghost_value_cf<T>(r, Qsrc, is_cf_kind(r.kind) ? L.aux[t].cf : T(0), homog);
+    }
+    L.Q[r.dst] = v;
+    return;
+  }
+  if (b < nfb + L.ntile) {
+    const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane(b - nfb));
+    const PatchInfo& pi = tr.pi;
+    const int c = tr.start + threadIdx.x;
+    if (c >= pi.mx * pi.my) return;
+    const int j = cell_row(c, pi.mx, pi.magic);
+    const int i = c - j * pi.mx;
+    const int qc = pi.qoff + (i + 1) + (j + 1) * (pi.mx + 2);
+    L.Q[qc] = Qsrc[qc];
+    return;
+  }
+  const int t = (b - nfb - L.ntile) * kTile + threadIdx.x;
+  if (t < L.nfoot) restrict_item<T>(L, Pq, t, 0, Pdup, Qsrc);
+}
+
 // the parent part of every coarse-fine record of a level from the current
 // parent, into aux (what a fill stores, without touching Q)
 template <typename T>
@@ -1162,11 +1233,12 @@ __device__ __forceinline__ void ring_item(const LevelDev<T>& L, const T* __restr
 
 template <typename T>
 __device__ __forceinline__ void restrict_item(const LevelDev<T>& L, T* __restrict__ Pq, int t,
-                                              int qc_only, T* __restrict__ Pdup = nullptr) {
+                                              int qc_only, T* __restrict__ Pdup = nullptr,
+                                              const T* __restrict__ Qsrc = nullptr) {
   const FootRec fr = L.foot[t];
   const FootRec& pi = fr;  // patch offsets carried inline
   const int ldq = pi.mx + 2;
-  const T* __restrict__ Q = L.Q;
+  const T* __restrict__ Q = Qsrc ? Qsrc : L.Q;
   const int q00 = pi.qoff + (2 * fr.ci + 1) + (2 * fr.cj + 1) * ldq;
   const T R = T(0.25) * (((Q[q00] + Q[q00 + 1]) + Q[q00 + ldq]) + Q[q00 + ldq + 1]);
   L.QC[pi.qcoff + (fr.ci + 1) + (fr.cj + 1) * (pi.mx / 2 + 2)] = R;
@@ -2282,6 +2354,15 @@ void launch_fill_copy(const LevelDev<T>& L, const LevelDev<T>* parent, int homog
 }
 
 template <typename T>
+void launch_fill_copy_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int homog, const T* Qsrc, T* dup,
+                               hipStream_t s) {
+  const int nfb = (int)nblk(L.nghost);
+  const int nb = nfb + L.ntile + (int)nblk(L.nfoot);
+  if (!nb) return;
+  hipLaunchKernelGGL(k_fill_copy_restrict<T>, dim3(nb), dim3(kTile), 0, s, L, homog, Qsrc, nfb, parent.Q, dup);
+}
+
+template <typename T>
 void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s) {
   if (!L.nring_all) return;
   hipLaunchKernelGGL(k_ring<T>, dim3(nblk(L.nring_all)), dim3(kTile), 0, s, L, (const T*)parent.Q);
@@ -2638,6 +2719,8 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_fill<T>(const LevelDev<T>&, const LevelDev<T>*, int, hipStream_t);               \
   template void launch_relax_region<T>(const LevelDev<T>&, const LevelDev<T>*, int, int, T, int, hipStream_t); \
   template void launch_fill_copy<T>(const LevelDev<T>&, const LevelDev<T>*, int, const T*, hipStream_t); \
+  template void launch_fill_copy_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, const T*, T*,        \
+                                             hipStream_t);                                                  \
   template void launch_refresh_cf<T>(const LevelDev<T>&, const LevelDev<T>*, hipStream_t);              \
   template void launch_ring<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                    \
   template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, T*);       \

@@ -508,10 +508,11 @@ bool build_sweep_tiles(const HLevel& L, std::vector<SweepTile>& out) {
 // sibling cell a copy face ghost copies, the inward neighbour of a Dirichlet
 // side, and the interpolation sources of a coarse-fine face (the record's
 // cells, which face_records_arith has checked to follow the fixed stencil),
-// plus the cell itself.  A tile owns a rectangle of one patch; walking the
-// passes of a call backwards from its own cells gives each needed cell its
-// depth (region_code), and the canvas is the bounding box of those cells
-// with one slot of margin, so that every read of a pass stays in it.
+// plus the cell itself.  A tile owns a set of cells (pieces of one or more
+// patches); walking the passes of a call backwards from its own cells gives
+// each needed cell its depth (region_code), and the canvas is the bounding
+// box of those cells with one slot of margin, so that every read of a pass
+// stays in it.
 struct RegionBuild {
   std::vector<RegionTile> tiles;
   std::vector<RegionPatch> patches;  // kRegionMaxPatches per tile
@@ -590,19 +591,13 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
     }
     return true;
   };
-  // owner patch and local position of a cell id
-  auto locate = [&](long long c, int& p, int& i, int& j) {
-    const int gi = (int)(c % n0), gj = (int)(c / n0);
-    p = own.at(gi + 1, gj + 1) - 1;
-    i = gi - (L.box[p].ilo - 1);
-    j = gj - (L.box[p].jlo - 1);
-  };
   // the dependencies of every cell, once (CSR over the level's cells in
   // patch order, entries as global ids)
   const long long nglob = (long long)n0 * L.n[1];
   std::vector<int> cell_ix((size_t)nglob, -1);  // global id -> level cell index (patch order)
   std::vector<long long> gid;                    // level cell index -> global id
-  std::vector<unsigned char> colour;
+  std::vector<unsigned char> colour, face;
+  std::vector<int> cpatch, ci_, cj_;  // owner patch and local position of each cell
   gid.reserve((size_t)L.ncell);
   for (int p = 0; p < L.np; ++p)
     for (int j = 0; j < L.pi[p].my; ++j)
@@ -611,6 +606,10 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
         cell_ix[(size_t)g] = (int)gid.size();
         gid.push_back(g);
         colour.push_back((unsigned char)((i + j + L.pi[p].par0) & 1));
+        face.push_back((unsigned char)(i == 0 || j == 0 || i == L.pi[p].mx - 1 || j == L.pi[p].my - 1));
+        cpatch.push_back(p);
+        ci_.push_back(i);
+        cj_.push_back(j);
       }
   const int ncell = (int)gid.size();
   std::vector<int> dptr(ncell + 1, 0), dlist;
@@ -625,17 +624,23 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
           dptr[x + 1] = (int)dlist.size();
         }
   }
-  std::vector<int> depth_of(ncell, -1);  // reset after each tile through `order`
-  // one tile: own rectangle [i0, i1) x [j0, j1) of patch p
-  auto make_tile = [&](int p, int i0, int i1, int j0, int j1, bool& fits) -> bool {
-    fits = false;
+  std::vector<int> depth_of(ncell, -1);  // reset after each draft through `order`
+  // a tile drafted from a set of own cells (level cell indices)
+  struct Draft {
+    std::vector<int> own;
+    int cx0 = 0, cy0 = 0, cw = 0, ch = 0;
+    std::vector<int> pl;  // region patches
+    std::vector<unsigned> load, red, black;
+    std::vector<int> rcell, rdep;  // the region's cells and their depths
+  };
+  const int per_colour = kRegionThreads * kRegionK, per_load = kRegionThreads * kRegionKL;
+  // false when the region does not fit a workgroup
+  auto draft = [&](Draft& D) -> bool {
     std::vector<int> order;  // level cell indices of the region, in discovery order
-    for (int j = j0; j < j1; ++j)
-      for (int i = i0; i < i1; ++i) {
-        const int c = cell_ix[(size_t)cell_of(p, i, j)];
-        depth_of[c] = rdepth;
-        order.push_back(c);
-      }
+    for (int c : D.own) {
+      depth_of[c] = rdepth;
+      order.push_back(c);
+    }
     // pass q updates colour (q - 1) & 1 (0: colour 1)
     for (int q = rdepth; q >= 1; --q) {
       const int col = (q - 1) & 1;
@@ -661,85 +666,56 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
       gx0 = std::min(gx0, gi); gx1 = std::max(gx1, gi);
       gy0 = std::min(gy0, gj); gy1 = std::max(gy1, gj);
     }
-    const int cx0 = gx0 - 1, cy0 = gy0 - 1, cw = gx1 - gx0 + 3, ch = gy1 - gy0 + 3;
+    D.cx0 = gx0 - 1; D.cy0 = gy0 - 1; D.cw = gx1 - gx0 + 3; D.ch = gy1 - gy0 + 3;
     // level cell indices run in patch order, row by row: sorting them keeps
     // each list in (patch, j, i) order, the cells off every face of their
     // patch first (their updates take the kernel's short path, wave by wave)
     std::vector<int> sorted(order);
-    auto on_face = [&](int c) {
-      int cp, ci, cj;
-      locate(gid[c], cp, ci, cj);
-      return ci == 0 || cj == 0 || ci == L.pi[cp].mx - 1 || cj == L.pi[cp].my - 1;
-    };
     std::sort(sorted.begin(), sorted.end());
-    std::stable_partition(sorted.begin(), sorted.end(), [&](int c) { return !on_face(c); });
-    std::vector<int> pl;  // region patches
-    std::vector<unsigned> load, red, black;
+    std::stable_partition(sorted.begin(), sorted.end(), [&](int c) { return !face[c]; });
+    D.pl.clear(); D.load.clear(); D.red.clear(); D.black.clear();
+    D.rcell = order;
+    D.rdep.resize(order.size());
+    for (size_t x = 0; x < order.size(); ++x) D.rdep[x] = depth_of[order[x]];
     bool fit = true;
     for (int c : sorted) {
-      int cp, ci, cj;
-      locate(gid[c], cp, ci, cj);
+      const int cp = cpatch[c], ci = ci_[c], cj = cj_[c];
       int ps = -1;
-      for (size_t k = 0; k < pl.size(); ++k)
-        if (pl[k] == cp) ps = (int)k;
+      for (size_t k = 0; k < D.pl.size(); ++k)
+        if (D.pl[k] == cp) ps = (int)k;
       if (ps < 0) {
-        if ((int)pl.size() >= kRegionMaxPatches) { fit = false; break; }
-        pl.push_back(cp);
-        ps = (int)pl.size() - 1;
+        if ((int)D.pl.size() >= kRegionMaxPatches) { fit = false; break; }
+        D.pl.push_back(cp);
+        ps = (int)D.pl.size() - 1;
       }
       const int d = depth_of[c];
       const unsigned code = region_code(ps, ci, cj, d);
-      if (d == 0) load.push_back(code);
-      else if (colour[c] == 0) red.push_back(code);
-      else black.push_back(code);
+      if (d == 0) D.load.push_back(code);
+      else if (colour[c] == 0) D.red.push_back(code);
+      else D.black.push_back(code);
     }
     for (int c : order) depth_of[c] = -1;
-    const int per_colour = kRegionThreads * kRegionK, per_load = kRegionThreads * kRegionKL;
-    if (!fit || (long long)cw * ch > kRegionMaxSlots || (int)red.size() > per_colour ||
-        (int)black.size() > per_colour || (int)load.size() > per_load)
-      return true;  // does not fit: split
-    RegionTile rt;
-    std::memset(&rt, 0, sizeof rt);
-    rt.cx0 = cx0; rt.cy0 = cy0; rt.cw = cw; rt.ch = ch;
-    rt.np = (int)pl.size();
-    rt.nload = (int)load.size(); rt.nred = (int)red.size(); rt.nblack = (int)black.size();
-    for (int k = 0; k < kRegionMaxPatches; ++k) {
-      RegionPatch rp;
-      std::memset(&rp, 0, sizeof rp);
-      if (k < (int)pl.size()) {
-        rp.pi = L.pi[pl[k]];
-        rp.ox = L.box[pl[k]].ilo - 1 - cx0;
-        rp.oy = L.box[pl[k]].jlo - 1 - cy0;
-      }
-      out.patches.push_back(rp);
-      out.pid.push_back(k < (int)pl.size() ? pl[k] : -1);
-    }
-    const size_t base = out.cells.size();
-    out.cells.resize(base + kRegionStride, kRegionNone);
-    std::copy(red.begin(), red.end(), out.cells.begin() + base);
-    std::copy(black.begin(), black.end(), out.cells.begin() + base + per_colour);
-    std::copy(load.begin(), load.end(), out.cells.begin() + base + 2 * per_colour);
-    out.tiles.push_back(rt);
-    out.max_slots = std::max(out.max_slots, cw * ch);
-    fits = true;
-    return true;
+    return fit && (long long)D.cw * D.ch <= kRegionMaxSlots && (int)D.red.size() <= per_colour &&
+           (int)D.black.size() <= per_colour && (int)D.load.size() <= per_load;
   };
-  // patches in order, each halved along its longer side until the region of
-  // every piece fits a workgroup
-  std::vector<std::array<int, 5>> work;
-  for (int p = 0; p < L.np; ++p) work.push_back({p, 0, L.pi[p].mx, 0, L.pi[p].my});
-  for (size_t w = 0; w < work.size(); ++w) {
-    std::array<int, 5> t = work[w];
-    std::vector<std::array<int, 5>> stack{t};
+  // 1. every patch, halved along its longer side until the region of every
+  //    piece fits a workgroup
+  std::vector<Draft> drafts;
+  for (int p = 0; p < L.np; ++p) {
+    std::vector<std::array<int, 5>> stack{{p, 0, L.pi[p].mx, 0, L.pi[p].my}};
     while (!stack.empty()) {
       const std::array<int, 5> r = stack.back();
       stack.pop_back();
-      bool fits = false;
-      if (!make_tile(r[0], r[1], r[2], r[3], r[4], fits)) return false;
-      if (fits) continue;
+      Draft D;
+      for (int j = r[3]; j < r[4]; ++j)
+        for (int i = r[1]; i < r[2]; ++i) D.own.push_back(cell_ix[(size_t)cell_of(r[0], i, j)]);
+      if (draft(D)) {
+        drafts.push_back(std::move(D));
+        continue;
+      }
       const int w0 = r[2] - r[1], h0 = r[4] - r[3];
       if (w0 <= 2 && h0 <= 2) return false;
-      // the second half is pushed first so that tiles stay in spatial order
+      // the second half is pushed first so that pieces stay in spatial order
       if (w0 >= h0) {
         const int m = r[1] + w0 / 2;
         stack.push_back({r[0], m, r[2], r[3], r[4]});
@@ -750,6 +726,117 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
         stack.push_back({r[0], r[1], r[2], r[3], m});
       }
     }
+  }
+  // 2. adjacent pieces merged while the merged region still fits: fewer,
+  //    fuller workgroups, and the cells between merged pieces are computed
+  //    once instead of in both halos.  Smallest first, each with the smallest
+  //    neighbour that fits; passes until nothing merges.
+  std::vector<int> tile_of(ncell, -1);
+  for (size_t t = 0; t < drafts.size(); ++t)
+    for (int c : drafts[t].own) tile_of[c] = (int)t;
+  std::vector<char> alive(drafts.size(), 1);
+  auto neighbours = [&](int t) {
+    std::vector<int> nb;
+    for (int c : drafts[t].own) {
+      const long long g = gid[c];
+      const int gi = (int)(g % n0), gj = (int)(g / n0);
+      const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+      for (int d = 0; d < 4; ++d) {
+        const int ni = gi + di[d], nj = gj + dj[d];
+        if (ni < 0 || nj < 0 || ni >= n0 || nj >= L.n[1]) continue;
+        const int y = cell_ix[(size_t)ni + (size_t)nj * n0];
+        if (y < 0 || tile_of[y] == t) continue;
+        if (std::find(nb.begin(), nb.end(), tile_of[y]) == nb.end()) nb.push_back(tile_of[y]);
+      }
+    }
+    std::sort(nb.begin(), nb.end(), [&](int a, int b) {
+      return drafts[a].own.size() != drafts[b].own.size() ? drafts[a].own.size() < drafts[b].own.size() : a < b;
+    });
+    return nb;
+  };
+  // whether the region of A's and B's own cells fits: that region is the
+  // union of theirs, each cell at the larger of its two depths (the walk is
+  // a max over own cells), so the counts follow without a walk
+  auto can_merge = [&](const Draft& A, const Draft& B) -> bool {
+    int npl = (int)A.pl.size();
+    for (int p : B.pl)
+      if (std::find(A.pl.begin(), A.pl.end(), p) == A.pl.end() && ++npl > kRegionMaxPatches) return false;
+    const long long x0 = std::min(A.cx0, B.cx0), x1 = std::max(A.cx0 + A.cw, B.cx0 + B.cw);
+    const long long y0 = std::min(A.cy0, B.cy0), y1 = std::max(A.cy0 + A.ch, B.cy0 + B.ch);
+    if ((x1 - x0) * (y1 - y0) > kRegionMaxSlots) return false;
+    for (size_t x = 0; x < A.rcell.size(); ++x) depth_of[A.rcell[x]] = A.rdep[x];
+    for (size_t x = 0; x < B.rcell.size(); ++x) depth_of[B.rcell[x]] = std::max(depth_of[B.rcell[x]], B.rdep[x]);
+    int nred = 0, nblack = 0, nload = 0;
+    auto count = [&](int c) {
+      if (depth_of[c] < 0) return;
+      if (depth_of[c] == 0) ++nload;
+      else if (colour[c] == 0) ++nred;
+      else ++nblack;
+      depth_of[c] = -1;
+    };
+    for (int c : A.rcell) count(c);
+    for (int c : B.rcell) count(c);
+    return nred <= per_colour && nblack <= per_colour && nload <= per_load;
+  };
+  for (bool changed = true; changed;) {
+    changed = false;
+    std::vector<int> order;
+    for (size_t t = 0; t < drafts.size(); ++t)
+      if (alive[t]) order.push_back((int)t);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return drafts[a].own.size() < drafts[b].own.size(); });
+    for (int t : order) {
+      if (!alive[t]) continue;
+      for (int u : neighbours(t)) {
+        if ((long long)drafts[t].own.size() + (long long)drafts[u].own.size() > 2LL * per_colour) continue;
+        if (!can_merge(drafts[t], drafts[u])) continue;
+        Draft M;
+        M.own = drafts[t].own;
+        M.own.insert(M.own.end(), drafts[u].own.begin(), drafts[u].own.end());
+        if (!draft(M)) return false;  // can_merge said it fits
+        for (int c : drafts[t].own) tile_of[c] = u;
+        drafts[u] = std::move(M);
+        drafts[t] = Draft();
+        alive[t] = 0;
+        changed = true;
+        break;
+      }
+    }
+  }
+  // 3. the tiles in spatial order (the launch gives each XCD a contiguous
+  //    range): by the first own cell of each, in level cell order
+  std::vector<int> keep;
+  for (size_t t = 0; t < drafts.size(); ++t)
+    if (alive[t]) {
+      std::sort(drafts[t].own.begin(), drafts[t].own.end());
+      keep.push_back((int)t);
+    }
+  std::sort(keep.begin(), keep.end(), [&](int a, int b) { return drafts[a].own[0] < drafts[b].own[0]; });
+  for (int t : keep) {
+    const Draft& D = drafts[t];
+    RegionTile rt;
+    std::memset(&rt, 0, sizeof rt);
+    rt.cx0 = D.cx0; rt.cy0 = D.cy0; rt.cw = D.cw; rt.ch = D.ch;
+    rt.np = (int)D.pl.size();
+    rt.nload = (int)D.load.size(); rt.nred = (int)D.red.size(); rt.nblack = (int)D.black.size();
+    for (int k = 0; k < kRegionMaxPatches; ++k) {
+      RegionPatch rp;
+      std::memset(&rp, 0, sizeof rp);
+      if (k < (int)D.pl.size()) {
+        rp.pi = L.pi[D.pl[k]];
+        rp.ox = L.box[D.pl[k]].ilo - 1 - D.cx0;
+        rp.oy = L.box[D.pl[k]].jlo - 1 - D.cy0;
+      }
+      out.patches.push_back(rp);
+      out.pid.push_back(k < (int)D.pl.size() ? D.pl[k] : -1);
+    }
+    const size_t base = out.cells.size();
+    out.cells.resize(base + kRegionStride, kRegionNone);
+    std::copy(D.red.begin(), D.red.end(), out.cells.begin() + base);
+    std::copy(D.black.begin(), D.black.end(), out.cells.begin() + base + per_colour);
+    std::copy(D.load.begin(), D.load.end(), out.cells.begin() + base + 2 * per_colour);
+    out.tiles.push_back(rt);
+    out.max_slots = std::max(out.max_slots, D.cw * D.ch);
   }
   return !out.tiles.empty();
 }
@@ -1386,7 +1473,8 @@ struct Seq {
     const LevelDev<T>& Lk = L[k - 1];
     return nsweeps > 0 && Lk.nregion > 0 && 2 * nsweeps <= Lk.rdepth && h->opts.smoother == 0;
   }
-  void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false) {
+  // defer_fill: a region relax leaves its fill to the caller's fused launch
+  void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false, bool defer_fill = false) {
     const bool qs_current = qs_ready == k;
     qs_ready = 0;
     if (region(k, nsweeps)) {
@@ -1396,7 +1484,7 @@ struct Seq {
       // parent's last change or a preparing correction; else refresh them
       if (first_inline == 1 || (first_inline == 0 && !cf_current)) launch_refresh_cf<T>(lev(k), par(k), s);
       launch_relax_region<T>(lev(k), par(k), nsweeps, first_inline == 0 ? 1 : 0, (T)h->opts.omega, homog, s);
-      launch_fill_copy<T>(lev(k), par(k), homog, lev(k).Qs, s);
+      if (!defer_fill) launch_fill_copy<T>(lev(k), par(k), homog, lev(k).Qs, s);
       return;
     }
     if (nsweeps > 0 && h->opts.smoother == 1 && tiled_fits<T>(lev(k))) {
@@ -1506,14 +1594,19 @@ struct Seq {
       coarsest();
       return;
     }
-    // below the top level the fill in the previous step is current
-    relax(k, h->opts.npre, 0, k < h->nlev);
+    // below the top level the fill in the previous step is current.  After
+    // a region relax its fill and the restriction run as one launch (the
+    // fill's QC ring refresh is redone by the load's ring gather)
+    const bool fuse = region(k, h->opts.npre);
+    relax(k, h->opts.npre, 0, k < h->nlev, fuse);
+    T* dup = nullptr;
     if (k <= h->root && k - 1 > kc && odd_stream(k - 1, h->opts.npre)) {
-      launch_restrict<T>(lev(k), lev(k - 1), 0, s, lev(k - 1).Qs);
+      dup = lev(k - 1).Qs;
       qs_ready = k - 1;
-    } else {
-      restrict_(k);
     }
+    if (fuse) launch_fill_copy_restrict<T>(lev(k), lev(k - 1), homog, lev(k).Qs, dup, s);
+    else if (dup) launch_restrict<T>(lev(k), lev(k - 1), 0, s, dup);
+    else restrict_(k);
     fill(k - 1);
     launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s);
     vcycle(k - 1, kc);

@@ -137,3 +137,18 @@ def test_region_depths_follow_the_pass_colours(name):
                     for ni, nj in ((i - 1, j), (i + 1, j), (i, j - 1), (i, j + 1)):
                         if 0 <= ni < mx and 0 <= nj < my:
                             assert depth.get((p, ni, nj), -1) >= q - 1, (k, p, i, j, q)
+
+
+def test_tiles_merge_pieces_of_several_patches():
+    """Adjacent pieces share a tile when the merged region still fits one
+    workgroup: on the L-shaped union a tile owns cells of two patches, and
+    the starfish-like packing keeps every tile within the workgroup's lanes."""
+    H = CASES["lshape"](seed=None)
+    merged = 0
+    for k, (rdepth, tiles) in _tables(H).items():
+        for T in tiles:
+            own_patches = {p for p, i, j, d in T["cells"] if d == rdepth}
+            merged += len(own_patches) > 1
+            assert T["nred"] <= 1024 and T["nblack"] <= 1024 and T["nload"] <= 1024
+            assert T["cw"] * T["ch"] <= 4096
+    assert merged >= 2
