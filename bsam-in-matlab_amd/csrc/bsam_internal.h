@@ -217,8 +217,11 @@ struct BatchItem {
   const LevelDev<float>* LV32;   // FP32 levels (mixed precision) or null
   float* const* rout32;          // per-level FP32 defect arrays (mixed) or null
   ResidualOut* out;              // residual scalars of this item
+  ResidualOut* parts;            // per-workgroup residual partials (nwg > 1)
   CoarsestDev cst;
-  int nlev, root, phase, pad;
+  int nlev, root, phase;
+  int wg0, nwg;                  // the item's workgroups in its launch
+  int pad;
 };
 
 }  // namespace bsam

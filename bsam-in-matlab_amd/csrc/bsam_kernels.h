@@ -146,9 +146,15 @@ void launch_history(const LevelDev<double>& L, double* Uo, double* Uoo, int rota
 void launch_regrid_level(const LevelDev<double>& L, const double* Pq, int conslin, int hist, double* Uoo_k,
                          const double* Uoo_all, const int* fc, const int* oq, const int* oc, const double* Qold,
                          const double* Uoo_old, hipStream_t s);
-// one workgroup (1024 threads) per item: the item's preamble or whole cycle;
-// max_m: largest coarsest order among the items (LDS right side)
-void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre, int npost, double omega,
-                        int mass_corr, int mixed, int inner, hipStream_t s);
+// a group of nwg_i workgroups (1024 threads each) per item: the item's
+// preamble or whole cycle; grp_item maps each of the launch's nwg workgroups
+// to its item, ctr holds one zeroed barrier counter per item and *err (zero)
+// is set when a group barrier gave up waiting; max_m: largest coarsest order
+// among the items (LDS right side).  A cooperative launch: returns its
+// hipError_t (every workgroup must be resident at once)
+int launch_batch_cycle(const BatchItem* items, const int* grp_item, int nwg, unsigned* ctr, int* err, int max_m,
+                       int npre, int npost, double omega, int mass_corr, int mixed, int inner, hipStream_t s);
+// resident k_batch_cycle workgroups per CU for a coarsest order max_m (0: unknown)
+int batch_cycle_blocks_per_cu(int max_m);
 
 }  // namespace bsam

@@ -864,50 +864,6 @@ __global__ void __launch_bounds__(kTile) k_fill_copy(LevelDev<T> L, const T* __r
   L.Q[qc] = Qsrc[qc];
 }
 
-// The fill that follows a region relax and the restriction that follows the
-// pre-smoothing (ebVcycle.m:21-22) as one launch.  Nothing in it reads what
-// it writes: the ghosts read the new interior from Qsrc and the parent parts
-// of coarse-fine records from aux (current: the region launch read them and
-// the parent has not changed since), the restriction reads Qsrc and writes
-// the parent and QC's interior, and the interior copy writes Q.  The fill's
-// QC ring refresh is left out: the load that follows the restriction gathers
-// the ring again from the restricted parent before anything reads it.
-// Blocks: [0, nfb) ghosts, then one per 256-cell tile (copy), then the
-// restriction's footprint cells.
-template <typename T>
-__global__ void __launch_bounds__(kTile) k_fill_copy_restrict(LevelDev<T> L, int homog, const T* __restrict__ Qsrc,
-                                                              int nfb, T* __restrict__ Pq, T* __restrict__ Pdup) {
-  const int b = blockIdx.x;
-  if (b < nfb) {
-    const int t = b * kTile + threadIdx.x;
-    if (t >= L.nghost) return;
-    const GhostRec r = L.ghost[t];
-    T v;
-    if (r.kind == G_PCORNER) {
-      const T a = ghost_value_cf<T>(L.ghost[r.s[0]], Qsrc, L.aux[r.s[0]].cf, homog);
-      const T c = ghost_value_cf<T>(L.ghost[r.s[1]], Qsrc, L.aux[r.s[1]].cf, homog);
-      v = (a + c) - Qsrc[r.s[2]];
-    } else {
-      v = ghost_value_cf<T>(r, Qsrc, is_cf_kind(r.kind) ? L.aux[t].cf : T(0), homog);
-    }
-    L.Q[r.dst] = v;
-    return;
-  }
-  if (b < nfb + L.ntile) {
-    const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane(b - nfb));
-    const PatchInfo& pi = tr.pi;
-    const int c = tr.start + threadIdx.x;
-    if (c >= pi.mx * pi.my) return;
-    const int j = cell_row(c, pi.mx, pi.magic);
-    const int i = c - j * pi.mx;
-    const int qc = pi.qoff + (i + 1) + (j + 1) * (pi.mx + 2);
-    L.Q[qc] = Qsrc[qc];
-    return;
-  }
-  const int t = (b - nfb - L.ntile) * kTile + threadIdx.x;
-  if (t < L.nfoot) restrict_item<T>(L, Pq, t, 0, Pdup, Qsrc);
-}
-
 // the parent part of every coarse-fine record of a level from the current
 // parent, into aux (what a fill stores, without touching Q)
 template <typename T>
@@ -1246,6 +1202,51 @@ __device__ __forceinline__ void restrict_item(const LevelDev<T>& L, T* __restric
   if (Pdup) Pdup[fr.pq] = R;  // the parent's second array (streaming sweeps)
 }
 
+// The fill that follows a region relax and the restriction that follows the
+// pre-smoothing (ebVcycle.m:21-22) as one launch.  Nothing in it reads what
+// it writes: the ghosts read the new interior from Qsrc and the parent parts
+// of coarse-fine records from aux (current: the region launch read them and
+// the parent has not changed since), the restriction reads Qsrc and writes
+// the parent and QC's interior, and the interior copy writes Q.  The fill's
+// QC ring refresh is left out: the load that follows the restriction gathers
+// the ring again from the restricted parent before anything reads it.
+// Blocks: [0, nfb) ghosts, then one per 256-cell tile (copy), then the
+// restriction's footprint cells.
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_fill_copy_restrict(LevelDev<T> L, int homog, const T* __restrict__ Qsrc,
+                                                              int nfb, T* __restrict__ Pq, T* __restrict__ Pdup) {
+  const int b = blockIdx.x;
+  if (b < nfb) {
+    const int t = b * kTile + threadIdx.x;
+    if (t >= L.nghost) return;
+    const GhostRec r = L.ghost[t];
+    T v;
+    if (r.kind == G_PCORNER) {
+      const T a = ghost_value_cf<T>(L.ghost[r.s[0]], Qsrc, L.aux[r.s[0]].cf, homog);
+      const T c = ghost_value_cf<T>(L.ghost[r.s[1]], Qsrc, L.aux[r.s[1]].cf, homog);
+      v = (a + c) - Qsrc[r.s[2]];
+    } else {
+      v = ghost_value_cf<T>(r, Qsrc, is_cf_kind(r.kind) ? L.aux[t].cf : T(0), homog);
+    }
+    L.Q[r.dst] = v;
+    return;
+  }
+  if (b < nfb + L.ntile) {
+    const TileRec tr = load_tile(L.tiles, __builtin_amdgcn_readfirstlane(b - nfb));
+    const PatchInfo& pi = tr.pi;
+    const int c = tr.start + threadIdx.x;
+    if (c >= pi.mx * pi.my) return;
+    const int j = cell_row(c, pi.mx, pi.magic);
+    const int i = c - j * pi.mx;
+    const int qc = pi.qoff + (i + 1) + (j + 1) * (pi.mx + 2);
+    L.Q[qc] = Qsrc[qc];
+    return;
+  }
+  const int t = (b - nfb - L.ntile) * kTile + threadIdx.x;
+  if (t < L.nfoot) restrict_item<T>(L, Pq, t, 0, Pdup, Qsrc);
+}
+
+
 // L(u) at one cell of a padded array (ebKernels.m:13-25)
 // the level's h^2 and reciprocals for div_h2
 template <typename T>
@@ -1537,6 +1538,49 @@ __global__ void __launch_bounds__(64) k_coarsest(CoarsestDev C, LevelDev<T> L1, 
   if (i < C.N) coarsest_row<T>(C, rhs, i, L1.n[0], L1.Q);
 }
 
+// ---------------------------------------------------------------- groups
+// A group of workgroups that runs one hierarchy's steps together: the items
+// of a step are strided over the group's threads and dependent steps are
+// separated by sync().  With one workgroup sync() is __syncthreads().  With
+// several, thread 0 of each workgroup releases the workgroup's writes (an
+// agent-scope fence after the workgroup barrier), counts in on the group's
+// counter and polls it until every workgroup of the group has arrived, then
+// acquires the others' writes.  The wait is bounded: after kGrpSpinLimit
+// polls it sets *err and stops waiting (as does every later wait once *err is
+// set), so a launch whose workgroups were not all resident ends with an error
+// instead of hanging.  The counter is zero when the launch starts; target
+// grows by nwg per barrier.
+constexpr unsigned kGrpSpinLimit = 1u << 22;
+struct Grp {
+  int rank, size;  // thread index in the group, threads in the group
+  int wg, nwg;     // workgroup index in the group, workgroups in the group
+  unsigned* ctr;
+  int* err;
+  unsigned target;
+  __device__ static Grp block() { return Grp{(int)threadIdx.x, (int)blockDim.x, 0, 1, nullptr, nullptr, 0u}; }
+  __device__ void sync() {
+    __syncthreads();
+    if (nwg > 1) {
+      if (threadIdx.x == 0) {
+        target += (unsigned)nwg;
+        __threadfence();
+        __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned polls = 0;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+          if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+          if (++polls > kGrpSpinLimit) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        __threadfence();
+      }
+      __syncthreads();
+    }
+  }
+};
+
 // ---------------------------------------------------------------- coarse cycle
 // The V-cycle of levels 1..kc inside one workgroup (ebVcycle.m:15-35 for
 // k <= kc): every step is a workgroup-strided loop over the step's items,
@@ -1544,7 +1588,7 @@ __global__ void __launch_bounds__(64) k_coarsest(CoarsestDev C, LevelDev<T> L1, 
 // ghosts in its first pass; after the correction the fill is folded into the
 // first post-smoothing pass (inline ghosts), as in the multi-kernel form.
 template <typename T>
-__device__ void blk_relax(const LevelDev<T>& L, const T* Pq, int nsweeps, T omega, int homog, int first_inl) {
+__device__ void blk_relax(Grp& g, const LevelDev<T>& L, const T* Pq, int nsweeps, T omega, int homog, int first_inl) {
   const int ncell = L.ntile * kTile;
   for (int it = 0; it < nsweeps; ++it)
     for (int colour = 1; colour <= 2; ++colour) {
@@ -1552,72 +1596,75 @@ __device__ void blk_relax(const LevelDev<T>& L, const T* Pq, int nsweeps, T omeg
       const int inl = (it == 0 && colour == 1) ? first_inl : 1;
       if (L.even_mx) {
         if (inl) {
-          for (int x = threadIdx.x; x < (ncell >> 1); x += blockDim.x)
+          for (int x = g.rank; x < (ncell >> 1); x += g.size)
             relax_item_colour<T, false>(L, Pq, x, colour, omega, homog, kInlRecords);
         } else {
-          for (int x = threadIdx.x; x < (ncell >> 1); x += blockDim.x)
+          for (int x = g.rank; x < (ncell >> 1); x += g.size)
             relax_item_colour<T, false>(L, Pq, x, colour, omega, homog, 0);
         }
       } else {
         if (inl) {
-          for (int x = threadIdx.x; x < ncell; x += blockDim.x)
+          for (int x = g.rank; x < ncell; x += g.size)
             relax_item<T, false>(L, Pq, x / kTile, x % kTile, colour, omega, homog, kInlRecords);
         } else {
-          for (int x = threadIdx.x; x < ncell; x += blockDim.x)
+          for (int x = g.rank; x < ncell; x += g.size)
             relax_item<T, false>(L, Pq, x / kTile, x % kTile, colour, omega, homog, 0);
         }
       }
-      __syncthreads();
+      g.sync();
     }
   if (nsweeps > 0) {
     const int n = fill_items<T>(L, Pq);
-    for (int x = threadIdx.x; x < n; x += blockDim.x) fill_item<T>(L, Pq, x, homog);
-    __syncthreads();
+    for (int x = g.rank; x < n; x += g.size) fill_item<T>(L, Pq, x, homog);
+    g.sync();
   }
 }
 
 template <typename T>
-__device__ void blk_fill(const LevelDev<T>& L, const T* Pq, int homog) {
+__device__ void blk_fill(Grp& g, const LevelDev<T>& L, const T* Pq, int homog) {
   const int n = fill_items<T>(L, Pq);
-  for (int x = threadIdx.x; x < n; x += blockDim.x) fill_item<T>(L, Pq, x, homog);
-  __syncthreads();
+  for (int x = g.rank; x < n; x += g.size) fill_item<T>(L, Pq, x, homog);
+  g.sync();
 }
 
 // ebVcycle.m:15-35 for levels kc..1 inside one workgroup; rhs is LDS scratch
 // of at least C.M doubles for the coarsest solve
 template <typename T>
-__device__ void blk_vcycle(const LevelDev<T>* __restrict__ LV, int kc, const CoarsestDev& C, int npre,
+__device__ void blk_vcycle(Grp& g, const LevelDev<T>* __restrict__ LV, int kc, const CoarsestDev& C, int npre,
                            int npost, T omega, int homog, int mass_corr, double* rhs) {
   for (int k = kc; k >= 2; --k) {
     const LevelDev<T>& L = LV[k - 1];
     const LevelDev<T>& P = LV[k - 2];
-    blk_relax<T>(L, P.Q, npre, omega, homog, 0);
-    for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
-    __syncthreads();
-    blk_fill<T>(P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
+    blk_relax<T>(g, L, P.Q, npre, omega, homog, 0);
+    for (int x = g.rank; x < L.nfoot; x += g.size) restrict_item<T>(L, P.Q, x, 0);
+    g.sync();
+    blk_fill<T>(g, P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
     const int n = load_items<T>(L, P, mass_corr);
-    for (int x = threadIdx.x; x < n; x += blockDim.x) load_item<T>(L, P, x, mass_corr);
-    __syncthreads();
+    for (int x = g.rank; x < n; x += g.size) load_item<T>(L, P, x, mass_corr);
+    g.sync();
   }
   {
+    // the coarsest solve in the group's first workgroup (rhs is its LDS)
     const LevelDev<T>& L1 = LV[0];
-    for (int j = threadIdx.x; j < C.M; j += blockDim.x)
-      rhs[j] = j < C.N ? (homog ? (double)L1.F[j] : (double)L1.F[j] + C.bcRhs[j]) : 0.0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < C.N; i += blockDim.x) coarsest_row<T>(C, rhs, i, L1.n[0], L1.Q);
-    __syncthreads();
-    blk_fill<T>(L1, (const T*)nullptr, homog);
+    if (g.wg == 0) {
+      for (int j = threadIdx.x; j < C.M; j += blockDim.x)
+        rhs[j] = j < C.N ? (homog ? (double)L1.F[j] : (double)L1.F[j] + C.bcRhs[j]) : 0.0;
+      __syncthreads();
+      for (int i = threadIdx.x; i < C.N; i += blockDim.x) coarsest_row<T>(C, rhs, i, L1.n[0], L1.Q);
+    }
+    g.sync();
+    blk_fill<T>(g, L1, (const T*)nullptr, homog);
   }
   for (int k = 2; k <= kc; ++k) {
     const LevelDev<T>& L = LV[k - 1];
     const T* Pq = LV[k - 2].Q;
     const int ncell = L.ntile * kTile;
-    for (int x = threadIdx.x; x < ncell; x += blockDim.x) correct_item<T>(L, Pq, x / kTile, x % kTile, 0);
-    __syncthreads();
+    for (int x = g.rank; x < ncell; x += g.size) correct_item<T>(L, Pq, x / kTile, x % kTile, 0);
+    g.sync();
     if (npost > 0) {
-      blk_relax<T>(L, Pq, npost, omega, homog, 1);
+      blk_relax<T>(g, L, Pq, npost, omega, homog, 1);
     } else {
-      blk_fill<T>(L, Pq, homog);
+      blk_fill<T>(g, L, Pq, homog);
     }
   }
 }
@@ -1629,12 +1676,13 @@ __device__ void blk_vcycle(const LevelDev<T>* __restrict__ LV, int kc, const Coa
 template <typename T>
 __global__ void __launch_bounds__(1024) k_filldown_small(const LevelDev<T>* __restrict__ LV, int kfrom, int kto,
                                                           int homog) {
+  Grp g = Grp::block();
   for (int k = kfrom; k >= kto; --k) {
     const LevelDev<T>& L = LV[k - 1];
     const LevelDev<T>& P = LV[k - 2];
     for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
     __syncthreads();
-    blk_fill<T>(P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
+    blk_fill<T>(g, P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
   }
 }
 
@@ -1757,8 +1805,9 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __rest
                                                        int homog, int mass_corr, int staged) {
   extern __shared__ __attribute__((aligned(16))) double smem[];
   double* rhs = smem;
+  Grp g = Grp::block();
   if (!staged) {
-    blk_vcycle<T>(LV, kc, C, npre, npost, omega, homog, mass_corr, rhs);
+    blk_vcycle<T>(g, LV, kc, C, npre, npost, omega, homog, mass_corr, rhs);
     return;
   }
   // levels 1..kc live in LDS for the whole cycle; the cycle writes Q, F and QC
@@ -1767,7 +1816,7 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __rest
   __shared__ int nseg_total[2];
   char* base = reinterpret_cast<char*>(smem) + stage_align((long long)C.M * (long long)sizeof(double));
   blk_stage_levels<T>(LV, kc, LS, base, seg, nseg_total);
-  blk_vcycle<T>(LS, kc, C, npre, npost, omega, homog, mass_corr, rhs);
+  blk_vcycle<T>(g, LS, kc, C, npre, npost, omega, homog, mass_corr, rhs);
   __syncthreads();
   if (threadIdx.x == 0) {  // write back what the cycle changes: Q, F, QC
     int ns = 0, words = 0;
@@ -2030,70 +2079,97 @@ __device__ __forceinline__ void block_reduce4_wg(double a, double b, double c, l
 }
 
 template <typename T>
-__device__ void blk_reset_force(const LevelDev<T>* __restrict__ LV, int nlev) {
+__device__ void blk_reset_force(Grp& g, const LevelDev<T>* __restrict__ LV, int nlev) {
   for (int k = 0; k < nlev; ++k) {
     const LevelDev<T>& L = LV[k];
     T* __restrict__ F = L.F;
     const T* __restrict__ FT = L.FT;
     const int n = L.ncell;
-    for (int x = threadIdx.x; x < n; x += blockDim.x) F[x] = FT[x];
+    for (int x = g.rank; x < n; x += g.size) F[x] = FT[x];
   }
-  __syncthreads();
+  g.sync();
 }
 
 template <typename T>
-__device__ void blk_filldown(const LevelDev<T>* __restrict__ LV, int nlev, int homog) {
+__device__ void blk_filldown(Grp& g, const LevelDev<T>* __restrict__ LV, int nlev, int homog) {
   for (int k = nlev; k >= 2; --k) {  // ebTransfer.m:73-79
     const LevelDev<T>& L = LV[k - 1];
     const LevelDev<T>& P = LV[k - 2];
-    for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
-    __syncthreads();
-    blk_fill<T>(P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
+    for (int x = g.rank; x < L.nfoot; x += g.size) restrict_item<T>(L, P.Q, x, 0);
+    g.sync();
+    blk_fill<T>(g, P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
   }
 }
 
 // composite residual (ebCompResidual.m:23-84): QC refresh of the AMR levels,
 // then every visible cell of levels >= root
-__device__ void blk_residual(const LevelDev<double>* __restrict__ LV, int nlev, int root, int mass_corr,
-                             float* const* rout, ResidualOut* out) {
+__device__ void blk_residual(Grp& g, const LevelDev<double>* __restrict__ LV, int nlev, int root, int mass_corr,
+                             float* const* rout, ResidualOut* out, ResidualOut* parts) {
   if (mass_corr) {
     for (int k = max(2, root + 1); k <= nlev; ++k) {
       const LevelDev<double>& L = LV[k - 1];
       const double* Pq = LV[k - 2].Q;
       const int n = L.nring_all + L.nfoot;
-      for (int x = threadIdx.x; x < n; x += blockDim.x) {
+      for (int x = g.rank; x < n; x += g.size) {
         if (x < L.nring_all) ring_item<double>(L, Pq, x);
         else restrict_item<double>(L, (double*)nullptr, x - L.nring_all, 1);
       }
     }
-    __syncthreads();
+    g.sync();
   }
   double a = 0.0, b = 0.0, f = 0.0;
   long long n = 0;
   for (int k = root; k <= nlev; ++k) {
     const int ncell = LV[k - 1].ntile * kTile;
-    for (int x = threadIdx.x; x < ncell; x += blockDim.x)
+    for (int x = g.rank; x < ncell; x += g.size)
       residual_item<double, float>(LV, nlev, k - 1, x / kTile, x % kTile, mass_corr, rout, root, a, b, f, n);
   }
-  block_reduce4_wg(a, b, f, n, out);
+  if (g.nwg == 1) {
+    block_reduce4_wg(a, b, f, n, out);
+    return;
+  }
+  // per-workgroup partials, combined by the group's first workgroup in
+  // workgroup order
+  block_reduce4_wg(a, b, f, n, parts + g.wg);
+  g.sync();
+  if (g.wg == 0 && threadIdx.x == 0) {
+    ResidualOut r = parts[0];
+    for (int w = 1; w < g.nwg; ++w) {
+      r.linf = fmax(r.linf, parts[w].linf);
+      r.l2sq = r.l2sq + parts[w].l2sq;
+      r.fnorm = fmax(r.fnorm, parts[w].fnorm);
+      r.ncell += parts[w].ncell;
+    }
+    *out = r;
+  }
 }
 
-__global__ void __launch_bounds__(1024) k_batch_cycle(const BatchItem* __restrict__ items, int npre, int npost,
-                                                      double omega, int mass_corr, int mixed, int inner) {
+// One cycle (or the stage preamble) of every batch item; item i runs on the
+// group of workgroups [wg0, wg0 + nwg) of the launch (grp_item maps a
+// workgroup to its item) and counts its barriers on ctr[i].  Every
+// workgroup of the launch must be resident at once (a cooperative launch).
+__global__ void __launch_bounds__(1024) k_batch_cycle(const BatchItem* __restrict__ items,
+                                                      const int* __restrict__ grp_item, unsigned* ctr, int* err,
+                                                      int npre, int npost, double omega, int mass_corr, int mixed,
+                                                      int inner) {
   extern __shared__ double rhs[];
-  const BatchItem& it = items[blockIdx.x];
+  const int ii = grp_item[blockIdx.x];
+  const BatchItem& it = items[ii];
+  const int wg = (int)blockIdx.x - it.wg0;
+  Grp g{wg * (int)blockDim.x + (int)threadIdx.x, it.nwg * (int)blockDim.x, wg, it.nwg, ctr + ii, err, 0u};
   const LevelDev<double>* LV = it.LV64;
   const int nlev = it.nlev;
   if (it.phase == 0) {  // ebSolveCycles.m:11-19: fill every level, baseline residual
-    for (int k = 1; k <= nlev; ++k) blk_fill<double>(LV[k - 1], k > 1 ? (const double*)LV[k - 2].Q : nullptr, 0);
-    blk_residual(LV, nlev, it.root, mass_corr, mixed ? it.rout32 : nullptr, it.out);
+    for (int k = 1; k <= nlev; ++k)
+      blk_fill<double>(g, LV[k - 1], k > 1 ? (const double*)LV[k - 2].Q : nullptr, 0);
+    blk_residual(g, LV, nlev, it.root, mass_corr, mixed ? it.rout32 : nullptr, it.out, it.parts);
     return;
   }
   if (!mixed) {  // ebSolveCycles.m:28-41
-    blk_reset_force<double>(LV, nlev);
-    blk_vcycle<double>(LV, nlev, it.cst, npre, npost, omega, 0, mass_corr, rhs);
-    blk_filldown<double>(LV, nlev, 0);
-    blk_residual(LV, nlev, it.root, mass_corr, nullptr, it.out);
+    blk_reset_force<double>(g, LV, nlev);
+    blk_vcycle<double>(g, LV, nlev, it.cst, npre, npost, omega, 0, mass_corr, rhs);
+    blk_filldown<double>(g, LV, nlev, 0);
+    blk_residual(g, LV, nlev, it.root, mass_corr, nullptr, it.out, it.parts);
     return;
   }
   // FP32 V-cycles on the correction problem, FP64 update, filldown, ghosts,
@@ -2101,33 +2177,33 @@ __global__ void __launch_bounds__(1024) k_batch_cycle(const BatchItem* __restric
   const LevelDev<float>* L32 = it.LV32;
   for (int k = 0; k < nlev; ++k) {
     const LevelDev<float>& L = L32[k];
-    for (int x = threadIdx.x; x < L.nq; x += blockDim.x) L.Q[x] = 0.0f;
-    for (int x = threadIdx.x; x < L.nqc; x += blockDim.x) L.QC[x] = 0.0f;
+    for (int x = g.rank; x < L.nq; x += g.size) L.Q[x] = 0.0f;
+    for (int x = g.rank; x < L.nqc; x += g.size) L.QC[x] = 0.0f;
   }
-  __syncthreads();
+  g.sync();
   for (int i = 0; i < inner; ++i) {
-    blk_reset_force<float>(L32, nlev);
-    blk_vcycle<float>(L32, nlev, it.cst, npre, npost, (float)omega, 1, mass_corr, rhs);
-    if (i + 1 < inner) blk_filldown<float>(L32, nlev, 1);
+    blk_reset_force<float>(g, L32, nlev);
+    blk_vcycle<float>(g, L32, nlev, it.cst, npre, npost, (float)omega, 1, mass_corr, rhs);
+    if (i + 1 < inner) blk_filldown<float>(g, L32, nlev, 1);
   }
   for (int k = 0; k < nlev; ++k) {
     const LevelDev<double>& L = LV[k];
     const float* E = L32[k].Q;
     const int ncell = L.ntile * kTile;
-    for (int x = threadIdx.x; x < ncell; x += blockDim.x) {
+    for (int x = g.rank; x < ncell; x += g.size) {
       const TileRec& tr = L.tiles[x / kTile];
       const int c = tr.start + x % kTile;
       if (c >= tr.pi.mx * tr.pi.my) continue;
       const int j = cell_row(c, tr.pi.mx, tr.pi.magic);
-      const int ii = c - j * tr.pi.mx;
-      const int qc = tr.pi.qoff + (ii + 1) + (j + 1) * (tr.pi.mx + 2);
+      const int i2 = c - j * tr.pi.mx;
+      const int qc = tr.pi.qoff + (i2 + 1) + (j + 1) * (tr.pi.mx + 2);
       L.Q[qc] = L.Q[qc] + (double)E[qc];
     }
   }
-  __syncthreads();
-  blk_filldown<double>(LV, nlev, 0);
-  for (int k = 1; k <= nlev; ++k) blk_fill<double>(LV[k - 1], k > 1 ? (const double*)LV[k - 2].Q : nullptr, 0);
-  blk_residual(LV, nlev, it.root, mass_corr, it.rout32, it.out);
+  g.sync();
+  blk_filldown<double>(g, LV, nlev, 0);
+  for (int k = 1; k <= nlev; ++k) blk_fill<double>(g, LV[k - 1], k > 1 ? (const double*)LV[k - 2].Q : nullptr, 0);
+  blk_residual(g, LV, nlev, it.root, mass_corr, it.rout32, it.out, it.parts);
 }
 
 // ---------------------------------------------------------------- elementwise
@@ -2521,11 +2597,21 @@ void launch_regrid_level(const LevelDev<double>& L, const double* Pq, int consli
                      oc, Qold, Uoo_old);
 }
 
-void launch_batch_cycle(const BatchItem* items, int nitems, int max_m, int npre, int npost, double omega,
-                        int mass_corr, int mixed, int inner, hipStream_t s) {
-  if (nitems <= 0) return;
-  hipLaunchKernelGGL(k_batch_cycle, dim3(nitems), dim3(1024), (size_t)max_m * sizeof(double), s, items, npre,
-                     npost, omega, mass_corr, mixed, inner);
+int batch_cycle_blocks_per_cu(int max_m) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch_cycle, 1024, (size_t)max_m * sizeof(double)) !=
+      hipSuccess)
+    return 0;
+  return nb;
+}
+
+int launch_batch_cycle(const BatchItem* items, const int* grp_item, int nwg, unsigned* ctr, int* err, int max_m,
+                       int npre, int npost, double omega, int mass_corr, int mixed, int inner, hipStream_t s) {
+  if (nwg <= 0) return 0;
+  void* args[] = {(void*)&items, (void*)&grp_item, (void*)&ctr, (void*)&err, (void*)&npre, (void*)&npost,
+                  (void*)&omega, (void*)&mass_corr, (void*)&mixed, (void*)&inner};
+  return (int)hipLaunchCooperativeKernel((const void*)k_batch_cycle, dim3(nwg), dim3(1024), args,
+                                         (unsigned)((size_t)max_m * sizeof(double)), s);
 }
 
 // ---------------------------------------------------------------- tagging

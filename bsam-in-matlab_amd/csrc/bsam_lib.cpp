@@ -129,6 +129,7 @@ struct bsam_hier {
   double dom[4] = {0, 0, 0, 0};
   bsam_solve_opts opts{};
   std::vector<HLevel> H;                   // index 0 = level 1
+  std::vector<double> ainv_host;           // host-only contexts: the coarsest inverse (column-major)
   std::vector<LevelDev<double>> d64;
   std::vector<LevelDev<float>> d32;
   CoarsestDev cst{};
@@ -1824,7 +1825,10 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
       return rc;
   h->cst.N = h->H[0].n[0] * h->H[0].n[1];
   h->cst.M = M;
-  if (ctx->device < 0) return 0;  // host-only: no device state
+  if (ctx->device < 0) {  // host-only: no device state; the host inverse is kept for export
+    h->ainv_host = std::move(Ainv);
+    return 0;
+  }
   if ((rc = upload(h, &h->cst.Ainv, Ainv))) return rc;
   if ((rc = upload(h, &h->cst.bcRhs, bcRhs))) return rc;
   // device levels
@@ -2505,6 +2509,7 @@ long long bsam_export_size(const bsam_hier* h, int k, int what) {
       region_export(h, L, v);
       return (long long)v.size();
     }
+    case 6: return k == 1 ? (long long)h->cst.M * h->cst.M : -1;
     default: return -1;
   }
 }
@@ -2543,6 +2548,19 @@ int bsam_export(const bsam_hier* h, int k, int what, long long* out) {
       std::vector<long long> v;
       region_export(h, L, v);
       for (long long i = 0; i < n; ++i) out[i] = v[(size_t)i];
+      break;
+    }
+    case 6: {  // the coarsest inverse's bit patterns
+      std::vector<double> a((size_t)n);
+      if (h->ctx->device >= 0) {
+        HIPCHK(hipSetDevice(h->ctx->device));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(a.data(), h->cst.Ainv, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+      } else {
+        if (h->ainv_host.size() != (size_t)n) return fail(BSAM_E_ARG, "bsamgpu:arg: no coarsest inverse");
+        a = h->ainv_host;
+      }
+      std::memcpy(out, a.data(), (size_t)n * sizeof(double));
       break;
     }
   }
@@ -2734,6 +2752,7 @@ int bsam_update_coeffs(bsam_hier* h, const bsam_hier_desc* d) {
   h->singular = singular;
   if (h->ctx->device < 0) {
     h->cst.M = M;
+    h->ainv_host = std::move(Ainv);
     return 0;
   }
   HIPCHK(hipSetDevice(h->ctx->device));
@@ -3041,19 +3060,37 @@ int bsam_parabolic_step(bsam_hier* h, double dt, int scheme, const bsam_solve_op
 // hierarchy runs its stage preamble or one whole cycle (same arithmetic as
 // the hierarchy's own solve), so requests of any size and level count share
 // the device without changing each other's results.
+// Batch items run as groups of workgroups: an item gets one workgroup per
+// kBatchCellsPerWg cells of its largest level, at most kBatchMaxWg and at
+// most what the device holds resident at once.  The count depends on the
+// hierarchy and the device alone, so an item's arithmetic (the order of its
+// residual partial sums included) is the same in any batch.
+constexpr long long kBatchCellsPerWg = 8192;
+constexpr int kBatchMaxWg = 16;
+
 struct bsam_batch {
   bsam_ctx* ctx = nullptr;
   int cap = 0;
+  int ncu = 0;
   hipStream_t stream = nullptr;
   DevBufs<char> mem;
   BatchItem* d_items = nullptr;
   ResidualOut* d_out = nullptr;
+  ResidualOut* d_parts = nullptr;  // kBatchMaxWg per item
+  int* d_grp = nullptr;            // workgroup -> item, kBatchMaxWg per item
+  unsigned* d_ctr = nullptr;       // one barrier counter per item
+  int* d_err = nullptr;
   BatchItem* h_items = nullptr;  // pinned
   ResidualOut* h_out = nullptr;  // pinned
+  int* h_grp = nullptr;          // pinned
+  int* h_err = nullptr;          // pinned
+  int bpc_m = -1, bpc = 0;       // resident workgroups per CU for coarsest order bpc_m
   ~bsam_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
     if (h_items) (void)hipHostFree(h_items);
     if (h_out) (void)hipHostFree(h_out);
+    if (h_grp) (void)hipHostFree(h_grp);
+    if (h_err) (void)hipHostFree(h_err);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -3065,11 +3102,19 @@ int bsam_batch_create(bsam_ctx* ctx, int capacity, bsam_batch** out) {
   auto b = std::make_unique<bsam_batch>();
   b->ctx = ctx;
   b->cap = capacity;
+  HIPCHK(hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
   int rc;
+  const size_t nw = (size_t)capacity * kBatchMaxWg;
   if ((rc = b->mem.alloc(&b->d_items, capacity))) return rc;
   if ((rc = b->mem.alloc(&b->d_out, capacity))) return rc;
+  if ((rc = b->mem.alloc(&b->d_parts, nw))) return rc;
+  if ((rc = b->mem.alloc(&b->d_grp, nw))) return rc;
+  if ((rc = b->mem.alloc(&b->d_ctr, capacity))) return rc;
+  if ((rc = b->mem.alloc(&b->d_err, 1))) return rc;
   HIPCHK(hipHostMalloc((void**)&b->h_items, capacity * sizeof(BatchItem), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void**)&b->h_out, capacity * sizeof(ResidualOut), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&b->h_grp, nw * sizeof(int), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void**)&b->h_err, sizeof(int), hipHostMallocDefault));
   HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   *out = b.release();
   return 0;
@@ -3095,25 +3140,67 @@ int bsam_batch_step(bsam_batch* b, bsam_hier* const* hs, const int* phase, int n
       if (hs[j] == h) return fail(BSAM_E_ARG, "bsamgpu:arg: hierarchy listed twice in one batch step");
     if (mixed)
       if (int rc = ensure32(h)) return rc;
+    max_m = std::max(max_m, h->cst.M);
+  }
+  // resident workgroups of the launch (every group's workgroups must run at once)
+  if (b->bpc_m != max_m) {
+    b->bpc = batch_cycle_blocks_per_cu(max_m);
+    b->bpc_m = max_m;
+  }
+  const int cap_wg = b->bpc * b->ncu;
+  if (cap_wg < 1) return fail(BSAM_E_HIP, "bsamgpu:hip: the batch kernel cannot be resident on this device");
+  // groups, then launches of whole groups that fit the resident capacity
+  std::vector<int> first_item{0}, first_wg{0};
+  int wg = 0, wg_chunk = 0;
+  for (int i = 0; i < n; ++i) {
+    bsam_hier* h = hs[i];
+    long long cells = 1;
+    for (int k = 0; k < h->nlev; ++k) cells = std::max(cells, (long long)h->H[k].ncell);
+    const int g = (int)std::min<long long>({(cells + kBatchCellsPerWg - 1) / kBatchCellsPerWg,
+                                            (long long)kBatchMaxWg, (long long)cap_wg});
+    if (wg_chunk + g > cap_wg) {
+      first_item.push_back(i);
+      first_wg.push_back(wg);
+      wg_chunk = 0;
+    }
     BatchItem& it = b->h_items[i];
     it.LV64 = h->lv64;
     it.LV32 = mixed ? h->lv32 : nullptr;
     it.rout32 = mixed ? h->rout32 : nullptr;
     it.out = b->d_out + i;
+    it.parts = b->d_parts + (size_t)i * kBatchMaxWg;
     it.cst = h->cst;
     it.nlev = h->nlev;
     it.root = h->root;
     it.phase = phase[i];
+    it.wg0 = wg_chunk;
+    it.nwg = g;
     it.pad = 0;
-    max_m = std::max(max_m, h->cst.M);
+    for (int w = 0; w < g; ++w) b->h_grp[wg + w] = i;
+    wg += g;
+    wg_chunk += g;
   }
+  first_item.push_back(n);
+  first_wg.push_back(wg);
   hipStream_t s = b->stream;
   HIPCHK(hipMemcpyAsync(b->d_items, b->h_items, n * sizeof(BatchItem), hipMemcpyHostToDevice, s));
-  launch_batch_cycle(b->d_items, n, max_m, o->npre, o->npost, o->omega, o->mass_correction, mixed ? 1 : 0,
-                     std::max(1, o->inner_cycles), s);
+  HIPCHK(hipMemcpyAsync(b->d_grp, b->h_grp, (size_t)wg * sizeof(int), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(b->d_ctr, 0, n * sizeof(unsigned), s));
+  HIPCHK(hipMemsetAsync(b->d_err, 0, sizeof(int), s));
+  for (size_t c = 0; c + 1 < first_item.size(); ++c) {
+    const int e = launch_batch_cycle(b->d_items, b->d_grp + first_wg[c], first_wg[c + 1] - first_wg[c], b->d_ctr,
+                                     b->d_err, max_m, o->npre, o->npost, o->omega, o->mass_correction,
+                                     mixed ? 1 : 0, std::max(1, o->inner_cycles), s);
+    if (e != (int)hipSuccess)
+      return fail(BSAM_E_HIP, "bsamgpu:hip: batch launch of %d workgroups failed: %s",
+                  first_wg[c + 1] - first_wg[c], hipGetErrorString((hipError_t)e));
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(b->h_out, b->d_out, n * sizeof(ResidualOut), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(b->h_err, b->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (*b->h_err)
+    return fail(BSAM_E_HIP, "bsamgpu:hip: a batch group's barrier gave up waiting (workgroups not resident)");
   for (int i = 0; i < n; ++i) {
     const ResidualOut& r = b->h_out[i];
     out[i].linf = r.linf;

@@ -326,7 +326,9 @@ int bsam_set_level(bsam_hier *h, int k, int field, const double *in);
  *           [ntile, passes covered, then per tile: canvas x0 y0 width height,
  *           patches np, depth-0 cells, colour-1 cells, colour-2 cells,
  *           (patch ox oy) x np, the cell entries: patch slot (bits 0-3) |
- *           i (4-15) | j (16-27) | depth (28-31)]; [0, 0] when none */
+ *           i (4-15) | j (16-27) | depth (28-31)]; [0, 0] when none
+ * what = 6: k = 1 only: the bit patterns of the coarsest inverse (M x M,
+ *           column-major; the device copy on a device context) */
 long long bsam_export_size(const bsam_hier *h, int k, int what);
 int bsam_export(const bsam_hier *h, int k, int what, long long *out);
 

@@ -322,3 +322,18 @@ def test_streaming_sweep_solve_equals_oracle():
     assert st["resHist"] == so["resHist"]
     _cmp_all(Orc, G, "streaming solve")
     G.close()
+
+
+@pytest.mark.parametrize("name", ["plain", "neumann"])
+def test_device_coarsest_inverse_is_the_host_inverse(name):
+    """The coarsest inverse the device applies is the host routine's, bit
+    for bit (the upload path), for a regular and a KKT system."""
+    H = CASES[name](seed=None)
+    print(f"case {name}", flush=True)
+    G = GpuHierarchy(H)
+    Hh = GpuHierarchy(H, device=-1)
+    dev, host = G.export(1, 6), Hh.export(1, 6)
+    assert dev.size == host.size and dev.size >= 256 * 256
+    assert np.array_equal(dev, host)
+    G.close()
+    Hh.close()

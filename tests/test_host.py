@@ -188,3 +188,16 @@ def test_update_coeffs_validates_structure_and_values():
     with pytest.raises(BsamError) as e:
         G.update_coeffs(bad)
     assert "boundary type" in str(e.value)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_host_coarsest_inverse_equals_oracle(name):
+    """The library's coarsest inverse (bsam_lu.cpp, host code) equals the
+    NumPy oracle's lu_inverse bit for bit: regular and KKT (all-Neumann)
+    systems of order 256 / 257."""
+    H = CASES[name](seed=None)
+    Hh = GpuHierarchy(H, device=-1)
+    got = Hh.export(1, 6)
+    Hh.close()
+    ref = np.asfortranarray(O.OracleHierarchy(H).coarsest["Ainv"]).ravel(order="F").view(np.int64)
+    assert got.size == ref.size and np.array_equal(got, ref)
