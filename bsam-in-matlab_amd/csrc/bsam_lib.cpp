@@ -167,6 +167,8 @@ struct bsam_hier {
   hipGraphExec_t gexec = nullptr;
   std::vector<double> graph_key;          // the options the captured cycle was built with
   ~bsam_hier() {
+    // nothing the stream still runs may outlive the graph, pinned buffer or memory it reads
+    if (stream) (void)hipStreamSynchronize(stream);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     if (hres) (void)hipHostFree(hres);

@@ -399,9 +399,13 @@ int bsam_parabolic_step(bsam_hier *h, double dt, int scheme, const bsam_solve_op
 
 /* Batched stage steps for serving many independent hierarchies
  * (no counterpart in the reference, which solves one hierarchy at a time).
- * One launch advances n hierarchies of the batch's device; each runs in its
- * own workgroup with exactly the arithmetic of its own bsam_solve_cycles, so
- * results do not depend on what else is in the batch.  phase[i] = 0 runs the
+ * One cooperative launch advances n hierarchies of the batch's device; each
+ * runs on its own group of workgroups (one per 8192 cells of its largest
+ * level, at most 16; groups that do not fit the device's resident capacity
+ * go to a following launch) with exactly the arithmetic of its own
+ * bsam_solve_cycles, so results do not depend on what else is in the batch.
+ * A group barrier that waits too long fails the call with BSAM_E_HIP instead
+ * of hanging.  phase[i] = 0 runs the
  * stage preamble of ebSolveCycles.m:11-19 (fill every level, baseline
  * residual), 1 runs one cycle of ebSolveCycles.m:28-41 (reset F, V-cycle,
  * filldown, residual; with opts->precision == BSAM_MIXED the FP32-inner
