@@ -142,9 +142,38 @@ constexpr int kRegionKL = 1;          // depth-0 cells per lane
 // unused ones kRegionNone; the patches of a tile take kRegionMaxPatches slots
 constexpr int kRegionStride = kRegionThreads * (2 * kRegionK + kRegionKL);
 constexpr unsigned kRegionNone = 0xffffffffu;
+// depth code of a cell updated in every pass but written by another tile: a
+// source of a ghost or restriction that this tile writes
+constexpr int kRegionVirtual = 15;
 struct alignas(16) RegionTile {
   int cx0, cy0, cw, ch;   // canvas origin (0-based level index) and size
   int np, nload, nred, nblack;
+  int gofs, ngh;          // the tile's ghost records in the level's rghost list
+  int rofs, nrs;          // the coarse cells it restricts in the level's rrs list
+};
+// A ghost record the tile evaluates after the last pass (ebFillGhosts from
+// its final values), with everything but the parent parts resolved on the
+// host: the record's kind and boundary value (for a physical corner, those of
+// its two side records), the aux entries holding the parent parts (-1: none),
+// and the canvas slots of its fine sources -- [0..1] the record's own; for a
+// physical corner [0..1] and [2..3] those of its side records and [4] the
+// diagonal cell
+constexpr unsigned short kRegionNoSlot = 0xffff;
+struct alignas(16) RegionGhost {
+  double c, ca, cb;   // boundary values (G_DIR: g, G_NEU: h g) of the record / its sides
+  int dst;            // Q index written
+  int kinds;          // kind | side A's kind << 4 | side B's kind << 8
+  int rec, reca, recb;// aux entries of the parent parts (-1: none)
+  unsigned s01, s23, s45;  // slots 0..5, two per word (low half first)
+};
+__host__ __device__ inline unsigned region_slot(const RegionGhost& g, int k) {
+  const unsigned w = k < 2 ? g.s01 : k < 4 ? g.s23 : g.s45;
+  return (k & 1) ? (w >> 16) : (w & 0xffffu);
+}
+// A coarse cell the tile restricts: QC index, parent Q index, canvas slot of
+// the block's first fine cell
+struct alignas(16) RegionRestrict {
+  int qc, pq, slot, pad;
 };
 struct RegionPatch {
   PatchInfo pi;  // the patch record
@@ -180,6 +209,9 @@ struct LevelDev {
   RegionTile* region;
   RegionPatch* rpatch;    // kRegionMaxPatches per tile
   unsigned* rcell;        // kRegionStride per tile
+  RegionGhost* rghost;    // per tile (RegionTile::gofs, ngh)
+  RegionRestrict* rrs;    // per tile (RegionTile::rofs, nrs)
+  int rghmax, rrsmax;     // the most ghosts / restricted cells of one tile
   PatchInfo* P;
   TileRec* tiles;    // patch record + first cell per 256-cell tile
   int2* ctiles;      // (patch, first coarse cell) per tile of the footprint

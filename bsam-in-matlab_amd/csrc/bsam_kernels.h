@@ -35,15 +35,20 @@ template <typename T> struct StreamRows { static constexpr int value = sizeof(T)
 template <typename T>
 void launch_sweep_stream(const LevelDev<T>& L, const PatchInfo& P, const T* Qin, T* Qout, T omega, int homog,
                          int ncu, int rows_req, hipStream_t s);
-// a whole relax call of nsweeps red+black sweeps on the level's region tiles
-// (levels with L.nregion, 2 nsweeps <= L.rdepth), own cells into L.Qs; frame:
-// pass 1 reads internal-face ghosts from the ghost frame; the parent parts
-// in L.aux must be those of the current parent.  launch_fill_copy
-// (Qsrc = L.Qs) must follow: it copies the interior into Q and fills the
-// ghosts from it
+// A whole relax call of nsweeps red+black sweeps on the level's region tiles
+// (levels with L.nregion, 2 nsweeps <= L.rdepth), own cells into Qout (null:
+// L.Qs); frame: pass 1 reads internal-face ghosts from the ghost frame; the
+// parent parts in L.aux must be those of the current parent.
+// ghosts = 0: launch_fill_copy (Qsrc = Qout) must follow, which copies the
+// interior into Q and fills the ghosts from it.  ghosts = 1: the launch also
+// writes every ghost of the level into Qout (the fill without its QC ring
+// refresh), so Qout holds the level's complete new state; ring adds the QC
+// ring refresh (from the parent); Rq (with Rdup) adds the restriction into
+// the parent and QC's interior.
 template <typename T>
 void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, int frame, T omega, int homog,
-                         hipStream_t s);
+                         hipStream_t s, T* Qout = nullptr, int ghosts = 0, T* Rq = nullptr, T* Rdup = nullptr,
+                         int ring = 0);
 template <typename T>
 void launch_fill_copy(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, const T* Qsrc, hipStream_t s);
 // the fill after a region relax (parent parts from aux, no QC ring refresh)

@@ -771,9 +771,62 @@ __device__ __forceinline__ T region_update(const LevelDev<T>& L, const T* cv, in
 // parts of coarse-fine faces come from the face records' aux entries, which
 // must hold those of the current parent (a fill, a preparing correction or
 // launch_refresh_cf wrote them after the parent last changed).
+// the epilogue's data of one ghost record, staged in LDS during the prologue
+template <typename T>
+struct GhostOp {
+  T c, cf, ca, cfa, cb, cfb;
+  int dst, kinds;
+  unsigned s01, s23, s4;
+};
+
+// A ghost record's value from its kind, boundary value c, fine sources v0,
+// v1 and parent part cf: the expressions of ghost_value, in the same order
+template <typename T>
+__device__ __forceinline__ T region_ghost(int kind, T c, T cf, T v0, T v1, int homog) {
+  switch (kind) {
+    case G_COPY:
+      return v0;
+    case G_QIF:
+    case G_QIF_END:
+      return (T(A1_) * v0 + T(A2_) * v1) + cf;
+    case G_LIF:
+      return (cf + T(C23_) * v0) - T(C13_) * v1;
+    case G_LIF_CORNER:
+      return cf + T(C13_) * v0;
+    case G_DIR: {
+      const T g = homog ? T(0) : c;
+      return (T(C83_) * g - T(2) * v0) + T(C13_) * v1;
+    }
+    case G_NEU: {
+      const T hg = homog ? T(0) : c;
+      return v0 + hg;
+    }
+    default:
+      return T(0);
+  }
+}
+
+// dynamic LDS of a region launch: the canvas, then (ghosts) the tile's ghost
+// records, (ring) its staged slice of the QC ring refresh
+__host__ __device__ inline size_t region_align16(size_t n) { return (n + 15) & ~(size_t)15; }
+template <typename T>
+__host__ __device__ inline size_t region_lds_bytes(int rslots, int rghmax, int rring, int ghosts, int ring) {
+  size_t b = region_align16((size_t)rslots * sizeof(T));
+  if (ghosts) b += region_align16((size_t)rghmax * sizeof(GhostOp<T>));
+  if (ring) b += region_align16((size_t)rring * sizeof(int)) + region_align16((size_t)rring * sizeof(T));
+  return b;
+}
+
+// Qout: where the own cells go.  With ghosts set, the tile also writes the
+// ghost records next to its own cells from the final canvas (what the fill
+// after the call would write; parent parts from aux), so Qout is the level's
+// complete new state.  Rq: when given, the restriction of the tile's coarse
+// cells into the parent (and Rdup) and QC's interior (ebTransfer.m:10-33).
+// ring: the QC ring refresh of the fill, from the parent Pq.
 template <typename T, int NT, int K, int KL>
 __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __restrict__ Pq, T* __restrict__ Qout,
-                                                     int npass, int frame, T omega, int homog, int per) {
+                                                     int npass, int frame, T omega, int homog, int per, int ghosts,
+                                                     T* __restrict__ Rq, T* __restrict__ Rdup, int ring) {
   extern __shared__ __align__(16) unsigned char region_lds[];
   T* cv = reinterpret_cast<T*>(region_lds);
   __shared__ RegionPatch sp[kRegionMaxPatches];
@@ -793,8 +846,37 @@ __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __r
   for (int k = 0; k < KL; ++k) e0[k] = cl[2 * NT * K + tid + k * NT];
   if (tid < kRegionMaxPatches) sp[tid] = L.rpatch[(size_t)t * kRegionMaxPatches + tid];
   const int cw = tr.cw;
+  // the epilogue's records (at most one per lane), loaded beside the cells
+  unsigned char* tail = region_lds + region_align16((size_t)L.rslots * sizeof(T));
+  GhostOp<T>* gop = reinterpret_cast<GhostOp<T>*>(tail);
+  if (ghosts) tail += region_align16((size_t)L.rghmax * sizeof(GhostOp<T>));
+  const int rring = ring ? (L.nring_cf + L.nregion - 1) / L.nregion : 0;  // ring items per tile
+  const int rstage = rring < NT ? rring : NT;  // staged ring items (one per lane)
+  int* rdst = reinterpret_cast<int*>(tail);
+  T* rval = reinterpret_cast<T*>(tail + region_align16((size_t)rstage * sizeof(int)));
+  const bool my_gh = ghosts && tid < tr.ngh, my_rs = Rq && tid < tr.nrs;
+  const int rx = t * rring + tid;
+  const bool my_ring = ring && tid < rstage && rx < L.nring_cf;
+  RegionGhost ge;
+  RingRec rr;
+  int rs_qc = 0, rs_pq = 0, rs_slot = 0;  // this lane's coarse cell (registers until the epilogue)
+  if (my_gh) ge = L.rghost[tr.gofs + tid];
+  if (my_rs) {
+    const RegionRestrict& re = L.rrs[tr.rofs + tid];
+    rs_qc = re.qc;
+    rs_pq = re.pq;
+    rs_slot = re.slot;
+  }
+  if (my_ring) rr = L.ring_cf[rx];
   for (int x = tid; x < cw * tr.ch; x += NT) cv[x] = T(0);
   __syncthreads();
+  T gcf = T(0), gcfa = T(0), gcfb = T(0), rv = T(0);
+  if (my_gh) {
+    if (ge.rec >= 0) gcf = L.aux[ge.rec].cf;
+    if (ge.reca >= 0) gcfa = L.aux[ge.reca].cf;
+    if (ge.recb >= 0) gcfb = L.aux[ge.recb].cf;
+  }
+  if (my_ring) rv = Pq[rr.src];
   // every cell's loads, then their uses
   RegionCell<T> c1[K], c2[K];
   T q1[K], q2[K], fx1[K], fy1[K], fdx, fdy, q0[KL];
@@ -822,6 +904,21 @@ __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __r
     if (c1[k].slot >= 0) cv[c1[k].slot] = q1[k];
     if (c2[k].slot >= 0) cv[c2[k].slot] = q2[k];
   }
+  if (my_gh) {
+    GhostOp<T> o;
+    o.c = T(ge.c); o.ca = T(ge.ca); o.cb = T(ge.cb);
+    o.cf = gcf; o.cfa = gcfa; o.cfb = gcfb;
+    o.dst = ge.dst;
+    o.kinds = ge.kinds;
+    o.s01 = ge.s01;
+    o.s23 = ge.s23;
+    o.s4 = ge.s45 & 0xffffu;
+    gop[tid] = o;
+  }
+  if (my_ring) {
+    rdst[tid] = rr.dst;
+    rval[tid] = rv;
+  }
   __syncthreads();
   const int dshift = L.rdepth - npass;
   for (int p = 1; p <= npass; p += 2) {
@@ -841,6 +938,36 @@ __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __r
     if (c1[k].slot >= 0 && region_depth(c1[k].fl) == L.rdepth) Qout[qc1[k]] = cv[c1[k].slot];
     if (c2[k].slot >= 0 && region_depth(c2[k].fl) == L.rdepth) Qout[qc2[k]] = cv[c2[k].slot];
   }
+  if (my_gh) {  // one ghost record per lane (at most NT per tile)
+    const GhostOp<T> o = gop[tid];
+    const int kind = o.kinds & 15;
+    const unsigned a0 = o.s01 & 0xffffu, a1 = o.s01 >> 16;
+    const T v0 = cv[a0], v1 = a1 != kRegionNoSlot ? cv[a1] : T(0);
+    T v;
+    if (kind == G_PCORNER) {
+      const unsigned b0 = o.s23 & 0xffffu, b1 = o.s23 >> 16;
+      const T a = region_ghost<T>((o.kinds >> 4) & 15, o.ca, o.cfa, v0, v1, homog);
+      const T b = region_ghost<T>((o.kinds >> 8) & 15, o.cb, o.cfb, cv[b0], b1 != kRegionNoSlot ? cv[b1] : T(0),
+                                  homog);
+      v = (a + b) - cv[o.s4];
+    } else {
+      v = region_ghost<T>(kind, o.c, o.cf, v0, v1, homog);
+    }
+    Qout[o.dst] = v;
+  }
+  if (my_rs) {  // one coarse cell per lane (at most NT per tile)
+    const int sl = rs_slot;
+    const T R = T(0.25) * (((cv[sl] + cv[sl + 1]) + cv[sl + cw]) + cv[sl + cw + 1]);
+    L.QC[rs_qc] = R;
+    Rq[rs_pq] = R;
+    if (Rdup) Rdup[rs_pq] = R;
+  }
+  if (my_ring) L.QC[rdst[tid]] = rval[tid];
+  if (ring)  // a slice longer than the workgroup: the rest directly
+    for (int x = NT + tid; x < rring && t * rring + x < L.nring_cf; x += NT) {
+      const RingRec q = L.ring_cf[t * rring + x];
+      L.QC[q.dst] = Pq[q.src];
+    }
 }
 
 // ebFillGhosts after a region relax: the ghosts from the new interior values
@@ -1400,6 +1527,13 @@ __device__ __forceinline__ void correct_item(const LevelDev<T>& L, const T* __re
   if (fe) { keep_face_cf<T>(L, Pq, ie); pe = L.aux[ie].push; }
   if (fs) { keep_face_cf<T>(L, Pq, is); ps = L.aux[is].push; }
   if (fn) { keep_face_cf<T>(L, Pq, in_); pn = L.aux[in_].push; }
+  if (prep) {  // and those of the corner records, from the corner cells (a region launch's ghosts read them)
+    const int c0 = pi.gbase + 2 * my + 2 * mx;
+    if (i == 0 && j == 0) keep_face_cf<T>(L, Pq, c0);
+    if (i == mx - 1 && j == 0) keep_face_cf<T>(L, Pq, c0 + 1);
+    if (i == 0 && j == my - 1) keep_face_cf<T>(L, Pq, c0 + 2);
+    if (i == mx - 1 && j == my - 1) keep_face_cf<T>(L, Pq, c0 + 3);
+  }
   const int ldc = mx / 2 + 2;
   const int w0 = pi.qcoff + ((i >> 1) + 1) + ((j >> 1) + 1) * ldc;
   const int dx = (i & 1) ? 1 : -1;
@@ -2404,14 +2538,22 @@ void launch_fill(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, hip
 
 template <typename T>
 void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, int frame, T omega,
-                         int homog, hipStream_t s) {
+                         int homog, hipStream_t s, T* Qout, int ghosts, T* Rq, T* Rdup, int ring) {
   if (!L.nregion || nsweeps <= 0) return;
   const int per = (L.nregion + kXcds - 1) / kXcds;
-  const size_t lds = (size_t)L.rslots * sizeof(T);
+  const int rg = (ring && parent) ? 1 : 0;
+  const int rring = rg ? (L.nring_cf + L.nregion - 1) / L.nregion : 0;
+  const size_t lds = region_lds_bytes<T>(L.rslots, L.rghmax, std::min(rring, kRegionThreads), ghosts, rg);
+  static bool raised = false;  // the canvas plus staged records can pass 64 KB (gfx950: 160 KB per CU)
+  if (!raised) {
+    (void)hipFuncSetAttribute((const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStageLdsLimit);
+    raised = true;
+  }
   hipLaunchKernelGGL((k_relax_region<T, kRegionThreads, kRegionK, kRegionKL>), dim3(per * kXcds), dim3(kRegionThreads),
                      lds, s,
-                     L, parent ? (const T*)parent->Q : (const T*)nullptr, L.Qs, 2 * nsweeps, frame, omega, homog,
-                     per);
+                     L, parent ? (const T*)parent->Q : (const T*)nullptr, Qout ? Qout : L.Qs, 2 * nsweeps, frame,
+                     omega, homog, per, ghosts, Rq, Rdup, (ring && parent) ? 1 : 0);
 }
 
 template <typename T>
@@ -2803,7 +2945,8 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template bool tiled_fits<T>(const LevelDev<T>&);                                                      \
   template void launch_relax_tiled<T>(const LevelDev<T>&, const LevelDev<T>*, int, T, int, hipStream_t); \
   template void launch_fill<T>(const LevelDev<T>&, const LevelDev<T>*, int, hipStream_t);               \
-  template void launch_relax_region<T>(const LevelDev<T>&, const LevelDev<T>*, int, int, T, int, hipStream_t); \
+  template void launch_relax_region<T>(const LevelDev<T>&, const LevelDev<T>*, int, int, T, int, hipStream_t, T*, int, \
+                                       T*, T*, int);                                                        \
   template void launch_fill_copy<T>(const LevelDev<T>&, const LevelDev<T>*, int, const T*, hipStream_t); \
   template void launch_fill_copy_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, const T*, T*,        \
                                              hipStream_t);                                                  \

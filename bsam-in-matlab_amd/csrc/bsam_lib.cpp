@@ -521,7 +521,9 @@ struct RegionBuild {
   std::vector<RegionPatch> patches;  // kRegionMaxPatches per tile
   std::vector<int> pid;              // their patch indices (-1: unused slot)
   std::vector<unsigned> cells;       // kRegionStride per tile
-  int max_slots = 0;
+  std::vector<RegionGhost> ghosts;   // per tile, RegionTile::gofs / ngh
+  std::vector<RegionRestrict> rs;    // per tile, RegionTile::rofs / nrs
+  int max_slots = 0, max_gh = 0, max_rs = 0;
 };
 
 bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
@@ -628,6 +630,44 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
         }
   }
   std::vector<int> depth_of(ncell, -1);  // reset after each draft through `order`
+  std::vector<char> mark(ncell, 0);      // 1: own, 2: virtual own (reset after each draft)
+  // level cell of an interior Q index (-1: a ghost slot)
+  std::vector<int> qcell((size_t)std::max(1, L.nq), -1);
+  for (int p = 0; p < L.np; ++p)
+    for (int j = 0; j < L.pi[p].my; ++j)
+      for (int i = 0; i < L.pi[p].mx; ++i) qcell[(size_t)qidx(L.pi[p], i + 1, j + 1)] = cell_ix[(size_t)cell_of(p, i, j)];
+  // footprint item of each coarse cell, keyed by its first fine cell
+  std::vector<int> foot_at(ncell, -1);
+  for (size_t f = 0; f < L.foot.size(); ++f) {
+    const FootRec& fr = L.foot[f];
+    foot_at[(size_t)cell_ix[(size_t)cell_of(fr.p, 2 * fr.ci, 2 * fr.cj)]] = (int)f;
+  }
+  // the ghost records next to interior cell (p, i, j): its face positions and
+  // corner (records per patch: W my, E my, S mx, N mx, then the 4 corners)
+  auto cell_records = [&](int p, int i, int j, std::vector<int>& recs) {
+    const PatchInfo& P = L.pi[p];
+    const int b = P.gbase, mx = P.mx, my = P.my;
+    if (i == 0) recs.push_back(b + j);
+    if (i == mx - 1) recs.push_back(b + my + j);
+    if (j == 0) recs.push_back(b + 2 * my + i);
+    if (j == my - 1) recs.push_back(b + 2 * my + mx + i);
+    const int c0 = b + 2 * my + 2 * mx;
+    if (i == 0 && j == 0) recs.push_back(c0);
+    if (i == mx - 1 && j == 0) recs.push_back(c0 + 1);
+    if (i == 0 && j == my - 1) recs.push_back(c0 + 2);
+    if (i == mx - 1 && j == my - 1) recs.push_back(c0 + 3);
+  };
+  // the fine Q indices a record's value reads (-1 entries unused); false for
+  // a kind the tile does not evaluate
+  auto rec_sources = [&](int rec, int q[2]) -> bool {
+    const GhostRec& r = L.ghost[(size_t)rec];
+    q[0] = q[1] = -1;
+    switch (r.kind) {
+      case G_COPY: case G_LIF_CORNER: case G_NEU: q[0] = r.s[0]; return true;
+      case G_QIF: case G_QIF_END: case G_LIF: case G_DIR: q[0] = r.s[0]; q[1] = r.s[1]; return true;
+      default: return false;
+    }
+  };
   // a tile drafted from a set of own cells (level cell indices)
   struct Draft {
     std::vector<int> own;
@@ -635,14 +675,57 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
     std::vector<int> pl;  // region patches
     std::vector<unsigned> load, red, black;
     std::vector<int> rcell, rdep;  // the region's cells and their depths
+    std::vector<RegionGhost> gh;   // ghost records the tile writes
+    std::vector<RegionRestrict> rs;// coarse cells it restricts
   };
   const int per_colour = kRegionThreads * kRegionK, per_load = kRegionThreads * kRegionKL;
-  // false when the region does not fit a workgroup
+  // false when the region does not fit a workgroup (or a ghost record is of
+  // a kind the tile does not evaluate: `bad`)
+  bool bad = false;
   auto draft = [&](Draft& D) -> bool {
     std::vector<int> order;  // level cell indices of the region, in discovery order
     for (int c : D.own) {
       depth_of[c] = rdepth;
+      mark[c] = 1;
       order.push_back(c);
+    }
+    auto need_full = [&](int c) {  // a cell this tile's writes read at its final value
+      if (c < 0 || mark[c]) return;
+      mark[c] = 2;
+      depth_of[c] = rdepth;
+      order.push_back(c);
+    };
+    // ghost records next to own cells, and the coarse cells whose first fine
+    // cell is own: their fine sources are needed at their final values
+    std::vector<int> recs, blocks;
+    for (int c : D.own) {
+      cell_records(cpatch[c], ci_[c], cj_[c], recs);
+      if (foot_at[c] >= 0) blocks.push_back(c);
+    }
+    for (int rec : recs) {
+      const GhostRec& r = L.ghost[(size_t)rec];
+      int q[2];
+      if (r.kind == G_PCORNER) {
+        for (int side = 0; side < 2; ++side) {
+          if (!rec_sources(r.s[side], q)) { bad = true; return false; }
+          for (int t = 0; t < 2; ++t)
+            if (q[t] >= 0) need_full(qcell[(size_t)q[t]]);
+        }
+        need_full(qcell[(size_t)r.s[2]]);
+      } else {
+        if (!rec_sources(rec, q)) { bad = true; return false; }
+        for (int t = 0; t < 2; ++t)
+          if (q[t] >= 0) {
+            if (qcell[(size_t)q[t]] < 0) { bad = true; return false; }
+            need_full(qcell[(size_t)q[t]]);
+          }
+      }
+    }
+    for (int c : blocks) {
+      const int p = cpatch[c], i = ci_[c], j = cj_[c];
+      need_full(cell_ix[(size_t)cell_of(p, i + 1, j)]);
+      need_full(cell_ix[(size_t)cell_of(p, i, j + 1)]);
+      need_full(cell_ix[(size_t)cell_of(p, i + 1, j + 1)]);
     }
     // pass q updates colour (q - 1) & 1 (0: colour 1)
     for (int q = rdepth; q >= 1; --q) {
@@ -670,6 +753,10 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
       gy0 = std::min(gy0, gj); gy1 = std::max(gy1, gj);
     }
     D.cx0 = gx0 - 1; D.cy0 = gy0 - 1; D.cw = gx1 - gx0 + 3; D.ch = gy1 - gy0 + 3;
+    auto slot_of = [&](int c) -> unsigned short {
+      const int gi = (int)(gid[c] % n0), gj = (int)(gid[c] / n0);
+      return (unsigned short)((gi - D.cx0) + (gj - D.cy0) * D.cw);
+    };
     // level cell indices run in patch order, row by row: sorting them keeps
     // each list in (patch, j, i) order, the cells off every face of their
     // patch first (their updates take the kernel's short path, wave by wave)
@@ -680,7 +767,7 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
     D.rcell = order;
     D.rdep.resize(order.size());
     for (size_t x = 0; x < order.size(); ++x) D.rdep[x] = depth_of[order[x]];
-    bool fit = true;
+    bool fit = (long long)D.cw * D.ch <= kRegionMaxSlots;
     for (int c : sorted) {
       const int cp = cpatch[c], ci = ci_[c], cj = cj_[c];
       int ps = -1;
@@ -691,15 +778,69 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
         D.pl.push_back(cp);
         ps = (int)D.pl.size() - 1;
       }
-      const int d = depth_of[c];
+      const int d = mark[c] == 2 ? kRegionVirtual : depth_of[c];
       const unsigned code = region_code(ps, ci, cj, d);
       if (d == 0) D.load.push_back(code);
       else if (colour[c] == 0) D.red.push_back(code);
       else D.black.push_back(code);
     }
-    for (int c : order) depth_of[c] = -1;
-    return fit && (long long)D.cw * D.ch <= kRegionMaxSlots && (int)D.red.size() <= per_colour &&
-           (int)D.black.size() <= per_colour && (int)D.load.size() <= per_load;
+    D.gh.clear();
+    D.rs.clear();
+    if (fit) {
+      auto is_cf = [](int kind) { return kind == G_QIF || kind == G_QIF_END || kind == G_LIF || kind == G_LIF_CORNER; };
+      for (int rec : recs) {
+        const GhostRec& r = L.ghost[(size_t)rec];
+        RegionGhost g;
+        std::memset(&g, 0, sizeof g);
+        g.dst = r.dst;
+        g.rec = g.reca = g.recb = -1;
+        unsigned short sl[6];
+        for (int t = 0; t < 6; ++t) sl[t] = kRegionNoSlot;
+        int q[2];
+        if (r.kind == G_PCORNER) {
+          const GhostRec& ra = L.ghost[(size_t)r.s[0]];
+          const GhostRec& rb = L.ghost[(size_t)r.s[1]];
+          g.kinds = G_PCORNER | (ra.kind << 4) | (rb.kind << 8);
+          g.ca = ra.c;
+          g.cb = rb.c;
+          if (is_cf(ra.kind)) g.reca = r.s[0];
+          if (is_cf(rb.kind)) g.recb = r.s[1];
+          for (int side = 0; side < 2; ++side) {
+            rec_sources(r.s[side], q);
+            for (int t = 0; t < 2; ++t)
+              if (q[t] >= 0) sl[2 * side + t] = slot_of(qcell[(size_t)q[t]]);
+          }
+          sl[4] = slot_of(qcell[(size_t)r.s[2]]);
+        } else {
+          g.kinds = r.kind;
+          g.c = r.c;
+          if (is_cf(r.kind)) g.rec = rec;
+          rec_sources(rec, q);
+          for (int t = 0; t < 2; ++t)
+            if (q[t] >= 0) sl[t] = slot_of(qcell[(size_t)q[t]]);
+        }
+        g.s01 = (unsigned)sl[0] | ((unsigned)sl[1] << 16);
+        g.s23 = (unsigned)sl[2] | ((unsigned)sl[3] << 16);
+        g.s45 = (unsigned)sl[4] | ((unsigned)sl[5] << 16);
+        D.gh.push_back(g);
+      }
+      for (int c : blocks) {
+        const FootRec& fr = L.foot[(size_t)foot_at[c]];
+        RegionRestrict e;
+        e.qc = fr.qcoff + (fr.ci + 1) + (fr.cj + 1) * (fr.mx / 2 + 2);
+        e.pq = fr.pq;
+        e.slot = slot_of(c);
+        e.pad = foot_at[c];
+        D.rs.push_back(e);
+      }
+    }
+    for (int c : order) {
+      depth_of[c] = -1;
+      mark[c] = 0;
+    }
+    return fit && (int)D.red.size() <= per_colour && (int)D.black.size() <= per_colour &&
+           (int)D.load.size() <= per_load && (int)recs.size() <= kRegionThreads &&
+           (int)blocks.size() <= kRegionThreads;
   };
   // 1. every patch, halved along its longer side until the region of every
   //    piece fits a workgroup
@@ -716,6 +857,7 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
         drafts.push_back(std::move(D));
         continue;
       }
+      if (bad) return false;
       const int w0 = r[2] - r[1], h0 = r[4] - r[3];
       if (w0 <= 2 && h0 <= 2) return false;
       // the second half is pushed first so that pieces stay in spatial order
@@ -822,6 +964,12 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
     rt.cx0 = D.cx0; rt.cy0 = D.cy0; rt.cw = D.cw; rt.ch = D.ch;
     rt.np = (int)D.pl.size();
     rt.nload = (int)D.load.size(); rt.nred = (int)D.red.size(); rt.nblack = (int)D.black.size();
+    rt.gofs = (int)out.ghosts.size(); rt.ngh = (int)D.gh.size();
+    rt.rofs = (int)out.rs.size(); rt.nrs = (int)D.rs.size();
+    out.ghosts.insert(out.ghosts.end(), D.gh.begin(), D.gh.end());
+    out.rs.insert(out.rs.end(), D.rs.begin(), D.rs.end());
+    out.max_gh = std::max(out.max_gh, rt.ngh);
+    out.max_rs = std::max(out.max_rs, rt.nrs);
     for (int k = 0; k < kRegionMaxPatches; ++k) {
       RegionPatch rp;
       std::memset(&rp, 0, sizeof rp);
@@ -842,6 +990,65 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
     out.max_slots = std::max(out.max_slots, D.cw * D.ch);
   }
   return !out.tiles.empty();
+}
+
+// Every index the region kernel dereferences, against its buffer: each cell
+// entry inside its tile's patches and canvas, each ghost record's
+// destination and slots, each footprint item and its block's slots; every
+// ghost record and footprint item is written by exactly one tile.
+int check_region_tables(const HLevel& L, const RegionBuild& rb) {
+  auto bad = [](const char* what) {
+    return fail(BSAM_E_ARG, "bsamgpu:internal: region table %s out of range", what);
+  };
+  std::vector<int> gseen(L.ghost.size(), 0), fseen(L.foot.size(), 0);
+  std::vector<int> rec_at((size_t)std::max(1, L.nq), -1);  // ghost record of each Q destination
+  for (size_t r = 0; r < L.ghost.size(); ++r)
+    if (L.ghost[r].dst >= 0 && L.ghost[r].dst < L.nq) rec_at[(size_t)L.ghost[r].dst] = (int)r;
+  for (size_t t = 0; t < rb.tiles.size(); ++t) {
+    const RegionTile& T = rb.tiles[t];
+    const long long slots = (long long)T.cw * T.ch;
+    if (T.cw < 3 || T.ch < 3 || slots > kRegionMaxSlots || T.np < 1 || T.np > kRegionMaxPatches) return bad("tile");
+    const unsigned* c = rb.cells.data() + t * kRegionStride;
+    const int per_colour = kRegionThreads * kRegionK;
+    for (int e = 0; e < kRegionStride; ++e) {
+      if (c[e] == kRegionNone) continue;
+      const int ps = c[e] & 15, i = (c[e] >> 4) & 4095, j = (c[e] >> 16) & 4095;
+      if (ps >= T.np) return bad("patch slot");
+      const RegionPatch& rp = rb.patches[t * kRegionMaxPatches + ps];
+      const int x = rp.ox + i, y = rp.oy + j;
+      if (i >= rp.pi.mx || j >= rp.pi.my || x < 1 || y < 1 || x > T.cw - 2 || y > T.ch - 2) return bad("cell");
+      (void)per_colour;
+    }
+    if (T.gofs < 0 || T.ngh < 0 || (size_t)T.gofs + T.ngh > rb.ghosts.size()) return bad("ghost list");
+    if (T.ngh > kRegionThreads || T.nrs > kRegionThreads) return bad("list length");
+    for (int g = 0; g < T.ngh; ++g) {
+      const RegionGhost& G = rb.ghosts[(size_t)T.gofs + g];
+      if (G.dst < 0 || G.dst >= L.nq) return bad("ghost destination");
+      for (int rec : {G.rec, G.reca, G.recb})
+        if (rec < -1 || rec >= (int)L.ghost.size()) return bad("ghost record");
+      const int kind = G.kinds & 15;
+      if (kind > G_PCORNER || region_slot(G, 0) == kRegionNoSlot ||
+          (kind == G_PCORNER && (region_slot(G, 2) == kRegionNoSlot || region_slot(G, 4) == kRegionNoSlot)))
+        return bad("ghost form");
+      for (int k = 0; k < 6; ++k)
+        if (region_slot(G, k) != kRegionNoSlot && (long long)region_slot(G, k) >= slots) return bad("ghost slot");
+      if (rec_at[(size_t)G.dst] < 0) return bad("ghost destination record");
+      ++gseen[(size_t)rec_at[(size_t)G.dst]];
+    }
+    if (T.rofs < 0 || T.nrs < 0 || (size_t)T.rofs + T.nrs > rb.rs.size()) return bad("restriction list");
+    for (int e = 0; e < T.nrs; ++e) {
+      const RegionRestrict& r = rb.rs[(size_t)T.rofs + e];
+      if (r.pad < 0 || (size_t)r.pad >= L.foot.size() || r.slot < 0 || (long long)r.slot + T.cw + 1 >= slots ||
+          r.qc < 0 || r.qc >= L.nqc)
+        return bad("restriction entry");
+      ++fseen[(size_t)r.pad];
+    }
+  }
+  for (int v : gseen)
+    if (v != 1) return bad("ghost assignment");
+  for (int v : fseen)
+    if (v != 1) return bad("restriction assignment");
+  return 0;
 }
 
 void build_phys_level(HLevel& L) {
@@ -1394,21 +1601,28 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
   // whole relax calls per launch (region tiles) on levels with the standard
   // face layout that do not stream; the FP32 copy shares the tables
   D.nregion = 0; D.rdepth = 0; D.rslots = 0;
-  D.region = nullptr; D.rpatch = nullptr; D.rcell = nullptr;
+  D.region = nullptr; D.rpatch = nullptr; D.rcell = nullptr; D.rghost = nullptr; D.rrs = nullptr;
+  D.rghmax = 0; D.rrsmax = 0;
   if (D.arith && !D.stream && L.np > 0) {
     if (tables_from64) {
       D.nregion = d64->nregion; D.rdepth = d64->rdepth; D.rslots = d64->rslots;
       D.region = d64->region; D.rpatch = d64->rpatch; D.rcell = d64->rcell;
+      D.rghost = d64->rghost; D.rrs = d64->rrs; D.rghmax = d64->rghmax; D.rrsmax = d64->rrsmax;
     } else {
       const int rdepth = 2 * std::max(3, std::max(h->opts.npre, h->opts.npost));
       RegionBuild rb;
       if (rdepth <= kRegionMaxDepth && build_region_tiles(L, rdepth, rb)) {
+        if ((rc = check_region_tables(L, rb))) return rc;
         if ((rc = upload(h, &D.region, rb.tiles))) return rc;
         if ((rc = upload(h, &D.rpatch, rb.patches))) return rc;
         if ((rc = upload(h, &D.rcell, rb.cells))) return rc;
+        if ((rc = upload(h, &D.rghost, rb.ghosts))) return rc;
+        if ((rc = upload(h, &D.rrs, rb.rs))) return rc;
         D.nregion = (int)rb.tiles.size();
         D.rdepth = rdepth;
         D.rslots = rb.max_slots;
+        D.rghmax = rb.max_gh;
+        D.rrsmax = rb.max_rs;
       }
     }
   }
@@ -1462,8 +1676,18 @@ struct Seq {
   const LevelDev<T>* LVd;  // device array of the same descriptors
   int homog;
   hipStream_t s;
-  const LevelDev<T>& lev(int k) { return L[k - 1]; }
-  const LevelDev<T>* par(int k) { return k > 1 ? &L[k - 2] : nullptr; }
+  // the level descriptors as the buffers stand: in the fused cycle a level
+  // that smooths with region launches swaps Q and Qs at every relax call
+  // (pre-smoothing into Qs, post-smoothing back into Q)
+  std::vector<LevelDev<T>> cur;
+  const LevelDev<T>& lev(int k) {
+    if (cur.empty()) cur = L;
+    return cur[k - 1];
+  }
+  const LevelDev<T>* par(int k) {
+    if (cur.empty()) cur = L;
+    return k > 1 ? &cur[k - 2] : nullptr;
+  }
   void fill(int k) { launch_fill<T>(lev(k), par(k), homog, s); }
   // ebRelaxLevel.m:28-38; first_inline: 1 = the first pass evaluates ghosts
   // from the records (the preceding fill was skipped), 2 = the preceding
@@ -1476,8 +1700,18 @@ struct Seq {
     const LevelDev<T>& Lk = L[k - 1];
     return nsweeps > 0 && Lk.nregion > 0 && 2 * nsweeps <= Lk.rdepth && h->opts.smoother == 0;
   }
-  // defer_fill: a region relax leaves its fill to the caller's fused launch
-  void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false, bool defer_fill = false) {
+  // both relax calls of level k in the fused cycle take region launches that
+  // write the level's whole new state into the other buffer
+  bool swaps(int k) const {
+    return h->opts.npre > 0 && h->opts.npost > 0 && region(k, h->opts.npre) && region(k, h->opts.npost);
+  }
+  // defer_fill: a region relax leaves its fill to the caller's fused launch.
+  // swap: 1 = a region launch writes the own cells and every ghost into Qs
+  // with the QC ring refresh (the fill), and the level's Q and Qs swap; 2 =
+  // the same with the restriction into the parent (and rdup) in place of the
+  // ring refresh, which the FAS load's gather redoes
+  void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false, bool defer_fill = false,
+             int swap = 0, T* rdup = nullptr) {
     const bool qs_current = qs_ready == k;
     qs_ready = 0;
     if (region(k, nsweeps)) {
@@ -1486,6 +1720,14 @@ struct Seq {
       // The parent parts in aux are current after a fill that followed the
       // parent's last change or a preparing correction; else refresh them
       if (first_inline == 1 || (first_inline == 0 && !cf_current)) launch_refresh_cf<T>(lev(k), par(k), s);
+      if (swap) {
+        const LevelDev<T>& Lk = lev(k);
+        T* rq = swap == 2 ? cur[k - 2].Q : nullptr;
+        launch_relax_region<T>(Lk, par(k), nsweeps, first_inline == 0 ? 1 : 0, (T)h->opts.omega, homog, s, Lk.Qs, 1,
+                               rq, swap == 2 ? rdup : nullptr, swap == 1 ? 1 : 0);
+        std::swap(cur[k - 1].Q, cur[k - 1].Qs);
+        return;
+      }
       launch_relax_region<T>(lev(k), par(k), nsweeps, first_inline == 0 ? 1 : 0, (T)h->opts.omega, homog, s);
       if (!defer_fill) launch_fill_copy<T>(lev(k), par(k), homog, lev(k).Qs, s);
       return;
@@ -1597,19 +1839,25 @@ struct Seq {
       coarsest();
       return;
     }
-    // below the top level the fill in the previous step is current.  After
-    // a region relax its fill and the restriction run as one launch (the
-    // fill's QC ring refresh is redone by the load's ring gather)
-    const bool fuse = region(k, h->opts.npre);
-    relax(k, h->opts.npre, 0, k < h->nlev, fuse);
+    // below the top level the fill in the previous step is current.  A level
+    // whose relax calls both run as region launches smooths into its other
+    // buffer with its ghosts and the restriction in the same launch;
+    // otherwise after a region relax its fill and the restriction run as one
+    // launch (the fill's QC ring refresh is redone by the load's ring gather)
+    const bool sw = swaps(k);
+    const bool fuse = !sw && region(k, h->opts.npre);
     T* dup = nullptr;
-    if (k <= h->root && k - 1 > kc && odd_stream(k - 1, h->opts.npre)) {
-      dup = lev(k - 1).Qs;
-      qs_ready = k - 1;
+    if (k <= h->root && k - 1 > kc && odd_stream(k - 1, h->opts.npre)) dup = lev(k - 1).Qs;
+    relax(k, h->opts.npre, 0, k < h->nlev, fuse, sw ? 2 : 0, dup);
+    if (dup) qs_ready = k - 1;
+    if (sw) {
+    } else if (fuse) {
+      launch_fill_copy_restrict<T>(lev(k), lev(k - 1), homog, lev(k).Qs, dup, s);
+    } else if (dup) {
+      launch_restrict<T>(lev(k), lev(k - 1), 0, s, dup);
+    } else {
+      restrict_(k);
     }
-    if (fuse) launch_fill_copy_restrict<T>(lev(k), lev(k - 1), homog, lev(k).Qs, dup, s);
-    else if (dup) launch_restrict<T>(lev(k), lev(k - 1), 0, s, dup);
-    else restrict_(k);
     fill(k - 1);
     launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s);
     vcycle(k - 1, kc);
@@ -1621,7 +1869,7 @@ struct Seq {
       correct(k, prep);
     }
     if (h->opts.npost > 0) {
-      relax(k, h->opts.npost, prep ? 2 : 1);
+      relax(k, h->opts.npost, prep ? 2 : 1, false, false, sw ? 1 : 0);
     } else {
       fill(k);
     }
@@ -1687,6 +1935,17 @@ static void region_export(const bsam_hier* h, const HLevel& L, std::vector<long 
     for (int e = 0; e < t.nload; ++e) v.push_back(c[2 * per_colour + e]);
     for (int e = 0; e < t.nred; ++e) v.push_back(c[e]);
     for (int e = 0; e < t.nblack; ++e) v.push_back(c[per_colour + e]);
+    v.push_back(t.ngh);
+    for (int g = 0; g < t.ngh; ++g) {
+      const RegionGhost& G = rb.ghosts[(size_t)t.gofs + g];
+      v.push_back(G.dst);
+      for (int k = 0; k < 6; ++k) v.push_back(region_slot(G, k) == kRegionNoSlot ? -1 : (long long)region_slot(G, k));
+    }
+    v.push_back(t.nrs);
+    for (int e = 0; e < t.nrs; ++e) {
+      v.push_back(rb.rs[(size_t)t.rofs + e].pad);
+      v.push_back(rb.rs[(size_t)t.rofs + e].slot);
+    }
   }
 }
 
@@ -2621,13 +2880,17 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
   const bool sweep = stream || (h->opts.smoother == 2 && h->d64[k - 1].nsweep > 0);
   const bool region = region_level(h, k);
   for (int i = 0; i < nlaunch; ++i) {
-    if (region) {  // one relax call of npre sweeps per launch, Q -> Qs
+    if (region) {
+      // one relax call of npre sweeps per launch, Q -> Qs, in the solve's
+      // form: with the level's ghosts and the QC ring refresh when the fused
+      // cycle swaps the level's buffers (pre- and post-smoothing both run)
+      const int gh = h->opts.npre > 0 && h->opts.npost > 0 ? 1 : 0;
       if (precision == BSAM_MIXED)
         launch_relax_region<float>(h->d32[k - 1], k > 1 ? &h->d32[k - 2] : nullptr, h->opts.npre, 0,
-                                   (float)h->opts.omega, 1, h->stream);
+                                   (float)h->opts.omega, 1, h->stream, nullptr, gh, nullptr, nullptr, gh);
       else
         launch_relax_region<double>(h->d64[k - 1], k > 1 ? &h->d64[k - 2] : nullptr, h->opts.npre, 0,
-                                    h->opts.omega, 0, h->stream);
+                                    h->opts.omega, 0, h->stream, nullptr, gh, nullptr, nullptr, gh);
       continue;
     }
     if (stream) {  // streaming sweeps, Q and Qs in turn

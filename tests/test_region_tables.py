@@ -41,8 +41,16 @@ def parse(v):
             ps, i, j, d = e & 15, (e >> 4) & 4095, (e >> 16) & 4095, (e >> 28) & 15
             cells.append((pats[ps][0], i, j, d))
         pos += nload + nred + nblack
+        ngh = v[pos]
+        pos += 1
+        ghosts = [(v[pos + 7 * g], v[pos + 7 * g + 1:pos + 7 * g + 7]) for g in range(ngh)]
+        pos += 7 * ngh
+        nrs = v[pos]
+        pos += 1
+        rs = [(v[pos + 2 * e], v[pos + 2 * e + 1]) for e in range(nrs)]
+        pos += 2 * nrs
         tiles.append(dict(cx0=cx0, cy0=cy0, cw=cw, ch=ch, pats=pats, cells=cells, nload=nload, nred=nred,
-                          nblack=nblack))
+                          nblack=nblack, ghosts=ghosts, rs=rs))
     assert pos == len(v)
     return rdepth, tiles
 
@@ -127,7 +135,8 @@ def test_region_depths_follow_the_pass_colours(name):
     for k, (rdepth, tiles) in _tables(H).items():
         L = H.L(k)
         for T in tiles:
-            depth = {(p, i, j): d for p, i, j, d in T["cells"]}
+            # kRegionVirtual (15): a cell another tile owns, updated in every pass
+            depth = {(p, i, j): (rdepth if d == 15 else d) for p, i, j, d in T["cells"]}
             for (p, i, j), d in depth.items():
                 colour = (i + j + L.par0[p]) & 1
                 mx, my = L.dims(p)
