@@ -225,6 +225,9 @@ struct LevelDev {
   int* tgt_ent;      // entries into level k+1's corr array
   int* tgt_of_cell;  // per cell: target row or -1
   int* wmap;         // per QC cell: parent Q index
+  int* wgr;          // per QC cell: the parent ghost record at wmap (-1: interior)
+  int* ring_rec;     // per ring_all item: the parent ghost record at its source (-1: interior)
+  int pgh_ok;        // every such record is physical (the load may evaluate it)
   uint8_t* vis;      // per cell: 1 if not covered by the next level
   T *Q, *F, *FT, *CC, *ID, *DeW, *DnS, *QC, *G;
   FaceAux<T>* aux;   // per ghost record (null: parent parts not kept)

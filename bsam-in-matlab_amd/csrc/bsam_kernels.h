@@ -88,7 +88,12 @@ template <typename T>
 void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s, T* dup = nullptr);
 // ebCoarseLoad alone (QC ring as stored) and the fused ring + load + reflux
 template <typename T> void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
-template <typename T> void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s);
+// pghost: the parent's frame was not filled after the restriction; the load
+// evaluates the parent's physical ghosts it reads (requires L.pgh_ok).  PPq
+// (the parent's parent Q): the same launch also stores the parent's parent
+// parts in its aux, as that fill would have
+template <typename T> void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s,
+                                       int pghost = 0, int homog = 0, const T* PPq = nullptr);
 // prep != 0 (levels with L.arith): also store the parent parts of the face
 // records and push corrected boundary values into sibling ghosts, for a
 // following smoother pass that assembles its ghosts (kInlArith); dup (may be

@@ -1478,8 +1478,26 @@ __device__ __forceinline__ int load_items(const LevelDev<T>& L, const LevelDev<T
   return L.nfoot + L.nring_all + (mass_corr ? P.ntgt : 0);
 }
 
+// A parent value the FAS load reads: Pq[q], or, when the parent's frame was
+// not filled (rec >= 0: a physical ghost record of the parent), the value
+// the fill would have written, from the parent's interior
 template <typename T>
-__device__ __forceinline__ void load_item(const LevelDev<T>& L, const LevelDev<T>& P, int t, int mass_corr) {
+__device__ __forceinline__ T parent_at(const LevelDev<T>& P, const T* __restrict__ Pq, int q, int rec, int homog) {
+  if (rec < 0) return Pq[q];
+  const GhostRec r = P.ghost[rec];
+  if (r.kind == G_PCORNER) {
+    const T a = ghost_value_cf<T>(P.ghost[r.s[0]], Pq, T(0), homog);
+    const T b = ghost_value_cf<T>(P.ghost[r.s[1]], Pq, T(0), homog);
+    return (a + b) - Pq[r.s[2]];
+  }
+  return ghost_value_cf<T>(r, Pq, T(0), homog);
+}
+
+// pghost: the parent's frame was not filled after the restriction, so its
+// physical ghosts are evaluated here (homog: homogeneous boundary data)
+template <typename T>
+__device__ __forceinline__ void load_item(const LevelDev<T>& L, const LevelDev<T>& P, int t, int mass_corr,
+                                          int pghost = 0, int homog = 0) {
   if (t < L.nfoot) {
     const FootRec fr = L.foot[t];
     const FootRec& pi = fr;  // patch offsets carried inline
@@ -1490,15 +1508,29 @@ __device__ __forceinline__ void load_item(const LevelDev<T>& L, const LevelDev<T
     const T* __restrict__ QC = L.QC;
     const T* __restrict__ Pq = P.Q;
     const T u = QC[w];
-    const T vw = fr.ci == 0 ? Pq[L.wmap[w - 1]] : QC[w - 1];
-    const T ve = fr.ci == cmx - 1 ? Pq[L.wmap[w + 1]] : QC[w + 1];
-    const T vs = fr.cj == 0 ? Pq[L.wmap[w - ldc]] : QC[w - ldc];
-    const T vn = fr.cj == cmy - 1 ? Pq[L.wmap[w + ldc]] : QC[w + ldc];
+    T vw, ve, vs, vn;
+    if (pghost) {
+      vw = fr.ci == 0 ? parent_at<T>(P, Pq, L.wmap[w - 1], L.wgr[w - 1], homog) : QC[w - 1];
+      ve = fr.ci == cmx - 1 ? parent_at<T>(P, Pq, L.wmap[w + 1], L.wgr[w + 1], homog) : QC[w + 1];
+      vs = fr.cj == 0 ? parent_at<T>(P, Pq, L.wmap[w - ldc], L.wgr[w - ldc], homog) : QC[w - ldc];
+      vn = fr.cj == cmy - 1 ? parent_at<T>(P, Pq, L.wmap[w + ldc], L.wgr[w + ldc], homog) : QC[w + ldc];
+    } else {
+      vw = fr.ci == 0 ? Pq[L.wmap[w - 1]] : QC[w - 1];
+      ve = fr.ci == cmx - 1 ? Pq[L.wmap[w + 1]] : QC[w + 1];
+      vs = fr.cj == 0 ? Pq[L.wmap[w - ldc]] : QC[w - ldc];
+      vn = fr.cj == cmy - 1 ? Pq[L.wmap[w + ldc]] : QC[w + ldc];
+    }
     const T Lc = op5<T>(u, vw, ve, vs, vn, P.DeW[fr.dew_w], P.DeW[fr.dew_e], P.DnS[fr.dns_s], P.DnS[fr.dns_n],
                         P.CC[fr.pc], level_h2<T>(P));
     P.F[fr.pc] = R + Lc;
   } else if (t < L.nfoot + L.nring_all) {
-    ring_item<T>(L, P.Q, t - L.nfoot);
+    if (pghost) {
+      const int x = t - L.nfoot;
+      const RingRec rr = L.ring_all[x];
+      L.QC[rr.dst] = parent_at<T>(P, P.Q, rr.src, L.ring_rec[x], homog);
+    } else {
+      ring_item<T>(L, P.Q, t - L.nfoot);
+    }
   } else {
     reflux_item<T>(L, P, t - L.nfoot - L.nring_all, 1);
   }
@@ -1627,9 +1659,17 @@ __global__ void __launch_bounds__(kTile) k_reflux(LevelDev<T> L, LevelDev<T> P) 
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kTile) k_load(LevelDev<T> L, LevelDev<T> P, int mass_corr) {
+__global__ void __launch_bounds__(kTile) k_load(LevelDev<T> L, LevelDev<T> P, int mass_corr, int pghost, int homog,
+                                                const T* __restrict__ PPq) {
   const int t = blockIdx.x * kTile + threadIdx.x;
-  if (t < load_items<T>(L, P, mass_corr)) load_item<T>(L, P, t, mass_corr);
+  const int n = load_items<T>(L, P, mass_corr);
+  if (t < n) {
+    load_item<T>(L, P, t, mass_corr, pghost, homog);
+  } else if (PPq && t < n + P.nghost) {
+    // the parent's parent parts from its own parent (what the left-out fill
+    // of the parent stores): the parent's pre-smoothing assembles with them
+    keep_face_cf<T>(P, PPq, t - n);
+  }
 }
 
 template <typename T>
@@ -2607,10 +2647,13 @@ void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mas
 }
 
 template <typename T>
-void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s) {
+void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s, int pghost,
+                 int homog, const T* PPq) {
   const long long n = (long long)L.nfoot + L.nring_all + (mass_corr ? parent.ntgt : 0);
   if (!n) return;
-  hipLaunchKernelGGL(k_load<T>, dim3(nblk(n)), dim3(kTile), 0, s, L, parent, mass_corr);
+  const long long nr = (PPq && parent.aux) ? parent.nghost : 0;
+  hipLaunchKernelGGL(k_load<T>, dim3(nblk(n + nr)), dim3(kTile), 0, s, L, parent, mass_corr, pghost, homog,
+                     nr ? PPq : (const T*)nullptr);
 }
 
 template <typename T>
@@ -2954,7 +2997,7 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_ring<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                    \
   template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, T*);       \
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
-  template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);               \
+  template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, int, const T*); \
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, T*);        \
   template void launch_filldown_small<T>(const LevelDev<T>*, int, int, int, hipStream_t);                  \
   template void launch_reset_force<T>(const LevelDev<T>*, int, int, hipStream_t);                         \

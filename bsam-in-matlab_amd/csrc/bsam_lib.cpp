@@ -97,6 +97,12 @@ struct HLevel {
   std::vector<uint8_t> vis;
   std::vector<int8_t> klass;      // ghost classes per patch (export)
   std::vector<int> klass_off;
+  // parent physical ghosts the FAS load reads: per QC window cell and per
+  // ring item, the parent ghost record at the source (-1: an interior cell);
+  // pgh_ok when every such record is physical (the load can evaluate them
+  // from the parent's interior instead of reading a filled frame)
+  std::vector<int> wgr, ring_rec;
+  int pgh_ok = 0;
 };
 
 template <typename T>
@@ -992,6 +998,38 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
   return !out.tiles.empty();
 }
 
+// The parent ghost records behind the FAS load's parent reads of level F
+// (window cells and ring items whose source is a ghost slot of C), and
+// whether all of them are physical: side records of a Dirichlet or Neumann
+// face, or physical corners between two such sides.
+void parent_ghost_sources(HLevel& F, const HLevel& C) {
+  std::vector<int> rec_at((size_t)std::max(1, C.nq), -1);
+  for (size_t r = 0; r < C.ghost.size(); ++r)
+    if (C.ghost[r].dst >= 0 && C.ghost[r].dst < C.nq) rec_at[(size_t)C.ghost[r].dst] = (int)r;
+  auto physical = [&](int r) {
+    const int kind = C.ghost[(size_t)r].kind;
+    return kind == G_DIR || kind == G_NEU;
+  };
+  bool ok = true;
+  auto note = [&](int q) -> int {
+    if (q < 0 || q >= C.nq) return -1;
+    const int r = rec_at[(size_t)q];
+    if (r < 0) return -1;
+    const GhostRec& g = C.ghost[(size_t)r];
+    if (g.kind == G_PCORNER) {
+      if (!physical(g.s[0]) || !physical(g.s[1])) ok = false;
+    } else if (!physical(r)) {
+      ok = false;
+    }
+    return r;
+  };
+  F.wgr.assign(F.wmap.size(), -1);
+  for (size_t w = 0; w < F.wmap.size(); ++w) F.wgr[w] = note(F.wmap[w]);
+  F.ring_rec.assign(F.ring_all.size(), -1);
+  for (size_t t = 0; t < F.ring_all.size(); ++t) F.ring_rec[t] = note(F.ring_all[t].src);
+  F.pgh_ok = ok ? 1 : 0;
+}
+
 // Every index the region kernel dereferences, against its buffer: each cell
 // entry inside its tile's patches and canvas, each ghost record's
 // destination and slots, each footprint item and its block's slots; every
@@ -1550,6 +1588,7 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
     D.ring_cf = d64->ring_cf; D.foot = d64->foot; D.corr = d64->corr; D.tgt_cell = d64->tgt_cell;
     D.tgt_ptr = d64->tgt_ptr; D.tgt_ent = d64->tgt_ent; D.tgt_of_cell = d64->tgt_of_cell;
     D.wmap = d64->wmap; D.vis = d64->vis; D.stiles = d64->stiles;
+    D.wgr = d64->wgr; D.ring_rec = d64->ring_rec;
   } else {
     if ((rc = upload(h, &D.P, L.pi))) return rc;
     if ((rc = upload(h, &D.tiles, L.tiles))) return rc;
@@ -1566,7 +1605,10 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
     if ((rc = upload(h, &D.tgt_of_cell, L.tgt_of_cell))) return rc;
     if ((rc = upload(h, &D.wmap, L.wmap))) return rc;
     if ((rc = upload(h, &D.vis, L.vis))) return rc;
+    if ((rc = upload(h, &D.wgr, L.wgr))) return rc;
+    if ((rc = upload(h, &D.ring_rec, L.ring_rec))) return rc;
   }
+  D.pgh_ok = L.pgh_ok;
   if ((rc = upload_cast(h, &D.Q, L.Q))) return rc;
   if ((rc = upload_cast(h, &D.FT, L.FT))) return rc;
   if ((rc = upload_cast(h, &D.F, L.FT))) return rc;
@@ -1803,6 +1845,7 @@ struct Seq {
   // into a level the child covers entirely (below the root), or the
   // correction (a one-patch level, every cell).  qs_ready names that level.
   int qs_ready = 0;
+  int unfilled = 0;  // the level whose fill after the restriction was left out
   bool odd_stream(int k, int nsweeps) const {  // relax(k, nsweeps) takes the streaming path and copies
     return (nsweeps & 1) && L[k - 1].stream && h->opts.smoother != 3 &&
            !(h->opts.smoother == 1 && tiled_fits<T>(L[k - 1]));
@@ -1848,7 +1891,11 @@ struct Seq {
     const bool fuse = !sw && region(k, h->opts.npre);
     T* dup = nullptr;
     if (k <= h->root && k - 1 > kc && odd_stream(k - 1, h->opts.npre)) dup = lev(k - 1).Qs;
-    relax(k, h->opts.npre, 0, k < h->nlev, fuse, sw ? 2 : 0, dup);
+    // a level whose fill after the restriction was left out starts its
+    // pre-smoothing from assembled ghosts (what that fill would have written)
+    const bool first_assembled = unfilled == k;
+    unfilled = 0;
+    relax(k, h->opts.npre, first_assembled ? 2 : 0, k < h->nlev, fuse, sw ? 2 : 0, dup);
     if (dup) qs_ready = k - 1;
     if (sw) {
     } else if (fuse) {
@@ -1858,8 +1905,17 @@ struct Seq {
     } else {
       restrict_(k);
     }
-    fill(k - 1);
-    launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s);
+    // the parent's fill after the restriction is left out when none of its
+    // readers needs it: the FAS load evaluates the parent's physical ghosts
+    // from the interior (pgh_ok: every parent ghost it reads is physical), and
+    // the parent's pre-smoothing is a buffer-swapping region launch that
+    // assembles its first pass's ghosts, or the parent is the coarsest level,
+    // whose solve reads no ghost (it is filled after the solve)
+    const bool pskip = lev(k).pgh_ok && (k - 1 == 1 ? kc <= 1 : (k - 1 > kc && swaps(k - 1)));
+    if (!pskip) fill(k - 1);
+    launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s, pskip ? 1 : 0, homog,
+                   pskip && k - 1 > 1 ? (const T*)lev(k - 2).Q : nullptr);
+    unfilled = pskip ? k - 1 : 0;
     vcycle(k - 1, kc);
     const bool prep = h->opts.npost > 0 && lev(k).arith && lev(k).aux && h->opts.smoother != 1;
     if (odd_stream(k, h->opts.npost)) {
@@ -2068,6 +2124,7 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
     int rc = build_level_tables(h->H[k], h->H[k - 1], qif);
     if (rc) return rc;
   }
+  for (int k = 1; k < h->nlev; ++k) parent_ghost_sources(h->H[k], h->H[k - 1]);
   for (auto& L : h->H)  // patch records are final now: copy them into the tiles
     for (TileRec& t : L.tiles) t.pi = L.pi[t.p];
   // singular iff all-Neumann level 1 and C == 0 everywhere (ebCoarsest.m:58-66)
