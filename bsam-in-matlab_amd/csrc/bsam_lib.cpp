@@ -103,6 +103,9 @@ struct HLevel {
   // from the parent's interior instead of reading a filled frame)
   std::vector<int> wgr, ring_rec;
   int pgh_ok = 0;
+  // the region tiles' ghost records as uploaded: their boundary values are
+  // copies of the ghost records' (refreshed with them by bsam_update_coeffs)
+  std::vector<RegionGhost> rghost;
 };
 
 template <typename T>
@@ -1659,6 +1662,7 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
         if ((rc = upload(h, &D.rpatch, rb.patches))) return rc;
         if ((rc = upload(h, &D.rcell, rb.cells))) return rc;
         if ((rc = upload(h, &D.rghost, rb.ghosts))) return rc;
+        L.rghost = rb.ghosts;
         if ((rc = upload(h, &D.rrs, rb.rs))) return rc;
         D.nregion = (int)rb.tiles.size();
         D.rdepth = rdepth;
@@ -3001,6 +3005,25 @@ int bsam_time_smoother(bsam_hier* h, int k, int precision, int nlaunch, double* 
 // library derived from those values is refreshed here: the physical ghost
 // data of the ghost records, the interface diffusivities of the reflux
 // records (ebGatherParentCoeffs.m:47-61) and the coarsest factorisation.
+// The boundary values a region tile's ghost records carry (the record's own,
+// or a physical corner's two side records') from the level's current ghost
+// records, found through the Q index each one writes.
+static void refresh_region_ghosts(HLevel& L) {
+  if (L.rghost.empty()) return;
+  std::vector<int> rec_at((size_t)std::max(1, L.nq), -1);
+  for (size_t r = 0; r < L.ghost.size(); ++r)
+    if (L.ghost[r].dst >= 0 && L.ghost[r].dst < L.nq) rec_at[(size_t)L.ghost[r].dst] = (int)r;
+  for (RegionGhost& g : L.rghost) {
+    const GhostRec& r = L.ghost[(size_t)rec_at[(size_t)g.dst]];
+    if ((g.kinds & 15) == G_PCORNER) {
+      g.ca = L.ghost[(size_t)r.s[0]].c;
+      g.cb = L.ghost[(size_t)r.s[1]].c;
+    } else {
+      g.c = r.c;
+    }
+  }
+}
+
 int bsam_update_coeffs(bsam_hier* h, const bsam_hier_desc* d) {
   if (!h) return fail(BSAM_E_ARG, "bsamgpu:arg: null hierarchy");
   if (!d || d->nlev != h->nlev || d->root_idx != h->root || !d->n || !d->np || !d->boxes || !d->bc_code ||
@@ -3062,6 +3085,7 @@ int bsam_update_coeffs(bsam_hier* h, const bsam_hier_desc* d) {
     const HLevel& C = h->H[k - 1];
     for (CorrRec& e : h->H[k].corr) e.cf = e.face < 2 ? C.DeW[e.cfidx] : C.DnS[e.cfidx];
   }
+  for (HLevel& L : h->H) refresh_region_ghosts(L);
   bool allneu = true;
   for (int sd = 0; sd < 4; ++sd) allneu = allneu && h->H[0].pi[0].bc[sd] != 1;
   double cmax = 0;
@@ -3105,6 +3129,8 @@ int bsam_update_coeffs(bsam_hier* h, const bsam_hier_desc* d) {
       HIPCHK(hipMemcpy(D.ghost, L.ghost.data(), L.ghost.size() * sizeof(GhostRec), hipMemcpyHostToDevice));
     if (!L.corr.empty())
       HIPCHK(hipMemcpy(D.corr, L.corr.data(), L.corr.size() * sizeof(CorrRec), hipMemcpyHostToDevice));
+    if (!L.rghost.empty() && D.rghost)  // shared with the FP32 level
+      HIPCHK(hipMemcpy(D.rghost, L.rghost.data(), L.rghost.size() * sizeof(RegionGhost), hipMemcpyHostToDevice));
     if (h->have32) {  // the FP32 levels share the tables; their value arrays are cast copies
       LevelDev<float>& E = h->d32[k];
       auto put32 = [&](float* dst, const std::vector<double>& src) -> int {

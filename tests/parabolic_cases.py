@@ -4,6 +4,9 @@
   heat_config   the heat problem at BaseCells 64 / MaxLevels 4, per-step RelTol 1e-12
   mms_bump      ParabolicBSAM/tests/runParabolicTests.m:21-33 (P1: traveling
                 Gaussian, time-dependent Dirichlet data and source, regrid every 2 steps)
+  mms_bump_64   the same at the next resolution of P1b (runParabolicTests.m:68-84,
+                BaseCells 64, Dt 0.025): its 64^2 root, with Dirichlet data that
+                change every step, runs region launches that write its ghosts
   reaction_t    runParabolicRegressions.m:6-8 (C = 1 + t, f = t e^-t, Neumann)
   diffusion_t   runParabolicRegressions.m:22-25 (D = 1 + t, time-dependent Dirichlet data)
   fixed_patch   runParabolicRegressions.m:43-50 (D = 1 + 0.2x, a fixed fine patch)
@@ -113,6 +116,13 @@ def fixed_patch_cn(**kw):
     return fixed_patch(**dict(dict(Scheme="cn"), **kw))
 
 
-CASES = {"heat_readme": heat_readme, "mms_bump": mms_bump, "reaction_t": reaction_t,
+def mms_bump_64(**kw):
+    thr = 5e-2 / 4
+    o = dict(Dt=0.025, BaseCells=64, TagThreshold=lambda L: thr / 4 ** (L - 1))
+    o.update(kw)
+    return mms_bump(**o)
+
+
+CASES = {"heat_readme": heat_readme, "mms_bump": mms_bump, "mms_bump_64": mms_bump_64, "reaction_t": reaction_t,
          "diffusion_t": diffusion_t, "fixed_patch": fixed_patch, "reaction_t_cn": reaction_t_cn,
          "diffusion_t_cn": diffusion_t_cn, "fixed_patch_cn": fixed_patch_cn}
