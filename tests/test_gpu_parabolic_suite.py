@@ -1,140 +1,42 @@
-"""The reference's parabolic verification suite through the device library
-(ParabolicBSAM/tests/runParabolicTests.m:20-160): every time step's solve,
-the BDF2 / BE / CN right sides and the conservative regrids run on the GPU
-(bsam_parabolic_step, bsam_regrid_level) under the host layer's ebp_solve,
-against the numbers published in verification/suite-output.txt:184-213.
-
-  P1a  Dt-Cauchy differences of BE, BDF2 and CN on a fixed uniform 128^2 mesh
-       (3 significant digits, as printed)
-  P1b  joint (h, Dt) refinement with the moving adaptive mesh: error to 5
-       digits, mean V-cycles per step and regrid count
-  P2   variable-D heat equation over 200 steps and 39 regrids with
-       derefinement: the step log every 50 steps (V, rel, levels, cells, mass),
-       regrids, mean V, final level count and the mass drift
-  P3   rotating source: stored cells, regrids, mean V and the finest level
-       covering the source at t = 1
-  P4   the P2 mean V to 2 decimals and a large-Dt BE run's max |u|
-"""
-import json
-import os
-
-import numpy as np
+"""The reference's parabolic verification suite through the device library:
+every time step's solve, the BDF2 / BE / CN right sides and the conservative
+regrids run on the GPU (bsam_parabolic_step, bsam_regrid_level) under the
+host layer's ebp_solve, against the numbers published in
+verification/suite-output.txt:184-213 (checks in parabolic_suite.py; the
+oracle runs the same checks in test_parabolic_suite.py)."""
 import pytest
 
+import parabolic_suite as PS
 from bsamgpu import parabolic as par
-from parabolic_cases import mms_bump
 
 pytestmark = pytest.mark.gpu
 
-GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "published.json")))["parabolic_suite"]
 
-
-def _close(sol):
+def device_solve(pp, opts):
+    cb = opts.Callback
+    if cb is not None:
+        opts.Callback = lambda S, t1, n: cb(S.H, t1, n)
+    sol = par.ebp_solve(pp, opts)
     sol["device"].close()
     return sol
 
 
 def test_p1a_temporal_cauchy_differences():
-    pp, _ = mms_bump()
-    for scheme, want in GOLD["P1a"]["diffs"].items():
-        U = []
-        for dt in (0.04 / 2 ** j for j in range(4)):
-            print(f"P1a {scheme} Dt={dt}", flush=True)
-            sol = _close(par.ebp_solve(pp, par.parabolic_options(
-                Scheme=scheme, Dt=dt, TFinal=0.4, RegridEvery=0, StepVerbose=0, BaseCells=128, MaxLevels=1,
-                RelTol=1e-10)))
-            H = sol["H"]
-            U.append(H.L(H.nlev).Q[0][1:-1, 1:-1].copy())
-        diffs = [float(np.max(np.abs(U[j] - U[j + 1]))) for j in range(3)]
-        assert [float(f"{d:.3g}") for d in diffs] == want, (scheme, diffs, want)
+    PS.p1a(device_solve)
 
 
 @pytest.mark.parametrize("j", [0, 1, 2])
 def test_p1b_joint_refinement_with_regrids(j):
-    want = GOLD["P1b"]["rows"][j]
-    thr = 5e-2 / 4 ** j
-    pp, o = mms_bump(Dt=0.05 / 2 ** j, BaseCells=32 * 2 ** j, TagThreshold=lambda L: thr / 4 ** (L - 1))
-    print(f"P1b j={j}", flush=True)
-    sol = _close(par.ebp_solve(pp, o))
-    assert f"{sol['err']['linf']:.4e}" == want["err_inf"]
-    assert f"{np.mean(sol['cycles']):.1f}" == want["meanV"]
-    assert sol["nregrids"] == want["regrids"]
-
-
-def _p2_problem():
-    D2 = lambda x, y: 1 + 0.5 * np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)  # noqa: E731
-
-    def u02(x, y):
-        return 0.5 + np.exp(-((x - 0.35) ** 2 + (y - 0.4) ** 2) / (2 * 0.03 ** 2)) \
-            + 0.7 * np.exp(-((x - 0.7) ** 2 + (y - 0.65) ** 2) / (2 * 0.04 ** 2))
-
-    return par.ParabolicProblem([0, 1, 0, 1], D2, 0, 0, ("neumann", 0), u02)
+    PS.p1b(device_solve, j)
 
 
 def test_p2_heat_mass_across_regrids_and_p4():
-    log = {}
-
-    def cb(S, t1, n):
-        if n % 50 == 0:
-            H = S.H
-            nc = sum(int(np.prod(H.L(k).dims(p))) for k in range(H.rootIdx, H.nlev + 1) for p in range(H.L(k).np))
-            log[n] = (H.nlev - H.rootIdx + 1, nc)
-
-    print("P2", flush=True)
-    sol = _close(par.ebp_solve(_p2_problem(), par.parabolic_options(
-        Scheme="bdf2", Dt=2e-3, TFinal=0.4, RegridEvery=5, StepVerbose=0, Callback=cb, BaseCells=64, MaxLevels=3,
-        Indicator="ulap", TagThreshold=lambda L: 1e-3 / 4 ** (L - 1), TagBuffer=2, RelTol=1e-9)))
-    want = GOLD["P2"]
-    for row in want["log"]:
-        n = row["step"]
-        got = {"V": sol["cycles"][n - 1], "rel": f"{sol['relres'][n - 1]:.1e}", "levels": log[n][0],
-               "cells": log[n][1], "mass": f"{sol['mass'][n]:.6e}"}
-        assert got == {k: row[k] for k in got}, (n, got, row)
-    assert len(sol["cycles"]) == want["steps"]
-    assert sol["nregrids"] == want["regrids"]
-    assert f"{np.mean(sol['cycles']):.1f}" == want["meanV"]
-    assert f"{np.mean(sol['cycles']):.2f}" == GOLD["P4"]["p2_meanV"]
-    assert len(sol["levels"]) == want["final_levels"]
-    drift = max(abs(m - sol["mass"][0]) for m in sol["mass"])
-    print(f"P2 mass drift {drift:.3e} (published {want['drift']})", flush=True)
-    assert f"{sol['mass'][0]:.4f}" == want["mass0"]
-    # the drift is the accumulated per-step defect at RelTol 1e-9: the same
-    # order as the published one and below the suite's bound
-    assert drift <= 1e-9 * abs(sol["mass"][0])
-    assert 0.5 * float(want["drift"]) <= drift <= 2 * float(want["drift"])
+    PS.p2_and_p4_mean(device_solve)
 
 
 def test_p3_rotating_source_tracking():
-    om, Rc, ss = 2 * np.pi, 0.22, 0.03
-    cx = lambda t: 0.5 + Rc * np.cos(om * t)  # noqa: E731
-    cy = lambda t: 0.5 + Rc * np.sin(om * t)  # noqa: E731
-
-    def src(x, y, t):
-        return 25 * np.exp(-((x - cx(t)) ** 2 + (y - cy(t)) ** 2) / (2 * ss ** 2))
-
-    pp = par.ParabolicProblem([0, 1, 0, 1], 1, 0, src, ("dirichlet", 0), lambda x, y: 0 * x)
-    print("P3", flush=True)
-    sol = _close(par.ebp_solve(pp, par.parabolic_options(
-        Scheme="bdf2", Dt=5e-3, TFinal=1.0, RegridEvery=4, StepVerbose=0, BaseCells=64, MaxLevels=3,
-        Indicator="ulap", TagThreshold=lambda L: 2e-3 / 4 ** (L - 1), TagBuffer=3, RelTol=1e-8)))
-    H = sol["H"]
-    L = H.L(H.nlev)
-    covered = False
-    for b in L.box:
-        x0, x1 = (b[0] - 1) * L.h[0], b[1] * L.h[0]
-        y0, y1 = (b[2] - 1) * L.h[1], b[3] * L.h[1]
-        covered |= bool(x0 <= cx(1.0) <= x1 and y0 <= cy(1.0) <= y1)
-    want = GOLD["P3"]
-    assert covered
-    assert sum(lv["cells"] for lv in sol["levels"]) == want["cells"]
-    assert sol["nregrids"] == want["regrids"]
-    assert f"{np.mean(sol['cycles']):.1f}" == want["meanV"]
+    PS.p3(device_solve)
 
 
 def test_p4_large_dt_backward_euler():
-    print("P4", flush=True)
-    sol = _close(par.ebp_solve(_p2_problem(), par.parabolic_options(
-        Scheme="be", Dt=0.1, TFinal=0.5, RegridEvery=0, StepVerbose=0, BaseCells=64, MaxLevels=1, RelTol=1e-9)))
-    H = sol["H"]
-    umax = max(float(np.max(np.abs(q))) for q in H.L(H.nlev).Q)
-    assert f"{umax:.3f}" == GOLD["P4"]["be_umax"]
+    PS.p4_large_dt(device_solve)
