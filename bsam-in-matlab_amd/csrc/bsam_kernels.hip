@@ -247,6 +247,24 @@ __device__ __forceinline__ T relax_combine(const LevelDev<T>& L, T qe, T qw, T q
   return omega == T(1) ? tmp : omega * tmp + (T(1) - omega) * self;
 }
 
+// relax_combine with the level's power-of-two spacing (P2: q / h^2 taken as
+// q * (1 / h^2), the same number) and omega == 1 (W1) fixed at compile time,
+// so the region passes carry no per-update branches for them
+template <typename T, bool P2, bool W1>
+__device__ __forceinline__ T relax_combine_s(const LevelDev<T>& L, T qe, T qw, T qn, T qs, T f, T de, T dw, T dn,
+                                             T ds, T id, T self, T omega) {
+  const T hx = (T)L.h[0], hy = (T)L.h[1];
+  const T hx2 = hx * hx, hy2 = hy * hy;
+  const T ihx2 = (T)L.ih2[0], ihy2 = (T)L.ih2[1];
+  T tmp = f + (P2 ? (qe * de) * ihx2 : (qe * de) / hx2);
+  tmp = tmp + (P2 ? (qw * dw) * ihx2 : (qw * dw) / hx2);
+  tmp = tmp + (P2 ? (qn * dn) * ihy2 : (qn * dn) / hy2);
+  tmp = tmp + (P2 ? (qs * ds) * ihy2 : (qs * ds) / hy2);
+  tmp = tmp * id;
+  if (W1) return tmp;
+  return omega * tmp + (T(1) - omega) * self;
+}
+
 // Neighbour values of cell (i, j) with its ghosts substituted: partial
 // physical ghosts (Dirichlet (8/3)g + (1/3)u_2, Neumann h g; g: 0 when
 // homogeneous) and, when assembled, coarse-fine ghosts from the cell's own
@@ -718,13 +736,13 @@ __device__ __forceinline__ bool region_face(int fl) { return (fl & 0x3ff) != 0; 
 // one cell's update from the canvas: the expressions of resolve_ghosts and
 // relax_combine, in the same operation order; fr* are the frame's ghost
 // values (pass 1 of a call that reads the frame)
-template <typename T>
+template <typename T, bool P2, bool W1>
 __device__ __forceinline__ T region_update(const LevelDev<T>& L, const T* cv, int cw, const RegionCell<T>& c,
                                            bool frame, T frx, T fry, T omega) {
   const int s = c.slot;
   const T self = cv[s], re = cv[s + 1], rw = cv[s - 1], rn = cv[s + cw], rs = cv[s - cw];
   if (!region_face(c.fl))  // off every face (the host lists these cells first, so whole waves take this path)
-    return relax_combine<T>(L, re, rw, rn, rs, c.f, c.de, c.dw, c.dn, c.ds, c.id, self, omega);
+    return relax_combine_s<T, P2, W1>(L, re, rw, rn, rs, c.f, c.de, c.dw, c.dn, c.ds, c.id, self, omega);
   T qe = re, qw = rw, qn = rn, qs = rs;
   const int xk = c.fl & 7, yk = (c.fl >> 5) & 7;
   if (xk) {
@@ -759,7 +777,7 @@ __device__ __forceinline__ T region_update(const LevelDev<T>& L, const T* cv, in
     }
     if (n) qn = v; else qs = v;
   }
-  return relax_combine<T>(L, qe, qw, qn, qs, c.f, c.de, c.dw, c.dn, c.ds, c.id, self, omega);
+  return relax_combine_s<T, P2, W1>(L, qe, qw, qn, qs, c.f, c.de, c.dw, c.dn, c.ds, c.id, self, omega);
 }
 
 // npass = 2 x sweeps (even, <= L.rdepth).  Workgroup h takes tile
@@ -823,7 +841,7 @@ __host__ __device__ inline size_t region_lds_bytes(int rslots, int rghmax, int r
 // complete new state.  Rq: when given, the restriction of the tile's coarse
 // cells into the parent (and Rdup) and QC's interior (ebTransfer.m:10-33).
 // ring: the QC ring refresh of the fill, from the parent Pq.
-template <typename T, int NT, int K, int KL>
+template <typename T, int NT, int K, int KL, bool P2, bool W1>
 __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __restrict__ Pq, T* __restrict__ Qout,
                                                      int npass, int frame, T omega, int homog, int per, int ghosts,
                                                      T* __restrict__ Rq, T* __restrict__ Rdup, int ring) {
@@ -925,12 +943,12 @@ __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __r
 #pragma unroll
     for (int k = 0; k < K; ++k)
       if (c1[k].slot >= 0 && region_depth(c1[k].fl) >= p + dshift)
-        cv[c1[k].slot] = region_update<T>(L, cv, cw, c1[k], frame && p == 1, fx1[k], fy1[k], omega);
+        cv[c1[k].slot] = region_update<T, P2, W1>(L, cv, cw, c1[k], frame && p == 1, fx1[k], fy1[k], omega);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; ++k)
       if (c2[k].slot >= 0 && region_depth(c2[k].fl) >= p + 1 + dshift)
-        cv[c2[k].slot] = region_update<T>(L, cv, cw, c2[k], false, T(0), T(0), omega);
+        cv[c2[k].slot] = region_update<T, P2, W1>(L, cv, cw, c2[k], false, T(0), T(0), omega);
     __syncthreads();
   }
 #pragma unroll
@@ -2584,16 +2602,28 @@ void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int ns
   const int rg = (ring && parent) ? 1 : 0;
   const int rring = rg ? (L.nring_cf + L.nregion - 1) / L.nregion : 0;
   const size_t lds = region_lds_bytes<T>(L.rslots, L.rghmax, std::min(rring, kRegionThreads), ghosts, rg);
+  // the kernel for the level's spacing and omega (compile-time forms of the
+  // update, the same numbers as the general one)
   static bool raised = false;  // the canvas plus staged records can pass 64 KB (gfx950: 160 KB per CU)
   if (!raised) {
-    (void)hipFuncSetAttribute((const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStageLdsLimit);
+    const void* forms[4] = {(const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, true>,
+                            (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, false>,
+                            (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, true>,
+                            (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, false>};
+    for (const void* f : forms)
+      (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStageLdsLimit);
     raised = true;
   }
-  hipLaunchKernelGGL((k_relax_region<T, kRegionThreads, kRegionK, kRegionKL>), dim3(per * kXcds), dim3(kRegionThreads),
-                     lds, s,
-                     L, parent ? (const T*)parent->Q : (const T*)nullptr, Qout ? Qout : L.Qs, 2 * nsweeps, frame,
-                     omega, homog, per, ghosts, Rq, Rdup, (ring && parent) ? 1 : 0);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(per * kXcds), dim3(kRegionThreads), lds, s, L,
+                       parent ? (const T*)parent->Q : (const T*)nullptr, Qout ? Qout : L.Qs, 2 * nsweeps, frame,
+                       omega, homog, per, ghosts, Rq, Rdup, (ring && parent) ? 1 : 0);
+  };
+  const bool w1 = omega == T(1);
+  if (L.pow2 && w1) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, true>);
+  else if (L.pow2) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, false>);
+  else if (w1) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, true>);
+  else go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, false>);
 }
 
 template <typename T>

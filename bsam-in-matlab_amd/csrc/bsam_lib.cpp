@@ -535,6 +535,27 @@ struct RegionBuild {
   int max_slots = 0, max_gh = 0, max_rs = 0;
 };
 
+// The form of the face ghosts of cell (p, i, j) as a sort key: 0 off every
+// face; else 1 + (x-face form, east) * 16 + (y-face form, north), a form
+// being the side's boundary type or its record kind
+static unsigned char face_form(const HLevel& L, int p, int i, int j) {
+  const PatchInfo& P = L.pi[p];
+  const int mx = P.mx, my = P.my;
+  auto form = [&](int side, int t) -> int {  // 1 DIR, 2 NEU, 3 COPY, 4 QIF, 5 QIF_END, 6 LIF(_CORNER)
+    if (P.bc[side]) return P.bc[side] == 1 ? 1 : 2;
+    const int off = side == 0 ? 0 : side == 1 ? my : side == 2 ? 2 * my : 2 * my + mx;
+    const int kind = L.ghost[(size_t)P.gbase + off + t].kind;
+    return kind == G_COPY ? 3 : kind == G_QIF ? 4 : kind == G_QIF_END ? 5 : 6;
+  };
+  int x = 0, y = 0;
+  if (i == mx - 1) x = 2 * form(1, j) + 1;
+  else if (i == 0) x = 2 * form(0, j);
+  if (j == my - 1) y = 2 * form(3, i) + 1;
+  else if (j == 0) y = 2 * form(2, i);
+  if (!x && !y) return 0;
+  return (unsigned char)(1 + x * 15 + y);
+}
+
 bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
   out = RegionBuild();
   if (L.np == 0 || rdepth < 2 || rdepth > kRegionMaxDepth || rdepth % 2) return false;
@@ -610,6 +631,8 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
   const long long nglob = (long long)n0 * L.n[1];
   std::vector<int> cell_ix((size_t)nglob, -1);  // global id -> level cell index (patch order)
   std::vector<long long> gid;                    // level cell index -> global id
+  // face: 0 off every face, else the form of the cell's face ghosts (below),
+  // so that a wave of face cells takes one branch of the update
   std::vector<unsigned char> colour, face;
   std::vector<int> cpatch, ci_, cj_;  // owner patch and local position of each cell
   gid.reserve((size_t)L.ncell);
@@ -620,7 +643,7 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
         cell_ix[(size_t)g] = (int)gid.size();
         gid.push_back(g);
         colour.push_back((unsigned char)((i + j + L.pi[p].par0) & 1));
-        face.push_back((unsigned char)(i == 0 || j == 0 || i == L.pi[p].mx - 1 || j == L.pi[p].my - 1));
+        face.push_back(face_form(L, p, i, j));
         cpatch.push_back(p);
         ci_.push_back(i);
         cj_.push_back(j);
@@ -766,12 +789,13 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
       const int gi = (int)(gid[c] % n0), gj = (int)(gid[c] / n0);
       return (unsigned short)((gi - D.cx0) + (gj - D.cy0) * D.cw);
     };
-    // level cell indices run in patch order, row by row: sorting them keeps
-    // each list in (patch, j, i) order, the cells off every face of their
-    // patch first (their updates take the kernel's short path, wave by wave)
+    // level cell indices run in patch order, row by row: each list is sorted
+    // by the cells' face form, then in (patch, j, i) order -- the cells off
+    // every face first (the kernel's short path), then the face cells grouped
+    // by the form of their ghosts, so that the waves of a pass take one
+    // branch each instead of the union of their lanes' branches
     std::vector<int> sorted(order);
-    std::sort(sorted.begin(), sorted.end());
-    std::stable_partition(sorted.begin(), sorted.end(), [&](int c) { return !face[c]; });
+    std::sort(sorted.begin(), sorted.end(), [&](int a, int b) { return face[a] != face[b] ? face[a] < face[b] : a < b; });
     D.pl.clear(); D.load.clear(); D.red.clear(); D.black.clear();
     D.rcell = order;
     D.rdep.resize(order.size());
