@@ -150,6 +150,7 @@ struct alignas(16) RegionTile {
   int np, nload, nred, nblack;
   int gofs, ngh;          // the tile's ghost records in the level's rghost list
   int rofs, nrs;          // the coarse cells it restricts in the level's rrs list
+  int wofs, nwin;         // its parent-window cells in the level's rwin list (fused correction)
 };
 // A ghost record the tile evaluates after the last pass (ebFillGhosts from
 // its final values), with everything but the parent parts resolved on the
@@ -178,7 +179,22 @@ struct alignas(16) RegionRestrict {
 struct RegionPatch {
   PatchInfo pi;  // the patch record
   int ox, oy;    // canvas position of the patch's cell (0, 0)
+  // the rectangle of the patch's QC window [ex0, ex0 + ew) x [ey0, ...) that
+  // the tile stages (parent value and QC) at window slots from eoff, row by row
+  int eoff, ex0, ey0, ew;
 };
+// A parent-window cell a tile stages for the correction folded into its
+// post-smoothing launch: the QC index and the parent Q index it maps to
+struct RegionWin {
+  int w, wm;
+};
+// The window slots of the parent sources of a region ghost record's
+// coarse-fine parts: [0..2] the record's own (or a physical corner's side A),
+// [3..5] a physical corner's side B; kRegionNoSlot where unused
+struct alignas(16) RegionGhostW {
+  unsigned short s[8];
+};
+constexpr int kRegionMaxWin = 2048;  // window cells per tile (two per lane)
 __host__ __device__ inline unsigned region_code(int ps, int i, int j, int d) {
   return (unsigned)ps | ((unsigned)i << 4) | ((unsigned)j << 16) | ((unsigned)d << 28);
 }
@@ -210,6 +226,10 @@ struct LevelDev {
   RegionPatch* rpatch;    // kRegionMaxPatches per tile
   unsigned* rcell;        // kRegionStride per tile
   RegionGhost* rghost;    // per tile (RegionTile::gofs, ngh)
+  RegionGhostW* rgw;      // parallel to rghost: window slots of their parent parts (rcorr)
+  RegionWin* rwin;        // per tile (RegionTile::wofs, nwin)
+  int rcorr;              // the post-smoothing launch can apply the coarse-grid correction itself
+  int rwinmax;            // the most window cells of one tile
   RegionRestrict* rrs;    // per tile (RegionTile::rofs, nrs)
   int rghmax, rrsmax;     // the most ghosts / restricted cells of one tile
   PatchInfo* P;

@@ -48,7 +48,7 @@ void launch_sweep_stream(const LevelDev<T>& L, const PatchInfo& P, const T* Qin,
 template <typename T>
 void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, int frame, T omega, int homog,
                          hipStream_t s, T* Qout = nullptr, int ghosts = 0, T* Rq = nullptr, T* Rdup = nullptr,
-                         int ring = 0);
+                         int ring = 0, int corr = 0);
 template <typename T>
 void launch_fill_copy(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, const T* Qsrc, hipStream_t s);
 // the fill after a region relax (parent parts from aux, no QC ring refresh)
@@ -80,6 +80,8 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
 void launch_tag_finish(const uint8_t* raw, const uint8_t* cover, int n1, int n2, int buffer, uint8_t* out,
                        hipStream_t s);
 template <typename T> void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
+template <typename T>
+void launch_ring_cf(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
 // restrict k -> k-1 + fill k-1 for k = kfrom..2 in one workgroup
 template <typename T>
 void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int homog, hipStream_t s);

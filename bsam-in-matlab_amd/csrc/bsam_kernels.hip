@@ -797,6 +797,8 @@ struct GhostOp {
   unsigned s01, s23, s4;
 };
 
+static_assert(sizeof(GhostOp<double>) <= 80, "the host's LDS bound of a region launch assumes 80-byte ghost records");
+
 // A ghost record's value from its kind, boundary value c, fine sources v0,
 // v1 and parent part cf: the expressions of ghost_value, in the same order
 template <typename T>
@@ -824,14 +826,66 @@ __device__ __forceinline__ T region_ghost(int kind, T c, T cf, T v0, T v1, int h
   }
 }
 
+// The coarse-grid correction of cell (i, j) of a region patch from the staged
+// parent window (Wc: parent values, Cc: QC values): ebTransfer.m:57-71 with
+// prolongBilin's (9, 3, 3, 1) / 16 weights, in correct_item's operation order
+template <typename T>
+__device__ __forceinline__ T region_correction(const RegionPatch& R, int i, int j, const T* Wc, const T* Cc) {
+  const int wx = (i >> 1) + 1, wy = (j >> 1) + 1;
+  const int s = R.eoff + (wx - R.ex0) + (wy - R.ey0) * R.ew;
+  const int sx = s + ((i & 1) ? 1 : -1), sy = s + ((j & 1) ? R.ew : -R.ew), sd = sy + ((i & 1) ? 1 : -1);
+  const T ec = Wc[s] - Cc[s], ex = Wc[sx] - Cc[sx], ey = Wc[sy] - Cc[sy], ed = Wc[sd] - Cc[sd];
+  return (((T(9) * ec + T(3) * ex) + T(3) * ey) + ed) / T(16);
+}
+
+// The parent part (cf_part) of a coarse-fine record from parent values at
+// window slots s0..s2 of the staged window
+template <typename T>
+__device__ __forceinline__ T region_cf_parts(int kind, T w0, T w1, T w2) {
+  switch (kind) {
+    case G_QIF:
+      return T(A3_) * w0;
+    case G_QIF_END:
+      return T(A3_) * ((T(B1_) * w0 + T(B2_) * w1) + T(B3_) * w2);
+    case G_LIF:
+    case G_LIF_CORNER:
+      return T(C23_) * w0;
+    default:
+      return T(0);
+  }
+}
+
+// The parent part of the coarse-fine record on side `side` (W E S N) at
+// position t of a region patch, from the staged window: cf_rec's stencils
+// (the host checked them against every record's parent indices)
+template <typename T>
+__device__ __forceinline__ T region_cf_window(int kind, int side, int t, const RegionPatch& R, const T* Wc) {
+  const int mx = R.pi.mx, my = R.pi.my, cmx = mx >> 1, cmy = my >> 1;
+  const int len = side < 2 ? my : mx, clen = side < 2 ? cmy : cmx, u = t + 1;
+  const int fixed = side == 0 ? 0 : side == 1 ? cmx + 1 : side == 2 ? 0 : cmy + 1;
+  auto at = [&](int along) {
+    const int wx = side < 2 ? fixed : along, wy = side < 2 ? along : fixed;
+    return Wc[R.eoff + (wx - R.ex0) + (wy - R.ey0) * R.ew];
+  };
+  if (kind == G_QIF_END) {
+    const int a0 = u == len ? clen + 1 : 0, st = u == len ? -1 : 1;
+    return region_cf_parts<T>(kind, at(a0), at(a0 + st), at(a0 + 2 * st));
+  }
+  const T w = at((u & 1) == 0 ? (u >> 1) : ((u + 1) >> 1));
+  return region_cf_parts<T>(kind, w, w, w);
+}
+
 // dynamic LDS of a region launch: the canvas, then (ghosts) the tile's ghost
-// records, (ring) its staged slice of the QC ring refresh
+// records, (ring) its staged slice of the QC ring refresh, (win) the parent
+// window of the folded correction: parent values, then QC values
 __host__ __device__ inline size_t region_align16(size_t n) { return (n + 15) & ~(size_t)15; }
 template <typename T>
-__host__ __device__ inline size_t region_lds_bytes(int rslots, int rghmax, int rring, int ghosts, int ring) {
+__host__ __device__ inline size_t region_lds_bytes(int rslots, int rghmax, int rring, int ghosts, int ring,
+                                                   int win = 0) {
   size_t b = region_align16((size_t)rslots * sizeof(T));
   if (ghosts) b += region_align16((size_t)rghmax * sizeof(GhostOp<T>));
   if (ring) b += region_align16((size_t)rring * sizeof(int)) + region_align16((size_t)rring * sizeof(T));
+  b += 2 * region_align16((size_t)win * sizeof(T));
   return b;
 }
 
@@ -840,11 +894,15 @@ __host__ __device__ inline size_t region_lds_bytes(int rslots, int rghmax, int r
 // after the call would write; parent parts from aux), so Qout is the level's
 // complete new state.  Rq: when given, the restriction of the tile's coarse
 // cells into the parent (and Rdup) and QC's interior (ebTransfer.m:10-33).
-// ring: the QC ring refresh of the fill, from the parent Pq.
+// ring: the QC ring refresh of the fill, from the parent Pq.  corr: the
+// coarse-grid correction of the V-cycle's up leg (ebTransfer.m:57-71) is
+// applied to every loaded cell first, and the parent parts of coarse-fine
+// ghosts come from the parent Pq, staged per patch window (L.rwin), in place
+// of the parent parts a preparing correction launch would have stored.
 template <typename T, int NT, int K, int KL, bool P2, bool W1>
 __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __restrict__ Pq, T* __restrict__ Qout,
                                                      int npass, int frame, T omega, int homog, int per, int ghosts,
-                                                     T* __restrict__ Rq, T* __restrict__ Rdup, int ring) {
+                                                     T* __restrict__ Rq, T* __restrict__ Rdup, int ring, int corr) {
   extern __shared__ __align__(16) unsigned char region_lds[];
   T* cv = reinterpret_cast<T*>(region_lds);
   __shared__ RegionPatch sp[kRegionMaxPatches];
@@ -872,13 +930,23 @@ __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __r
   const int rstage = rring < NT ? rring : NT;  // staged ring items (one per lane)
   int* rdst = reinterpret_cast<int*>(tail);
   T* rval = reinterpret_cast<T*>(tail + region_align16((size_t)rstage * sizeof(int)));
+  if (ring) tail += region_align16((size_t)rstage * sizeof(int)) + region_align16((size_t)rstage * sizeof(T));
+  T* Wc = reinterpret_cast<T*>(tail);  // the folded correction's window (corr)
+  T* Cc = reinterpret_cast<T*>(tail + region_align16((size_t)L.rwinmax * sizeof(T)));
+  const int nwin = corr ? tr.nwin : 0;
+  RegionWin wv[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (tid + k * NT < nwin) wv[k] = L.rwin[tr.wofs + tid + k * NT];
   const bool my_gh = ghosts && tid < tr.ngh, my_rs = Rq && tid < tr.nrs;
   const int rx = t * rring + tid;
   const bool my_ring = ring && tid < rstage && rx < L.nring_cf;
   RegionGhost ge;
   RingRec rr;
   int rs_qc = 0, rs_pq = 0, rs_slot = 0;  // this lane's coarse cell (registers until the epilogue)
+  RegionGhostW gw;
   if (my_gh) ge = L.rghost[tr.gofs + tid];
+  if (my_gh && corr) gw = L.rgw[tr.gofs + tid];
   if (my_rs) {
     const RegionRestrict& re = L.rrs[tr.rofs + tid];
     rs_qc = re.qc;
@@ -889,7 +957,14 @@ __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __r
   for (int x = tid; x < cw * tr.ch; x += NT) cv[x] = T(0);
   __syncthreads();
   T gcf = T(0), gcfa = T(0), gcfb = T(0), rv = T(0);
-  if (my_gh) {
+  T wval[2] = {T(0), T(0)}, cval[2] = {T(0), T(0)};
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+    if (tid + k * NT < nwin) {
+      wval[k] = Pq[wv[k].wm];
+      cval[k] = L.QC[wv[k].w];
+    }
+  if (my_gh && !corr) {
     if (ge.rec >= 0) gcf = L.aux[ge.rec].cf;
     if (ge.reca >= 0) gcfa = L.aux[ge.reca].cf;
     if (ge.recb >= 0) gcfb = L.aux[ge.recb].cf;
@@ -911,6 +986,44 @@ __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __r
   for (int k = 0; k < K; ++k) {
     region_load<T>(L, sp, cw, e1[k], homog, frame != 0, c1[k], q1[k], qc1[k], fx1[k], fy1[k], kx1[k], ky1[k]);
     region_load<T>(L, sp, cw, e2[k], homog, false, c2[k], q2[k], qc2[k], fdx, fdy, kx2[k], ky2[k]);
+  }
+  if (corr) {  // the staged window, then the corrections and parent parts from it
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+      if (tid + k * NT < nwin) {
+        Wc[tid + k * NT] = wval[k];
+        Cc[tid + k * NT] = cval[k];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < KL; ++k)
+      if (e0[k] != kRegionNone) {
+        const unsigned cd = e0[k];
+        q0[k] = q0[k] + region_correction<T>(sp[cd & 15], (cd >> 4) & 4095, (cd >> 16) & 4095, Wc, Cc);
+      }
+    auto fix = [&](unsigned cd, T& q, RegionCell<T>& c, int kx, int ky) {
+      if (cd == kRegionNone) return;
+      const RegionPatch& R = sp[cd & 15];
+      const int i = (cd >> 4) & 4095, j = (cd >> 16) & 4095;
+      q = q + region_correction<T>(R, i, j, Wc, Cc);
+      if (kx == G_QIF || kx == G_QIF_END || kx == G_LIF) c.bx = region_cf_window<T>(kx, i == 0 ? 0 : 1, j, R, Wc);
+      if (ky == G_QIF || ky == G_QIF_END || ky == G_LIF) c.by = region_cf_window<T>(ky, j == 0 ? 2 : 3, i, R, Wc);
+    };
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      fix(e1[k], q1[k], c1[k], kx1[k], ky1[k]);
+      fix(e2[k], q2[k], c2[k], kx2[k], ky2[k]);
+    }
+    if (my_gh) {
+      auto wat = [&](unsigned short sl) { return sl != kRegionNoSlot ? Wc[sl] : T(0); };
+      const int kind = ge.kinds & 15;
+      if (kind == G_PCORNER) {
+        gcfa = region_cf_parts<T>((ge.kinds >> 4) & 15, wat(gw.s[0]), wat(gw.s[1]), wat(gw.s[2]));
+        gcfb = region_cf_parts<T>((ge.kinds >> 8) & 15, wat(gw.s[3]), wat(gw.s[4]), wat(gw.s[5]));
+      } else {
+        gcf = region_cf_parts<T>(kind, wat(gw.s[0]), wat(gw.s[1]), wat(gw.s[2]));
+      }
+    }
   }
 #pragma unroll
   for (int k = 0; k < KL; ++k)
@@ -1655,6 +1768,16 @@ template <typename T>
 __global__ void __launch_bounds__(kTile) k_ring(LevelDev<T> L, const T* __restrict__ Pq) {
   const int t = blockIdx.x * kTile + threadIdx.x;
   if (t < L.nring_all) ring_item<T>(L, Pq, t);
+}
+
+// the QC ring refresh of ebFillGhosts (patches with coarse-fine faces) alone
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_ring_cf(LevelDev<T> L, const T* __restrict__ Pq) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t < L.nring_cf) {
+    const RingRec rr = L.ring_cf[t];
+    L.QC[rr.dst] = Pq[rr.src];
+  }
 }
 
 template <typename T>
@@ -2596,12 +2719,14 @@ void launch_fill(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, hip
 
 template <typename T>
 void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, int frame, T omega,
-                         int homog, hipStream_t s, T* Qout, int ghosts, T* Rq, T* Rdup, int ring) {
+                         int homog, hipStream_t s, T* Qout, int ghosts, T* Rq, T* Rdup, int ring, int corr) {
   if (!L.nregion || nsweeps <= 0) return;
   const int per = (L.nregion + kXcds - 1) / kXcds;
   const int rg = (ring && parent) ? 1 : 0;
   const int rring = rg ? (L.nring_cf + L.nregion - 1) / L.nregion : 0;
-  const size_t lds = region_lds_bytes<T>(L.rslots, L.rghmax, std::min(rring, kRegionThreads), ghosts, rg);
+  const int cr = (corr && parent && L.rcorr) ? 1 : 0;
+  const size_t lds = region_lds_bytes<T>(L.rslots, L.rghmax, std::min(rring, kRegionThreads), ghosts, rg,
+                                         cr ? L.rwinmax : 0);
   // the kernel for the level's spacing and omega (compile-time forms of the
   // update, the same numbers as the general one)
   static bool raised = false;  // the canvas plus staged records can pass 64 KB (gfx950: 160 KB per CU)
@@ -2617,7 +2742,7 @@ void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int ns
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(per * kXcds), dim3(kRegionThreads), lds, s, L,
                        parent ? (const T*)parent->Q : (const T*)nullptr, Qout ? Qout : L.Qs, 2 * nsweeps, frame,
-                       omega, homog, per, ghosts, Rq, Rdup, (ring && parent) ? 1 : 0);
+                       omega, homog, per, ghosts, Rq, Rdup, (ring && parent) ? 1 : 0, cr);
   };
   const bool w1 = omega == T(1);
   if (L.pow2 && w1) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, true>);
@@ -2654,6 +2779,12 @@ template <typename T>
 void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s) {
   if (!L.nring_all) return;
   hipLaunchKernelGGL(k_ring<T>, dim3(nblk(L.nring_all)), dim3(kTile), 0, s, L, (const T*)parent.Q);
+}
+
+template <typename T>
+void launch_ring_cf(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s) {
+  if (!L.nring_cf) return;
+  hipLaunchKernelGGL(k_ring_cf<T>, dim3(nblk(L.nring_cf)), dim3(kTile), 0, s, L, (const T*)parent.Q);
 }
 
 template <typename T>
@@ -2705,14 +2836,19 @@ void launch_coarse_cycle(const LevelDev<T>* LV_dev, const LevelDev<T>* LV_host, 
   const long long rhs_b = stage_align((long long)C.M * (long long)sizeof(double));
   long long stage_b = 0;
   for (int k = 0; k < kc; ++k) stage_b += stage_level_bytes<T>(LV_host[k], k + 1 < kc ? LV_host[k + 1].ncorr : -1);
-  const int staged = (kc <= kStageMaxLevels && rhs_b + stage_b <= kStageLdsLimit) ? 1 : 0;
-  const size_t lds = staged ? (size_t)(rhs_b + stage_b) : (size_t)C.M * sizeof(double);
-  static bool raised = false;  // allow the full 160 KB of gfx950 LDS for dynamic allocations
-  if (!raised) {
-    (void)hipFuncSetAttribute((const void*)k_coarse_cycle<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)kStageLdsLimit);
-    raised = true;
+  // the dynamic budget: gfx950's 160 KB of LDS per workgroup less the
+  // kernel's static arrays (the staged level descriptors among them), at
+  // most kStageLdsLimit
+  static long long cap = -1;
+  if (cap < 0) {
+    hipFuncAttributes fa{};
+    long long st = 8192;
+    if (hipFuncGetAttributes(&fa, (const void*)k_coarse_cycle<T>) == hipSuccess) st = (long long)fa.sharedSizeBytes;
+    cap = std::min<long long>(kStageLdsLimit, 160LL * 1024 - st);
+    (void)hipFuncSetAttribute((const void*)k_coarse_cycle<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
   }
+  const int staged = (kc <= kStageMaxLevels && rhs_b + stage_b <= cap) ? 1 : 0;
+  const size_t lds = staged ? (size_t)(rhs_b + stage_b) : (size_t)C.M * sizeof(double);
   hipLaunchKernelGGL(k_coarse_cycle<T>, dim3(1), dim3(1024), lds, s, LV_dev, kc, C, npre, npost, omega, homog,
                      mass_corr, staged);
 }
@@ -3019,12 +3155,13 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_relax_tiled<T>(const LevelDev<T>&, const LevelDev<T>*, int, T, int, hipStream_t); \
   template void launch_fill<T>(const LevelDev<T>&, const LevelDev<T>*, int, hipStream_t);               \
   template void launch_relax_region<T>(const LevelDev<T>&, const LevelDev<T>*, int, int, T, int, hipStream_t, T*, int, \
-                                       T*, T*, int);                                                        \
+                                       T*, T*, int, int);                                                   \
   template void launch_fill_copy<T>(const LevelDev<T>&, const LevelDev<T>*, int, const T*, hipStream_t); \
   template void launch_fill_copy_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, const T*, T*,        \
                                              hipStream_t);                                                  \
   template void launch_refresh_cf<T>(const LevelDev<T>&, const LevelDev<T>*, hipStream_t);              \
   template void launch_ring<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                    \
+  template void launch_ring_cf<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                 \
   template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, T*);       \
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, int, const T*); \

@@ -532,7 +532,9 @@ struct RegionBuild {
   std::vector<unsigned> cells;       // kRegionStride per tile
   std::vector<RegionGhost> ghosts;   // per tile, RegionTile::gofs / ngh
   std::vector<RegionRestrict> rs;    // per tile, RegionTile::rofs / nrs
-  int max_slots = 0, max_gh = 0, max_rs = 0;
+  std::vector<RegionWin> win;        // per tile, RegionTile::wofs / nwin (build_region_windows)
+  std::vector<RegionGhostW> gw;      // parallel to ghosts
+  int max_slots = 0, max_gh = 0, max_rs = 0, max_win = 0;
 };
 
 // The form of the face ghosts of cell (p, i, j) as a sort key: 0 off every
@@ -1023,6 +1025,186 @@ bool build_region_tiles(const HLevel& L, int rdepth, RegionBuild& out) {
     out.max_slots = std::max(out.max_slots, D.cw * D.ch);
   }
   return !out.tiles.empty();
+}
+
+// Parent-window sources of the coarse-fine part of ghost record `rec` of
+// level L (the stencils of cf_rec and of the coarse-fine corners): n window
+// coordinates (wx, wy) of the record's patch, in the order the record's
+// parent indices s[2..4] hold them; 0 for a record without a parent part.
+// rp: each record's patch.
+static int cf_window_sources(const HLevel& L, const std::vector<int>& rp, int rec, int wx[3], int wy[3]) {
+  const GhostRec& r = L.ghost[(size_t)rec];
+  if (r.kind != G_QIF && r.kind != G_QIF_END && r.kind != G_LIF && r.kind != G_LIF_CORNER) return 0;
+  const PatchInfo& P = L.pi[(size_t)rp[(size_t)rec]];
+  const int mx = P.mx, my = P.my, cmx = mx / 2, cmy = my / 2;
+  int t = rec - P.gbase, side;
+  if (t < my) side = 0;
+  else if ((t -= my) < my) side = 1;
+  else if ((t -= my) < mx) side = 2;
+  else if ((t -= mx) < mx) side = 3;
+  else {
+    t -= mx;  // corner t: SW SE NW NE
+    if (t < 0 || t > 3) return -1;
+    wx[0] = (t & 1) ? cmx + 1 : 0;
+    wy[0] = (t & 2) ? cmy + 1 : 0;
+    return 1;
+  }
+  const int len = side < 2 ? my : mx, clen = side < 2 ? cmy : cmx, u = t + 1;
+  const int fixed = side == 0 ? 0 : side == 1 ? cmx + 1 : side == 2 ? 0 : cmy + 1;
+  int along[3], n = 1;
+  if (r.kind == G_QIF_END) {
+    n = 3;
+    for (int k = 0; k < 3; ++k) along[k] = u == len ? clen + 1 - k : k;
+  } else {
+    along[0] = (u & 1) == 0 ? u / 2 : (u + 1) / 2;
+  }
+  for (int k = 0; k < n; ++k) {
+    wx[k] = side < 2 ? fixed : along[k];
+    wy[k] = side < 2 ? along[k] : fixed;
+  }
+  return n;
+}
+
+// The tables of the coarse-grid correction folded into a level's
+// post-smoothing region launch (ebTransfer.m:57-71 before ebRelaxLevel.m:28-38):
+// for every tile, per patch, the rectangle of the patch's QC window that its
+// region cells' corrections (the 2 x 2 window cells of each cell) and the
+// parent parts of its face cells' and ghost records' coarse-fine stencils
+// read, staged as (QC index, parent index) cells; for every ghost record of
+// the tile the window slots of its parent sources.  Every stencil is checked
+// against the record's parent indices; false (the level keeps the separate
+// correction launch) when one differs or a tile's window outgrows the
+// workgroup.
+bool build_region_windows(const HLevel& L, int parent_nq, RegionBuild& rb) {
+  rb.win.clear();
+  rb.gw.clear();
+  rb.max_win = 0;
+  if (L.wmap.empty() || rb.tiles.empty()) return false;
+  std::vector<int> rp(L.ghost.size(), -1);
+  for (int p = 0; p < L.np; ++p) {
+    const PatchInfo& P = L.pi[(size_t)p];
+    for (int t = 0; t < 2 * (P.mx + P.my) + 4; ++t) rp[(size_t)P.gbase + t] = p;
+  }
+  std::vector<int> rec_at((size_t)std::max(1, L.nq), -1);
+  for (size_t r = 0; r < L.ghost.size(); ++r)
+    if (L.ghost[r].dst >= 0 && L.ghost[r].dst < L.nq) rec_at[(size_t)L.ghost[r].dst] = (int)r;
+  // every coarse-fine record's stencil against its parent indices
+  for (size_t r = 0; r < L.ghost.size(); ++r) {
+    int wx[3], wy[3];
+    const int n = cf_window_sources(L, rp, (int)r, wx, wy);
+    if (n < 0) return false;
+    const PatchInfo& P = L.pi[(size_t)rp[r]];
+    const int ldc = P.mx / 2 + 2;
+    for (int k = 0; k < n; ++k) {
+      if (wx[k] < 0 || wy[k] < 0 || wx[k] > P.mx / 2 + 1 || wy[k] > P.my / 2 + 1) return false;
+      const int w = P.qcoff + wx[k] + wy[k] * ldc;
+      if (w < 0 || w >= L.nqc || L.wmap[(size_t)w] != L.ghost[r].s[2 + k]) return false;
+    }
+  }
+  const int per_colour = kRegionThreads * kRegionK;
+  for (size_t x = 0; x < rb.tiles.size(); ++x) {
+    RegionTile& T = rb.tiles[x];
+    RegionPatch* sp = rb.patches.data() + x * kRegionMaxPatches;
+    const int* pid = rb.pid.data() + x * kRegionMaxPatches;
+    int lo[kRegionMaxPatches][2], hi[kRegionMaxPatches][2];
+    for (int k = 0; k < kRegionMaxPatches; ++k) lo[k][0] = lo[k][1] = 1 << 30, hi[k][0] = hi[k][1] = -1;
+    auto add = [&](int ps, int wx, int wy) {
+      lo[ps][0] = std::min(lo[ps][0], wx); hi[ps][0] = std::max(hi[ps][0], wx);
+      lo[ps][1] = std::min(lo[ps][1], wy); hi[ps][1] = std::max(hi[ps][1], wy);
+    };
+    auto slot_of_patch = [&](int p) {
+      for (int k = 0; k < T.np; ++k)
+        if (pid[k] == p) return k;
+      return -1;
+    };
+    const unsigned* cells = rb.cells.data() + x * kRegionStride;
+    auto each_cell = [&](auto&& fn) {
+      for (int e = 0; e < kRegionStride; ++e) {
+        if (cells[e] == kRegionNone) continue;
+        const unsigned c = cells[e];
+        fn((int)(c & 15), (int)((c >> 4) & 4095), (int)((c >> 16) & 4095), e < 2 * per_colour);
+      }
+    };
+    // the cells' corrections and the updated face cells' parent parts
+    bool ok = true;
+    each_cell([&](int ps, int i, int j, bool updated) {
+      const int wx = (i >> 1) + 1, wy = (j >> 1) + 1;
+      add(ps, wx, wy);
+      add(ps, wx + ((i & 1) ? 1 : -1), wy + ((j & 1) ? 1 : -1));
+      if (!updated) return;
+      const PatchInfo& P = sp[ps].pi;
+      const int recs[4] = {i == 0 ? P.gbase + j : -1, i == P.mx - 1 ? P.gbase + P.my + j : -1,
+                           j == 0 ? P.gbase + 2 * P.my + i : -1, j == P.my - 1 ? P.gbase + 2 * P.my + P.mx + i : -1};
+      for (int side = 0; side < 4; ++side) {
+        if (recs[side] < 0 || P.bc[side]) continue;
+        int sx[3], sy[3];
+        const int n = cf_window_sources(L, rp, recs[side], sx, sy);
+        if (n < 0) ok = false;
+        for (int k = 0; k < n; ++k) add(ps, sx[k], sy[k]);
+      }
+    });
+    if (!ok) return false;
+    // the ghost records' parent parts
+    std::vector<RegionGhostW> gw((size_t)T.ngh);
+    for (int g = 0; g < T.ngh; ++g) {
+      RegionGhostW& o = gw[(size_t)g];
+      for (int k = 0; k < 8; ++k) o.s[k] = kRegionNoSlot;
+      const RegionGhost& G = rb.ghosts[(size_t)T.gofs + g];
+      const int rec = rec_at[(size_t)G.dst];
+      if (rec < 0) return false;
+      const GhostRec& r = L.ghost[(size_t)rec];
+      const int parts[2] = {r.kind == G_PCORNER ? r.s[0] : rec, r.kind == G_PCORNER ? r.s[1] : -1};
+      for (int q = 0; q < 2; ++q) {
+        if (parts[q] < 0) continue;
+        int sx[3], sy[3];
+        const int n = cf_window_sources(L, rp, parts[q], sx, sy);
+        if (n < 0) return false;
+        if (n == 0) continue;
+        const int ps = slot_of_patch(rp[(size_t)parts[q]]);
+        if (ps < 0) return false;
+        for (int k = 0; k < n; ++k) add(ps, sx[k], sy[k]);
+      }
+    }
+    // the rectangles, their window cells and the records' slots
+    T.wofs = (int)rb.win.size();
+    int nwin = 0;
+    for (int ps = 0; ps < T.np; ++ps) {
+      RegionPatch& R = sp[ps];
+      if (hi[ps][0] < 0) { R.eoff = nwin; R.ex0 = R.ey0 = 0; R.ew = 1; continue; }
+      const int ldc = R.pi.mx / 2 + 2;
+      if (lo[ps][0] < 0 || lo[ps][1] < 0 || hi[ps][0] > R.pi.mx / 2 + 1 || hi[ps][1] > R.pi.my / 2 + 1) return false;
+      R.eoff = nwin; R.ex0 = lo[ps][0]; R.ey0 = lo[ps][1]; R.ew = hi[ps][0] - lo[ps][0] + 1;
+      for (int wy = lo[ps][1]; wy <= hi[ps][1]; ++wy)
+        for (int wx = lo[ps][0]; wx <= hi[ps][0]; ++wx) {
+          const int w = R.pi.qcoff + wx + wy * ldc;
+          if (w < 0 || w >= L.nqc) return false;
+          const int wm = L.wmap[(size_t)w];
+          if (wm < 0 || wm >= parent_nq) return false;
+          rb.win.push_back(RegionWin{w, wm});
+          ++nwin;
+        }
+    }
+    if (nwin > kRegionMaxWin) return false;
+    T.nwin = nwin;
+    rb.max_win = std::max(rb.max_win, nwin);
+    for (int g = 0; g < T.ngh; ++g) {
+      RegionGhostW& o = gw[(size_t)g];
+      const RegionGhost& G = rb.ghosts[(size_t)T.gofs + g];
+      const GhostRec& r = L.ghost[(size_t)rec_at[(size_t)G.dst]];
+      const int parts[2] = {r.kind == G_PCORNER ? r.s[0] : rec_at[(size_t)G.dst], r.kind == G_PCORNER ? r.s[1] : -1};
+      for (int q = 0; q < 2; ++q) {
+        if (parts[q] < 0) continue;
+        int sx[3], sy[3];
+        const int n = cf_window_sources(L, rp, parts[q], sx, sy);
+        if (n <= 0) continue;
+        const RegionPatch& R = sp[slot_of_patch(rp[(size_t)parts[q]])];
+        for (int k = 0; k < n; ++k)
+          o.s[3 * q + k] = (unsigned short)(R.eoff + (sx[k] - R.ex0) + (sy[k] - R.ey0) * R.ew);
+      }
+    }
+    rb.gw.insert(rb.gw.end(), gw.begin(), gw.end());
+  }
+  return true;
 }
 
 // The parent ghost records behind the FAS load's parent reads of level F
@@ -1672,16 +1854,35 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
   D.nregion = 0; D.rdepth = 0; D.rslots = 0;
   D.region = nullptr; D.rpatch = nullptr; D.rcell = nullptr; D.rghost = nullptr; D.rrs = nullptr;
   D.rghmax = 0; D.rrsmax = 0;
+  D.rgw = nullptr; D.rwin = nullptr; D.rcorr = 0; D.rwinmax = 0;
   if (D.arith && !D.stream && L.np > 0) {
     if (tables_from64) {
       D.nregion = d64->nregion; D.rdepth = d64->rdepth; D.rslots = d64->rslots;
       D.region = d64->region; D.rpatch = d64->rpatch; D.rcell = d64->rcell;
       D.rghost = d64->rghost; D.rrs = d64->rrs; D.rghmax = d64->rghmax; D.rrsmax = d64->rrsmax;
+      D.rgw = d64->rgw; D.rwin = d64->rwin; D.rcorr = d64->rcorr; D.rwinmax = d64->rwinmax;
     } else {
       const int rdepth = 2 * std::max(3, std::max(h->opts.npre, h->opts.npost));
       RegionBuild rb;
       if (rdepth <= kRegionMaxDepth && build_region_tiles(L, rdepth, rb)) {
         if ((rc = check_region_tables(L, rb))) return rc;
+        // the correction folded into the post-smoothing launch (levels with a parent)
+        const long long kl = &L - h->H.data();
+        // the post-smoothing launch's LDS with the window: canvas, ghost
+        // records (at most 80 bytes each), ring staging, window values
+        auto a16 = [](long long b) { return (b + 15) / 16 * 16; };
+        const long long ntile = (long long)rb.tiles.size();
+        const long long rring = std::min<long long>(kRegionThreads, ((long long)L.ring_cf.size() + ntile - 1) / ntile);
+        auto lds_with = [&](int win) {
+          return a16(8LL * rb.max_slots) + a16(80LL * rb.max_gh) + a16(4 * rring) + a16(8 * rring) + 2 * a16(8LL * win);
+        };
+        if (kl > 0 && build_region_windows(L, h->H[(size_t)kl - 1].nq, rb) &&
+            lds_with(rb.max_win) + 8192 <= kStageLdsLimit) {
+          if ((rc = upload(h, &D.rwin, rb.win))) return rc;
+          if ((rc = upload(h, &D.rgw, rb.gw))) return rc;
+          D.rcorr = 1;
+          D.rwinmax = rb.max_win;
+        }
         if ((rc = upload(h, &D.region, rb.tiles))) return rc;
         if ((rc = upload(h, &D.rpatch, rb.patches))) return rc;
         if ((rc = upload(h, &D.rcell, rb.cells))) return rc;
@@ -1750,6 +1951,19 @@ struct Seq {
   // that smooths with region launches swaps Q and Qs at every relax call
   // (pre-smoothing into Qs, post-smoothing back into Q)
   std::vector<LevelDev<T>> cur;
+  // levels whose post-smoothing launch applied the correction and left out
+  // the fill's QC ring refresh (other workgroups of that launch read the old
+  // ring for their corrections); flush_rings refreshes them
+  std::vector<char> ringp;
+  void flush_ring(int k) {
+    if (k >= 2 && k <= (int)ringp.size() && ringp[k - 1]) {
+      launch_ring_cf<T>(lev(k), lev(k - 1), s);
+      ringp[k - 1] = 0;
+    }
+  }
+  void flush_rings() {
+    for (int k = 2; k <= (int)ringp.size(); ++k) flush_ring(k);
+  }
   const LevelDev<T>& lev(int k) {
     if (cur.empty()) cur = L;
     return cur[k - 1];
@@ -1780,8 +1994,10 @@ struct Seq {
   // with the QC ring refresh (the fill), and the level's Q and Qs swap; 2 =
   // the same with the restriction into the parent (and rdup) in place of the
   // ring refresh, which the FAS load's gather redoes
+  // corr: the launch applies the pending coarse-grid correction of level k
+  // first (swap 1 only, levels with L.rcorr)
   void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false, bool defer_fill = false,
-             int swap = 0, T* rdup = nullptr) {
+             int swap = 0, T* rdup = nullptr, bool corr = false) {
     const bool qs_current = qs_ready == k;
     qs_ready = 0;
     if (region(k, nsweeps)) {
@@ -1793,8 +2009,13 @@ struct Seq {
       if (swap) {
         const LevelDev<T>& Lk = lev(k);
         T* rq = swap == 2 ? cur[k - 2].Q : nullptr;
+        const bool cr = corr && swap == 1;
         launch_relax_region<T>(Lk, par(k), nsweeps, first_inline == 0 ? 1 : 0, (T)h->opts.omega, homog, s, Lk.Qs, 1,
-                               rq, swap == 2 ? rdup : nullptr, swap == 1 ? 1 : 0);
+                               rq, swap == 2 ? rdup : nullptr, swap == 1 && !cr ? 1 : 0, cr ? 1 : 0);
+        if (cr) {
+          if ((int)ringp.size() < h->nlev) ringp.assign((size_t)h->nlev, 0);
+          ringp[(size_t)k - 1] = 1;
+        }
         std::swap(cur[k - 1].Q, cur[k - 1].Qs);
         return;
       }
@@ -1946,6 +2167,12 @@ struct Seq {
     unfilled = pskip ? k - 1 : 0;
     vcycle(k - 1, kc);
     const bool prep = h->opts.npost > 0 && lev(k).arith && lev(k).aux && h->opts.smoother != 1;
+    if (sw && lev(k).rcorr) {
+      // the post-smoothing launch applies the correction itself, with the
+      // parent parts of its coarse-fine ghosts from the corrected parent
+      relax(k, h->opts.npost, 2, false, false, 1, nullptr, true);
+      return;
+    }
     if (odd_stream(k, h->opts.npost)) {
       launch_correct<T>(lev(k), lev(k - 1), 0, s, prep ? 1 : 0, lev(k).Qs);
       qs_ready = k;
@@ -2336,7 +2563,9 @@ int bsam_coarsest_solve(bsam_hier* h) {
 
 int bsam_vcycle(bsam_hier* h) {
   if (int rc_ = need_dev(h)) return rc_;
-  seq64(h, h->stream).vcycle(h->nlev, coarse_cut(h, h->opts.coarse_cells));
+  Seq<double> q = seq64(h, h->stream);
+  q.vcycle(h->nlev, coarse_cut(h, h->opts.coarse_cells));
+  q.flush_rings();
   return sync(h);
 }
 
@@ -2388,6 +2617,10 @@ static int enqueue_cycle64(bsam_hier* h, hipStream_t s) {
   Seq<double> q = seq64(h, s);
   enqueue_reset_force<double>(h, h->lv64, s);
   q.vcycle(h->nlev, coarse_cut(h, h->opts.coarse_cells));
+  // rings the correcting post-smoothing left out: the filldown's fills
+  // refresh those of levels below the top, the residual's QC refresh the top
+  // level's when it is an AMR level with flux corrections; else refresh it
+  if (h->nlev <= h->root || !h->opts.mass_correction) q.flush_ring(h->nlev);
   q.filldown();
   enqueue_residual(h, nullptr, s);
   return 0;
