@@ -246,6 +246,8 @@ struct LevelDev {
   int* tgt_of_cell;  // per cell: target row or -1
   int* wmap;         // per QC cell: parent Q index
   int* wgr;          // per QC cell: the parent ghost record at wmap (-1: interior)
+  int2* pcov;        // per parent Q index: (Q index of the first fine cell of the 2 x 2 block
+                     // that covers it, fine row stride), x = -1 where no patch covers it
   int* ring_rec;     // per ring_all item: the parent ghost record at its source (-1: interior)
   int pgh_ok;        // every such record is physical (the load may evaluate it)
   uint8_t* vis;      // per cell: 1 if not covered by the next level

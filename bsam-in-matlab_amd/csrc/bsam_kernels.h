@@ -82,6 +82,9 @@ void launch_tag_finish(const uint8_t* raw, const uint8_t* cover, int n1, int n2,
 template <typename T> void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
 template <typename T>
 void launch_ring_cf(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
+template <typename T>
+void launch_restrict_fill(const LevelDev<T>& F, const LevelDev<T>& P, const LevelDev<T>* PP, int homog,
+                          hipStream_t s);
 // restrict k -> k-1 + fill k-1 for k = kfrom..2 in one workgroup
 template <typename T>
 void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int homog, hipStream_t s);

@@ -159,6 +159,69 @@ __device__ __forceinline__ void fill_item(const LevelDev<T>& L, const T* __restr
   }
 }
 
+// A level's interior values as they stand after the restriction from its
+// child: a cell the child covers reads the 2 x 2 average of the child's cells
+// (restrict_item's expression), any other cell its own value
+template <typename T>
+struct CoveredQ {
+  const T* __restrict__ Q;
+  const T* __restrict__ Qf;
+  const int2* __restrict__ cov;
+  __device__ __forceinline__ T operator()(int q) const {
+    const int2 c = cov[q];
+    if (c.x < 0) return Q[q];
+    return T(0.25) * (((Qf[c.x] + Qf[c.x + 1]) + Qf[c.x + c.y]) + Qf[c.x + c.y + 1]);
+  }
+};
+
+// ghost_value with the interior read through V
+template <typename T, typename V>
+__device__ __forceinline__ T ghost_value_via(const GhostRec& r, const V& Q, const T* __restrict__ Pq, int homog) {
+  switch (r.kind) {
+    case G_COPY:
+      return Q(r.s[0]);
+    case G_QIF:
+    case G_QIF_END:
+      return (T(A1_) * Q(r.s[0]) + T(A2_) * Q(r.s[1])) + cf_part<T>(r, Pq);
+    case G_LIF:
+      return (cf_part<T>(r, Pq) + T(C23_) * Q(r.s[0])) - T(C13_) * Q(r.s[1]);
+    case G_LIF_CORNER:
+      return cf_part<T>(r, Pq) + T(C13_) * Q(r.s[0]);
+    case G_DIR: {
+      const T g = homog ? T(0) : T(r.c);
+      return (T(C83_) * g - T(2) * Q(r.s[0])) + T(C13_) * Q(r.s[1]);
+    }
+    case G_NEU: {
+      const T hg = homog ? T(0) : T(r.c);
+      return Q(r.s[0]) + hg;
+    }
+    default:
+      return T(0);
+  }
+}
+
+// fill_item with the interior read through V (its writes are fill_item's)
+template <typename T, typename V>
+__device__ __forceinline__ void fill_item_via(const LevelDev<T>& L, const T* __restrict__ Pq, int t, int homog,
+                                              const V& Q) {
+  if (t < L.nghost) {
+    const GhostRec r = L.ghost[t];
+    T v;
+    if (r.kind == G_PCORNER) {
+      const T a = ghost_value_via<T>(L.ghost[r.s[0]], Q, Pq, homog);
+      const T b = ghost_value_via<T>(L.ghost[r.s[1]], Q, Pq, homog);
+      v = (a + b) - Q(r.s[2]);
+    } else {
+      v = ghost_value_via<T>(r, Q, Pq, homog);
+      if (L.aux && is_cf_kind(r.kind)) L.aux[t].cf = cf_part<T>(r, Pq);
+    }
+    L.Q[r.dst] = v;
+  } else if (Pq && t < L.nghost + L.nring_cf) {
+    const RingRec rr = L.ring_cf[t - L.nghost];
+    L.QC[rr.dst] = Pq[rr.src];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ int fill_items(const LevelDev<T>& L, const T* Pq) {
   return L.nghost + (Pq ? L.nring_cf : 0);
@@ -1780,6 +1843,22 @@ __global__ void __launch_bounds__(kTile) k_ring_cf(LevelDev<T> L, const T* __res
   }
 }
 
+// A filldown step (ebTransfer.m:73-79: restrict level k, fill level k-1) as
+// one launch: items [0, nfoot) restrict F into P and F's QC interior; the
+// rest fill P's ghosts (and QC ring), reading P's covered cells as the
+// restriction of F's values (CoveredQ) rather than P's cells the first items
+// write, so nothing in the launch reads what another item writes
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_restrict_fill(LevelDev<T> F, LevelDev<T> P, const T* __restrict__ PPq,
+                                                         int homog) {
+  const int t = blockIdx.x * kTile + threadIdx.x;
+  if (t < F.nfoot) {
+    restrict_item<T>(F, P.Q, t, 0);
+    return;
+  }
+  fill_item_via<T>(P, PPq, t - F.nfoot, homog, CoveredQ<T>{P.Q, F.Q, F.pcov});
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kTile) k_restrict(LevelDev<T> L, T* __restrict__ Pq, int qc_only,
                                                     T* __restrict__ Pdup) {
@@ -2782,6 +2861,15 @@ void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s)
 }
 
 template <typename T>
+void launch_restrict_fill(const LevelDev<T>& F, const LevelDev<T>& P, const LevelDev<T>* PP, int homog,
+                          hipStream_t s) {
+  const long long n = (long long)F.nfoot + P.nghost + (PP ? P.nring_cf : 0);
+  if (!n) return;
+  hipLaunchKernelGGL(k_restrict_fill<T>, dim3(nblk(n)), dim3(kTile), 0, s, F, P, PP ? (const T*)PP->Q : nullptr,
+                     homog);
+}
+
+template <typename T>
 void launch_ring_cf(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s) {
   if (!L.nring_cf) return;
   hipLaunchKernelGGL(k_ring_cf<T>, dim3(nblk(L.nring_cf)), dim3(kTile), 0, s, L, (const T*)parent.Q);
@@ -3162,6 +3250,8 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_refresh_cf<T>(const LevelDev<T>&, const LevelDev<T>*, hipStream_t);              \
   template void launch_ring<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                    \
   template void launch_ring_cf<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                 \
+  template void launch_restrict_fill<T>(const LevelDev<T>&, const LevelDev<T>&, const LevelDev<T>*, int,   \
+                                        hipStream_t);                                                     \
   template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, T*);       \
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, int, const T*); \

@@ -1797,7 +1797,7 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
     D.ring_cf = d64->ring_cf; D.foot = d64->foot; D.corr = d64->corr; D.tgt_cell = d64->tgt_cell;
     D.tgt_ptr = d64->tgt_ptr; D.tgt_ent = d64->tgt_ent; D.tgt_of_cell = d64->tgt_of_cell;
     D.wmap = d64->wmap; D.vis = d64->vis; D.stiles = d64->stiles;
-    D.wgr = d64->wgr; D.ring_rec = d64->ring_rec;
+    D.wgr = d64->wgr; D.ring_rec = d64->ring_rec; D.pcov = d64->pcov;
   } else {
     if ((rc = upload(h, &D.P, L.pi))) return rc;
     if ((rc = upload(h, &D.tiles, L.tiles))) return rc;
@@ -1816,6 +1816,19 @@ int make_dev_level(bsam_hier* h, HLevel& L, LevelDev<T>& D, bool tables_from64, 
     if ((rc = upload(h, &D.vis, L.vis))) return rc;
     if ((rc = upload(h, &D.wgr, L.wgr))) return rc;
     if ((rc = upload(h, &D.ring_rec, L.ring_rec))) return rc;
+    // the parent's covered cells -> this level's 2 x 2 blocks (fused filldown steps)
+    D.pcov = nullptr;
+    const long long kl = &L - h->H.data();
+    if (kl > 0 && !L.foot.empty()) {
+      const HLevel& C = h->H[(size_t)kl - 1];
+      std::vector<int2> cov((size_t)C.nq, make_int2(-1, 0));
+      for (const FootRec& fr : L.foot) {
+        const int ldq = fr.mx + 2;
+        if (fr.pq < 0 || fr.pq >= C.nq) return fail(BSAM_E_ARG, "bsamgpu:internal: footprint parent index out of range");
+        cov[(size_t)fr.pq] = make_int2(fr.qoff + (2 * fr.ci + 1) + (2 * fr.cj + 1) * ldq, ldq);
+      }
+      if ((rc = upload(h, &D.pcov, cov))) return rc;
+    }
   }
   D.pgh_ok = L.pgh_ok;
   if ((rc = upload_cast(h, &D.Q, L.Q))) return rc;
@@ -2191,6 +2204,10 @@ struct Seq {
   void filldown(int klo = 2) {
     const int kb = std::max(filldown_cut(h), klo - 1);
     for (int kk = h->nlev; kk > kb; --kk) {
+      if (lev(kk).pcov) {  // the step as one launch
+        launch_restrict_fill<T>(lev(kk), lev(kk - 1), par(kk - 1), homog, s);
+        continue;
+      }
       restrict_(kk);
       fill(kk - 1);
     }
