@@ -2292,8 +2292,10 @@ __global__ void __launch_bounds__(kTile) k_qc_refresh_all(const LevelDev<T>* __r
   const LevelDev<T>& L = LV[it.x];
   if (it.y < L.nring_all) {
     ring_item<T>(L, LV[it.x - 1].Q, it.y);
-  } else {
+  } else if (it.y < L.nring_all + L.nfoot) {
     restrict_item<T>(L, (T*)nullptr, it.y - L.nring_all, 1);
+  } else {  // a coarse-fine record's parent part (the top level's, see qc_items)
+    keep_face_cf<T>(L, LV[it.x - 1].Q, it.y - L.nring_all - L.nfoot);
   }
 }
 

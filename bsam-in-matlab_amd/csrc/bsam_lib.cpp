@@ -1968,6 +1968,9 @@ struct Seq {
   // the fill's QC ring refresh (other workgroups of that launch read the old
   // ring for their corrections); flush_rings refreshes them
   std::vector<char> ringp;
+  // the top level's parent parts are current (the previous cycle's residual
+  // refreshed them after the filldown, or the solve's first fill stored them)
+  bool top_cf_current = false;
   void flush_ring(int k) {
     if (k >= 2 && k <= (int)ringp.size() && ringp[k - 1]) {
       launch_ring_cf<T>(lev(k), lev(k - 1), s);
@@ -2157,7 +2160,7 @@ struct Seq {
     // pre-smoothing from assembled ghosts (what that fill would have written)
     const bool first_assembled = unfilled == k;
     unfilled = 0;
-    relax(k, h->opts.npre, first_assembled ? 2 : 0, k < h->nlev, fuse, sw ? 2 : 0, dup);
+    relax(k, h->opts.npre, first_assembled ? 2 : 0, k < h->nlev || top_cf_current, fuse, sw ? 2 : 0, dup);
     if (dup) qs_ready = k - 1;
     if (sw) {
     } else if (fuse) {
@@ -2438,6 +2441,14 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
     if (k + 1 >= std::max(2, h->root + 1)) {
       const int n = (int)h->H[k].ring_all.size() + (int)h->H[k].foot.size();
       for (int t = 0; t < n; ++t) qc_items.push_back(make_int2(k, t));
+      // the top level's parent parts, from the parent as the filldown left it:
+      // the next cycle's first relax call of the top level reads them
+      if (k + 1 == h->nlev)
+        for (int g = 0; g < (int)h->H[k].ghost.size(); ++g) {
+          const int kind = h->H[k].ghost[(size_t)g].kind;
+          if (kind == G_QIF || kind == G_QIF_END || kind == G_LIF || kind == G_LIF_CORNER)
+            qc_items.push_back(make_int2(k, n + g));
+        }
     }
   }
   h->part_off[h->nlev] = h->nparts;
@@ -2633,6 +2644,7 @@ int bsam_comp_residual(bsam_hier* h, bsam_residual* out) {
 static int enqueue_cycle64(bsam_hier* h, hipStream_t s) {
   Seq<double> q = seq64(h, s);
   enqueue_reset_force<double>(h, h->lv64, s);
+  q.top_cf_current = h->opts.mass_correction && h->nlev >= std::max(2, h->root + 1);
   q.vcycle(h->nlev, coarse_cut(h, h->opts.coarse_cells));
   // rings the correcting post-smoothing left out: the filldown's fills
   // refresh those of levels below the top, the residual's QC refresh the top
