@@ -97,8 +97,10 @@ template <typename T> void launch_coarse_load(const LevelDev<T>& L, const LevelD
 // evaluates the parent's physical ghosts it reads (requires L.pgh_ok).  PPq
 // (the parent's parent Q): the same launch also stores the parent's parent
 // parts in its aux, as that fill would have
+// from_ft: the reflux sums start from FTrue instead of F (loops that reset F
+// once before their first cycle; see reflux_item)
 template <typename T> void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s,
-                                       int pghost = 0, int homog = 0, const T* PPq = nullptr);
+                                       int pghost = 0, int homog = 0, const T* PPq = nullptr, int from_ft = 0);
 // prep != 0 (levels with L.arith): also store the parent parts of the face
 // records and push corrected boundary values into sibling ghosts, for a
 // following smoother pass that assembles its ghosts (kInlArith); dup (may be
@@ -112,7 +114,7 @@ template <typename T> void launch_coarsest(const CoarsestDev& C, const LevelDev<
 // tables fit the workgroup's LDS they are staged there for the whole cycle.
 template <typename T>
 void launch_coarse_cycle(const LevelDev<T>* LV_dev, const LevelDev<T>* LV_host, int kc, const CoarsestDev& C,
-                         int npre, int npost, T omega, int homog, int mass_corr, hipStream_t s);
+                         int npre, int npost, T omega, int homog, int mass_corr, hipStream_t s, int from_ft = 0);
 
 // LDS staging of the coarse cycle: the arrays a V-cycle of levels 1..kc reads
 // or writes, in a fixed order shared by the host (sizing) and the kernel.

@@ -1651,10 +1651,16 @@ __device__ __forceinline__ void coarse_load_item(const LevelDev<T>& L, const Lev
   P.F[fr.pc] = R + Lc;
 }
 
+// from_ft: the sum starts from FTrue instead of F.  In a loop whose cycles
+// each begin with F = FTrue (ebSolveCycles.m:30-34) the two are equal at a
+// target cell, which only this item writes; the loop then resets F once
+// before its first cycle instead of before each one (the FAS load overwrites
+// covered cells without reading them, and no other cell's F changes).
 template <typename T>
-__device__ __forceinline__ void reflux_item(const LevelDev<T>& L, const LevelDev<T>& P, int t, int ring_from_parent) {
+__device__ __forceinline__ void reflux_item(const LevelDev<T>& L, const LevelDev<T>& P, int t, int ring_from_parent,
+                                            int from_ft = 0) {
   const int cell = P.tgt_cell[t];
-  double acc = (double)P.F[cell];
+  double acc = from_ft ? (double)P.FT[cell] : (double)P.F[cell];
   const H2<double> hc = level_h2_64<T>(P);
   for (int e = P.tgt_ptr[t]; e < P.tgt_ptr[t + 1]; ++e) {
     const CorrRec cr = L.corr[P.tgt_ent[e]];
@@ -1691,7 +1697,7 @@ __device__ __forceinline__ T parent_at(const LevelDev<T>& P, const T* __restrict
 // physical ghosts are evaluated here (homog: homogeneous boundary data)
 template <typename T>
 __device__ __forceinline__ void load_item(const LevelDev<T>& L, const LevelDev<T>& P, int t, int mass_corr,
-                                          int pghost = 0, int homog = 0) {
+                                          int pghost = 0, int homog = 0, int from_ft = 0) {
   if (t < L.nfoot) {
     const FootRec fr = L.foot[t];
     const FootRec& pi = fr;  // patch offsets carried inline
@@ -1726,7 +1732,7 @@ __device__ __forceinline__ void load_item(const LevelDev<T>& L, const LevelDev<T
       ring_item<T>(L, P.Q, t - L.nfoot);
     }
   } else {
-    reflux_item<T>(L, P, t - L.nfoot - L.nring_all, 1);
+    reflux_item<T>(L, P, t - L.nfoot - L.nring_all, 1, from_ft);
   }
 }
 
@@ -1880,11 +1886,11 @@ __global__ void __launch_bounds__(kTile) k_reflux(LevelDev<T> L, LevelDev<T> P) 
 
 template <typename T>
 __global__ void __launch_bounds__(kTile) k_load(LevelDev<T> L, LevelDev<T> P, int mass_corr, int pghost, int homog,
-                                                const T* __restrict__ PPq) {
+                                                const T* __restrict__ PPq, int from_ft) {
   const int t = blockIdx.x * kTile + threadIdx.x;
   const int n = load_items<T>(L, P, mass_corr);
   if (t < n) {
-    load_item<T>(L, P, t, mass_corr, pghost, homog);
+    load_item<T>(L, P, t, mass_corr, pghost, homog, from_ft);
   } else if (PPq && t < n + P.nghost) {
     // the parent's parent parts from its own parent (what the left-out fill
     // of the parent stores): the parent's pre-smoothing assembles with them
@@ -2025,7 +2031,7 @@ __device__ void blk_fill(Grp& g, const LevelDev<T>& L, const T* Pq, int homog) {
 // of at least C.M doubles for the coarsest solve
 template <typename T>
 __device__ void blk_vcycle(Grp& g, const LevelDev<T>* __restrict__ LV, int kc, const CoarsestDev& C, int npre,
-                           int npost, T omega, int homog, int mass_corr, double* rhs) {
+                           int npost, T omega, int homog, int mass_corr, double* rhs, int from_ft = 0) {
   for (int k = kc; k >= 2; --k) {
     const LevelDev<T>& L = LV[k - 1];
     const LevelDev<T>& P = LV[k - 2];
@@ -2034,7 +2040,7 @@ __device__ void blk_vcycle(Grp& g, const LevelDev<T>* __restrict__ LV, int kc, c
     g.sync();
     blk_fill<T>(g, P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
     const int n = load_items<T>(L, P, mass_corr);
-    for (int x = g.rank; x < n; x += g.size) load_item<T>(L, P, x, mass_corr);
+    for (int x = g.rank; x < n; x += g.size) load_item<T>(L, P, x, mass_corr, 0, 0, from_ft);
     g.sync();
   }
   {
@@ -2196,12 +2202,12 @@ __device__ void blk_stage_levels(const LevelDev<T>* __restrict__ LV, int kc, Lev
 template <typename T>
 __global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __restrict__ LV, int kc,
                                                        CoarsestDev C, int npre, int npost, T omega,
-                                                       int homog, int mass_corr, int staged) {
+                                                       int homog, int mass_corr, int staged, int from_ft) {
   extern __shared__ __attribute__((aligned(16))) double smem[];
   double* rhs = smem;
   Grp g = Grp::block();
   if (!staged) {
-    blk_vcycle<T>(g, LV, kc, C, npre, npost, omega, homog, mass_corr, rhs);
+    blk_vcycle<T>(g, LV, kc, C, npre, npost, omega, homog, mass_corr, rhs, from_ft);
     return;
   }
   // levels 1..kc live in LDS for the whole cycle; the cycle writes Q, F and QC
@@ -2210,7 +2216,7 @@ __global__ void __launch_bounds__(1024) k_coarse_cycle(const LevelDev<T>* __rest
   __shared__ int nseg_total[2];
   char* base = reinterpret_cast<char*>(smem) + stage_align((long long)C.M * (long long)sizeof(double));
   blk_stage_levels<T>(LV, kc, LS, base, seg, nseg_total);
-  blk_vcycle<T>(g, LS, kc, C, npre, npost, omega, homog, mass_corr, rhs);
+  blk_vcycle<T>(g, LS, kc, C, npre, npost, omega, homog, mass_corr, rhs, from_ft);
   __syncthreads();
   if (threadIdx.x == 0) {  // write back what the cycle changes: Q, F, QC
     int ns = 0, words = 0;
@@ -2899,12 +2905,12 @@ void launch_coarse_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mas
 
 template <typename T>
 void launch_load(const LevelDev<T>& L, const LevelDev<T>& parent, int mass_corr, hipStream_t s, int pghost,
-                 int homog, const T* PPq) {
+                 int homog, const T* PPq, int from_ft) {
   const long long n = (long long)L.nfoot + L.nring_all + (mass_corr ? parent.ntgt : 0);
   if (!n) return;
   const long long nr = (PPq && parent.aux) ? parent.nghost : 0;
   hipLaunchKernelGGL(k_load<T>, dim3(nblk(n + nr)), dim3(kTile), 0, s, L, parent, mass_corr, pghost, homog,
-                     nr ? PPq : (const T*)nullptr);
+                     nr ? PPq : (const T*)nullptr, from_ft);
 }
 
 template <typename T>
@@ -2922,7 +2928,7 @@ void launch_coarsest(const CoarsestDev& C, const LevelDev<T>& L1, int homog, hip
 
 template <typename T>
 void launch_coarse_cycle(const LevelDev<T>* LV_dev, const LevelDev<T>* LV_host, int kc, const CoarsestDev& C,
-                         int npre, int npost, T omega, int homog, int mass_corr, hipStream_t s) {
+                         int npre, int npost, T omega, int homog, int mass_corr, hipStream_t s, int from_ft) {
   const long long rhs_b = stage_align((long long)C.M * (long long)sizeof(double));
   long long stage_b = 0;
   for (int k = 0; k < kc; ++k) stage_b += stage_level_bytes<T>(LV_host[k], k + 1 < kc ? LV_host[k + 1].ncorr : -1);
@@ -2940,7 +2946,7 @@ void launch_coarse_cycle(const LevelDev<T>* LV_dev, const LevelDev<T>* LV_host, 
   const int staged = (kc <= kStageMaxLevels && rhs_b + stage_b <= cap) ? 1 : 0;
   const size_t lds = staged ? (size_t)(rhs_b + stage_b) : (size_t)C.M * sizeof(double);
   hipLaunchKernelGGL(k_coarse_cycle<T>, dim3(1), dim3(1024), lds, s, LV_dev, kc, C, npre, npost, omega, homog,
-                     mass_corr, staged);
+                     mass_corr, staged, from_ft);
 }
 
 template <typename T>
@@ -3256,7 +3262,7 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
                                         hipStream_t);                                                     \
   template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, T*);       \
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
-  template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, int, const T*); \
+  template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, int, const T*, int); \
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, T*);        \
   template void launch_filldown_small<T>(const LevelDev<T>*, int, int, int, hipStream_t);                  \
   template void launch_reset_force<T>(const LevelDev<T>*, int, int, hipStream_t);                         \
@@ -3267,7 +3273,7 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_coarsest<T>(const CoarsestDev&, const LevelDev<T>&, int, hipStream_t);           \
   template void launch_coarse_cycle<T>(const LevelDev<T>*, const LevelDev<T>*, int, const CoarsestDev&, int, int, T, \
                                        int, int, \
-                                       hipStream_t);                                                    \
+                                       hipStream_t, int);                                                    \
   template void launch_qc_refresh_all<T>(const LevelDev<T>*, const int2*, int, hipStream_t);            \
   template void launch_mass<T>(const LevelDev<T>&, ResidualOut*, hipStream_t);                          \
   template void launch_fill_value<T>(T*, long long, T, hipStream_t);

@@ -1971,6 +1971,9 @@ struct Seq {
   // the top level's parent parts are current (the previous cycle's residual
   // refreshed them after the filldown, or the solve's first fill stored them)
   bool top_cf_current = false;
+  // the FAS loads' reflux sums start from FTrue (a loop that resets F once,
+  // before its first cycle: see reflux_item)
+  int reflux_ft = 0;
   void flush_ring(int k) {
     if (k >= 2 && k <= (int)ringp.size() && ringp[k - 1]) {
       launch_ring_cf<T>(lev(k), lev(k - 1), s);
@@ -2140,7 +2143,7 @@ struct Seq {
   void vcycle(int k, int kc) {  // ebVcycle.m:15-35, fused
     if (k <= kc && kc >= 2) {
       launch_coarse_cycle<T>(LVd, L.data(), k, h->cst, h->opts.npre, h->opts.npost, (T)h->opts.omega, homog,
-                             h->opts.mass_correction, s);
+                             h->opts.mass_correction, s, reflux_ft);
       return;
     }
     if (k == 1) {
@@ -2179,7 +2182,7 @@ struct Seq {
     const bool pskip = lev(k).pgh_ok && (k - 1 == 1 ? kc <= 1 : (k - 1 > kc && swaps(k - 1)));
     if (!pskip) fill(k - 1);
     launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s, pskip ? 1 : 0, homog,
-                   pskip && k - 1 > 1 ? (const T*)lev(k - 2).Q : nullptr);
+                   pskip && k - 1 > 1 ? (const T*)lev(k - 2).Q : nullptr, reflux_ft);
     unfilled = pskip ? k - 1 : 0;
     vcycle(k - 1, kc);
     const bool prep = h->opts.npost > 0 && lev(k).arith && lev(k).aux && h->opts.smoother != 1;
@@ -2639,11 +2642,14 @@ int bsam_comp_residual(bsam_hier* h, bsam_residual* out) {
   return read_residual(h, out);
 }
 
-// one cycle of the plain FP64 loop: reset F, V-cycle, filldown, residual,
-// residual scalars to pinned host memory
+// one cycle of the plain FP64 loop: V-cycle, filldown, residual, residual
+// scalars to pinned host memory.  The loop's F = FTrue before every cycle
+// (ebSolveCycles.m:30-34) runs once, before the first (bsam_solve_cycles):
+// between cycles F differs from FTrue only where the FAS loads write, and
+// they overwrite covered cells and sum the reflux targets from FTrue.
 static int enqueue_cycle64(bsam_hier* h, hipStream_t s) {
   Seq<double> q = seq64(h, s);
-  enqueue_reset_force<double>(h, h->lv64, s);
+  q.reflux_ft = 1;
   q.top_cf_current = h->opts.mass_correction && h->nlev >= std::max(2, h->root + 1);
   q.vcycle(h->nlev, coarse_cut(h, h->opts.coarse_cells));
   // rings the correcting post-smoothing left out: the filldown's fills
@@ -2776,6 +2782,7 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
   int it = 0;
   while (it < h->opts.max_vcycles && rel > h->opts.rel_tol) {
     ++it;
+    if (it == 1 && !mixed) enqueue_reset_force<double>(h, h->lv64, h->stream);  // see enqueue_cycle64
     if (int rc = run_cycle(h, mixed)) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
     const ResidualOut r = *h->hres;
