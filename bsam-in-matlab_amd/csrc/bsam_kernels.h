@@ -82,12 +82,17 @@ void launch_tag_finish(const uint8_t* raw, const uint8_t* cover, int n1, int n2,
 template <typename T> void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
 template <typename T>
 void launch_ring_cf(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s);
+// C (may be null): F's child, whose QC ring (and, with cf, coarse-fine parent
+// parts) the launch also gathers from F, as the composite residual's refresh
 template <typename T>
 void launch_restrict_fill(const LevelDev<T>& F, const LevelDev<T>& P, const LevelDev<T>* PP, int homog,
-                          hipStream_t s);
-// restrict k -> k-1 + fill k-1 for k = kfrom..2 in one workgroup
+                          hipStream_t s, const LevelDev<T>* C = nullptr, int cf = 0);
+// restrict k -> k-1 + fill k-1 for k = kfrom..2 in one workgroup; step k
+// also gathers level k+1's QC ring from level k when kqc <= k+1 <= ktop, and
+// its parent parts when k+1 == cftop
 template <typename T>
-void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int homog, hipStream_t s);
+void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int homog, hipStream_t s, int kqc = 1 << 30,
+                           int ktop = 0, int cftop = 0);
 // dup (may be null): a second parent array that receives the restricted values too
 template <typename T>
 void launch_restrict(const LevelDev<T>& L, const LevelDev<T>& parent, int qc_only, hipStream_t s, T* dup = nullptr);

@@ -1854,15 +1854,26 @@ __global__ void __launch_bounds__(kTile) k_ring_cf(LevelDev<T> L, const T* __res
 // rest fill P's ghosts (and QC ring), reading P's covered cells as the
 // restriction of F's values (CoveredQ) rather than P's cells the first items
 // write, so nothing in the launch reads what another item writes
+// C, nring, ncf: the composite residual's QC refresh (ebCompResidual.m:23-39)
+// of F's child C, whose parent F is final after the previous step: its ring
+// (nring items) and, for the top level, its coarse-fine parent parts (ncf
+// items).  Nothing else in the launch writes C or reads what these write.
 template <typename T>
 __global__ void __launch_bounds__(kTile) k_restrict_fill(LevelDev<T> F, LevelDev<T> P, const T* __restrict__ PPq,
-                                                         int homog) {
+                                                         int homog, LevelDev<T> C, int nring, int ncf) {
   const int t = blockIdx.x * kTile + threadIdx.x;
   if (t < F.nfoot) {
     restrict_item<T>(F, P.Q, t, 0);
     return;
   }
-  fill_item_via<T>(P, PPq, t - F.nfoot, homog, CoveredQ<T>{P.Q, F.Q, F.pcov});
+  const int nf = fill_items<T>(P, PPq), x = t - F.nfoot;
+  if (x < nf) {
+    fill_item_via<T>(P, PPq, x, homog, CoveredQ<T>{P.Q, F.Q, F.pcov});
+  } else if (x < nf + nring) {
+    ring_item<T>(C, F.Q, x - nf);
+  } else if (x < nf + nring + ncf) {
+    keep_face_cf<T>(C, F.Q, x - nf - nring);
+  }
 }
 
 template <typename T>
@@ -2073,14 +2084,23 @@ __device__ void blk_vcycle(Grp& g, const LevelDev<T>* __restrict__ LV, int kc, c
 // restrict k -> k-1, fill k-1, for k = kfrom..kto (kto >= 2), with barriers
 // between the dependent steps (the small levels' steps would otherwise each
 // cost a launch).
+// kqc..ktop: levels whose QC ring (and, for ktop == cftop, parent parts) the
+// composite residual's refresh needs; step k gathers level k+1's from level
+// k, final after the step before (kqc > ktop: none).
 template <typename T>
 __global__ void __launch_bounds__(1024) k_filldown_small(const LevelDev<T>* __restrict__ LV, int kfrom, int kto,
-                                                          int homog) {
+                                                          int homog, int kqc, int ktop, int cftop) {
   Grp g = Grp::block();
   for (int k = kfrom; k >= kto; --k) {
     const LevelDev<T>& L = LV[k - 1];
     const LevelDev<T>& P = LV[k - 2];
     for (int x = threadIdx.x; x < L.nfoot; x += blockDim.x) restrict_item<T>(L, P.Q, x, 0);
+    if (k + 1 >= kqc && k + 1 <= ktop) {
+      const LevelDev<T>& C = LV[k];
+      for (int x = threadIdx.x; x < C.nring_all; x += blockDim.x) ring_item<T>(C, L.Q, x);
+      if (k + 1 == cftop)
+        for (int x = threadIdx.x; x < C.nghost; x += blockDim.x) keep_face_cf<T>(C, L.Q, x);
+    }
     __syncthreads();
     blk_fill<T>(g, P, k >= 3 ? (const T*)LV[k - 3].Q : (const T*)nullptr, homog);
   }
@@ -2870,11 +2890,12 @@ void launch_ring(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t s)
 
 template <typename T>
 void launch_restrict_fill(const LevelDev<T>& F, const LevelDev<T>& P, const LevelDev<T>* PP, int homog,
-                          hipStream_t s) {
-  const long long n = (long long)F.nfoot + P.nghost + (PP ? P.nring_cf : 0);
+                          hipStream_t s, const LevelDev<T>* C, int cf) {
+  const int nring = C ? C->nring_all : 0, ncf = (C && cf && C->aux) ? C->nghost : 0;
+  const long long n = (long long)F.nfoot + P.nghost + (PP ? P.nring_cf : 0) + nring + ncf;
   if (!n) return;
   hipLaunchKernelGGL(k_restrict_fill<T>, dim3(nblk(n)), dim3(kTile), 0, s, F, P, PP ? (const T*)PP->Q : nullptr,
-                     homog);
+                     homog, C ? *C : F, nring, ncf);
 }
 
 template <typename T>
@@ -2884,10 +2905,11 @@ void launch_ring_cf(const LevelDev<T>& L, const LevelDev<T>& parent, hipStream_t
 }
 
 template <typename T>
-void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int homog, hipStream_t s) {
+void launch_filldown_small(const LevelDev<T>* LV_dev, int kfrom, int kto, int homog, hipStream_t s, int kqc, int ktop,
+                           int cftop) {
   kto = std::max(kto, 2);
   if (kfrom < kto) return;
-  hipLaunchKernelGGL(k_filldown_small<T>, dim3(1), dim3(1024), 0, s, LV_dev, kfrom, kto, homog);
+  hipLaunchKernelGGL(k_filldown_small<T>, dim3(1), dim3(1024), 0, s, LV_dev, kfrom, kto, homog, kqc, ktop, cftop);
 }
 
 template <typename T>
@@ -3259,12 +3281,12 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_ring<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                    \
   template void launch_ring_cf<T>(const LevelDev<T>&, const LevelDev<T>&, hipStream_t);                 \
   template void launch_restrict_fill<T>(const LevelDev<T>&, const LevelDev<T>&, const LevelDev<T>*, int,   \
-                                        hipStream_t);                                                     \
+                                        hipStream_t, const LevelDev<T>*, int);                            \
   template void launch_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, T*);       \
   template void launch_coarse_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t);        \
   template void launch_load<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, int, const T*, int); \
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, T*);        \
-  template void launch_filldown_small<T>(const LevelDev<T>*, int, int, int, hipStream_t);                  \
+  template void launch_filldown_small<T>(const LevelDev<T>*, int, int, int, hipStream_t, int, int, int);   \
   template void launch_reset_force<T>(const LevelDev<T>*, int, int, hipStream_t);                         \
   template void launch_sweep_stream<T>(const LevelDev<T>&, const PatchInfo&, const T*, T*, T, int, int, int,     \
                                        hipStream_t);                                                         \

@@ -2207,17 +2207,32 @@ struct Seq {
   // ebTransfer.m:73-79 for the levels kk = nlev..klo (restrict kk into kk-1,
   // fill kk-1); the steps of the levels with at most kFilldownSmallCells cells
   // run as one single-workgroup launch
-  void filldown(int klo = 2) {
+  //
+  // qc: each step also does the composite residual's QC refresh
+  // (ebCompResidual.m:23-39) of the level above the one it restricts, whose
+  // parent is final after the step before: the rings of levels >= max(2,
+  // root+1) and the top level's coarse-fine parent parts.  qc_done tells
+  // whether every one of them found a step (the residual then skips its own
+  // refresh launch).
+  bool qc_done = false;
+  void filldown(int klo = 2, bool qc = false) {
     const int kb = std::max(filldown_cut(h), klo - 1);
+    const int kqc = std::max(2, h->root + 1), top = h->nlev;
+    // level kqc's ring is gathered by step kqc-1, which must exist
+    qc_done = qc && kqc - 1 >= std::max(klo, 2);
     for (int kk = h->nlev; kk > kb; --kk) {
+      const int kc = kk + 1;
+      const bool ring = qc_done && kc >= kqc && kc <= top;
       if (lev(kk).pcov) {  // the step as one launch
-        launch_restrict_fill<T>(lev(kk), lev(kk - 1), par(kk - 1), homog, s);
+        launch_restrict_fill<T>(lev(kk), lev(kk - 1), par(kk - 1), homog, s, ring ? &lev(kc) : nullptr,
+                                kc == top ? 1 : 0);
         continue;
       }
       restrict_(kk);
       fill(kk - 1);
+      if (ring) qc_done = false;
     }
-    launch_filldown_small<T>(LVd, kb, klo, homog, s);
+    launch_filldown_small<T>(LVd, kb, klo, homog, s, qc_done ? kqc : 1 << 30, top, qc_done ? top : 0);
   }
 };
 
@@ -2234,8 +2249,9 @@ void enqueue_reset_force(bsam_hier* h, const LevelDev<T>* LVd, hipStream_t s) {
 // residual with its reflux chunks, one reduces the parts in a fixed order into
 // dres and the pinned hres; rout (device array of per-level FP32 pointers)
 // receives r when given.
-void enqueue_residual(bsam_hier* h, float* const* rout, hipStream_t s) {
-  if (h->opts.mass_correction) launch_qc_refresh_all<double>(h->lv64, h->qc_items, h->nqc_items, s);
+// qc_fresh: the filldown did the QC refresh (Seq::filldown's qc_done)
+void enqueue_residual(bsam_hier* h, float* const* rout, hipStream_t s, bool qc_fresh = false) {
+  if (h->opts.mass_correction && !qc_fresh) launch_qc_refresh_all<double>(h->lv64, h->qc_items, h->nqc_items, s);
   launch_residual_all<double, float>(h->lv64, h->nlev, h->res_tiles, h->nres_tiles, h->opts.mass_correction,
                                      h->parts, rout, h->root, s);
   launch_reduce(h->parts, std::min(h->nres_tiles, kResidualParts), h->dres, s, h->hres);
@@ -2656,8 +2672,8 @@ static int enqueue_cycle64(bsam_hier* h, hipStream_t s) {
   // refresh those of levels below the top, the residual's QC refresh the top
   // level's when it is an AMR level with flux corrections; else refresh it
   if (h->nlev <= h->root || !h->opts.mass_correction) q.flush_ring(h->nlev);
-  q.filldown();
-  enqueue_residual(h, nullptr, s);
+  q.filldown(2, h->opts.mass_correction != 0);
+  enqueue_residual(h, nullptr, s, q.qc_done);
   return 0;
 }
 
