@@ -157,6 +157,10 @@ void launch_add_correction(const LevelDev<double>& L, const LevelDev<float>& E, 
 template <typename T> void launch_fill_value(T* a, long long n, T v, hipStream_t s);
 // F = FTrue on every level; max_cells: the largest level's cell count
 template <typename T> void launch_reset_force(const LevelDev<T>* LV_dev, int nlev, int max_cells, hipStream_t s);
+// Q = Q0[k], F = FTrue, QC = 0 on every level; max_elems: the largest of any
+// level's nq, ncell and nqc
+template <typename T>
+void launch_reset_levels(const LevelDev<T>* LV_dev, const T* const* Q0_dev, int nlev, int max_elems, hipStream_t s);
 template <typename TD, typename TS> void launch_convert(TD* d, const TS* src, long long n, hipStream_t s);
 void launch_bdf2_rhs(double* FT, const double* f, const double* Uo, const double* Uoo, double dt, int first,
                      long long n, hipStream_t s);

@@ -2454,6 +2454,21 @@ __global__ void __launch_bounds__(kTile) k_reset_force(const LevelDev<T>* __rest
   for (int x = blockIdx.x * kTile + threadIdx.x; x < n; x += gridDim.x * kTile) F[x] = FT[x];
 }
 
+// every level back to its initial state in one launch: Q = Q0, F = FTrue,
+// QC = 0 (blockIdx.y: the level)
+template <typename T>
+__global__ void __launch_bounds__(kTile) k_reset_levels(const LevelDev<T>* __restrict__ LV,
+                                                        const T* const* __restrict__ Q0) {
+  const LevelDev<T>& L = LV[blockIdx.y];
+  const T* __restrict__ q0 = Q0[blockIdx.y];
+  const int n = max(L.nq, max(L.ncell, L.nqc));
+  for (int x = blockIdx.x * kTile + threadIdx.x; x < n; x += gridDim.x * kTile) {
+    if (x < L.nq) L.Q[x] = q0[x];
+    if (x < L.ncell) L.F[x] = L.FT[x];
+    if (x < L.nqc) L.QC[x] = T(0);
+  }
+}
+
 // composite mean: sum of visible u * da (ebCompositeMean.m:4-23)
 template <typename T>
 __global__ void __launch_bounds__(kTile) k_mass(LevelDev<T> L, ResidualOut* __restrict__ part) {
@@ -2992,6 +3007,13 @@ void launch_reset_force(const LevelDev<T>* LV_dev, int nlev, int max_cells, hipS
   hipLaunchKernelGGL(k_reset_force<T>, dim3(gx, nlev), dim3(kTile), 0, s, LV_dev);
 }
 
+template <typename T>
+void launch_reset_levels(const LevelDev<T>* LV_dev, const T* const* Q0_dev, int nlev, int max_elems, hipStream_t s) {
+  if (nlev <= 0 || max_elems <= 0) return;
+  const unsigned gx = std::min(nblk(max_elems), 4096u);
+  hipLaunchKernelGGL(k_reset_levels<T>, dim3(gx, nlev), dim3(kTile), 0, s, LV_dev, Q0_dev);
+}
+
 void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s, ResidualOut* out_host) {
   hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kReduceThreads), 0, s, parts, nparts, out, out_host);
 }
@@ -3288,6 +3310,7 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_correct<T>(const LevelDev<T>&, const LevelDev<T>&, int, hipStream_t, int, T*);        \
   template void launch_filldown_small<T>(const LevelDev<T>*, int, int, int, hipStream_t, int, int, int);   \
   template void launch_reset_force<T>(const LevelDev<T>*, int, int, hipStream_t);                         \
+  template void launch_reset_levels<T>(const LevelDev<T>*, const T* const*, int, int, hipStream_t);       \
   template void launch_sweep_stream<T>(const LevelDev<T>&, const PatchInfo&, const T*, T*, T, int, int, int,     \
                                        hipStream_t);                                                         \
   template void launch_relax_sweep<T>(const LevelDev<T>&, const LevelDev<T>*, const T*, T*, int, int, T, int,   \

@@ -164,6 +164,7 @@ struct bsam_hier {
   ResidualOut* dres = nullptr;
   ResidualOut* hres = nullptr;             // pinned; the reduction writes it
   std::vector<double*> Q0;                 // uploaded initial guess per level
+  double** q0dev = nullptr;                // the same pointers, on the device (bsam_hier_reset)
   double* Uo = nullptr;                    // parabolic history (packed cells, all levels)
   double* Uoo = nullptr;
   bool have_uoo = false;                   // Uoo holds u^{n-1} (after one step or an upload)
@@ -2450,6 +2451,7 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
   h->Q0.resize(h->nlev);
   for (int k = 0; k < h->nlev; ++k)
     if ((rc = upload(h, &h->Q0[k], h->H[k].Q))) return rc;
+  if ((rc = upload(h, &h->q0dev, h->Q0))) return rc;
   h->part_off.resize(h->nlev + 1);
   h->nparts = 0;
   std::vector<int2> res_tiles, qc_items;
@@ -3200,12 +3202,9 @@ int bsam_export(const bsam_hier* h, int k, int what, long long* out) {
 
 int bsam_hier_reset(bsam_hier* h) {
   if (int rc_ = need_dev(h)) return rc_;
-  for (int k = 0; k < h->nlev; ++k) {
-    LevelDev<double>& D = h->d64[k];
-    HIPCHK(hipMemcpyAsync(D.Q, h->Q0[k], (size_t)D.nq * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    if (D.ncell) HIPCHK(hipMemcpyAsync(D.F, D.FT, (size_t)D.ncell * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-    launch_fill_value<double>(D.QC, D.nqc, 0.0, h->stream);
-  }
+  int mx = 0;
+  for (const LevelDev<double>& D : h->d64) mx = std::max(mx, std::max(D.nq, std::max(D.ncell, D.nqc)));
+  launch_reset_levels<double>(h->lv64, (const double* const*)h->q0dev, h->nlev, mx, h->stream);
   return sync(h);
 }
 
