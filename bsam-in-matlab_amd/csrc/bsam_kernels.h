@@ -48,7 +48,7 @@ void launch_sweep_stream(const LevelDev<T>& L, const PatchInfo& P, const T* Qin,
 template <typename T>
 void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, int frame, T omega, int homog,
                          hipStream_t s, T* Qout = nullptr, int ghosts = 0, T* Rq = nullptr, T* Rdup = nullptr,
-                         int ring = 0, int corr = 0);
+                         int ring = 0, int corr = 0, const LevelDev<T>* parent_dev = nullptr, int fload = 0);
 template <typename T>
 void launch_fill_copy(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, const T* Qsrc, hipStream_t s);
 // the fill after a region relax (parent parts from aux, no QC ring refresh)

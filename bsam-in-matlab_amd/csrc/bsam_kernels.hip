@@ -962,10 +962,16 @@ __host__ __device__ inline size_t region_lds_bytes(int rslots, int rghmax, int r
 // applied to every loaded cell first, and the parent parts of coarse-fine
 // ghosts come from the parent Pq, staged per patch window (L.rwin), in place
 // of the parent parts a preparing correction launch would have stored.
-template <typename T, int NT, int K, int KL, bool P2, bool W1>
+// fload (a level of one tile whose launch writes its ghosts and the
+// restriction): then the FAS load into the parent Pd (ebCoarseLoad), by the
+// same workgroup after a barrier, from the level's new state in Qout; bits: 1
+// load, 2 mass correction (reflux), 4 the parent's physical ghosts evaluated
+// from its interior, 8 reflux sums from FTrue (see load_item).
+template <typename T, int NT, int K, int KL, bool P2, bool W1, bool FL = false>
 __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __restrict__ Pq, T* __restrict__ Qout,
                                                      int npass, int frame, T omega, int homog, int per, int ghosts,
-                                                     T* __restrict__ Rq, T* __restrict__ Rdup, int ring, int corr) {
+                                                     T* __restrict__ Rq, T* __restrict__ Rdup, int ring, int corr,
+                                                     const LevelDev<T>* __restrict__ Pd, int fload) {
   extern __shared__ __align__(16) unsigned char region_lds[];
   T* cv = reinterpret_cast<T*>(region_lds);
   __shared__ RegionPatch sp[kRegionMaxPatches];
@@ -1162,6 +1168,15 @@ __global__ void __launch_bounds__(NT) k_relax_region(LevelDev<T> L, const T* __r
       const RingRec q = L.ring_cf[t * rring + x];
       L.QC[q.dst] = Pq[q.src];
     }
+  if (FL && fload) {  // the workgroup's stores above are visible to it after the barrier
+    __syncthreads();
+    LevelDev<T> Lc = L;
+    Lc.Q = Qout;
+    const LevelDev<T>& P = *Pd;
+    const int mc = (fload & 2) ? 1 : 0;
+    const int n = load_items<T>(Lc, P, mc);
+    for (int x = tid; x < n; x += NT) load_item<T>(Lc, P, x, mc, (fload & 4) ? 1 : 0, homog, (fload & 8) ? 1 : 0);
+  }
 }
 
 // ebFillGhosts after a region relax: the ghosts from the new interior values
@@ -2841,7 +2856,8 @@ void launch_fill(const LevelDev<T>& L, const LevelDev<T>* parent, int homog, hip
 
 template <typename T>
 void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int nsweeps, int frame, T omega,
-                         int homog, hipStream_t s, T* Qout, int ghosts, T* Rq, T* Rdup, int ring, int corr) {
+                         int homog, hipStream_t s, T* Qout, int ghosts, T* Rq, T* Rdup, int ring, int corr,
+                         const LevelDev<T>* parent_dev, int fload) {
   if (!L.nregion || nsweeps <= 0) return;
   const int per = (L.nregion + kXcds - 1) / kXcds;
   const int rg = (ring && parent) ? 1 : 0;
@@ -2853,21 +2869,33 @@ void launch_relax_region(const LevelDev<T>& L, const LevelDev<T>* parent, int ns
   // update, the same numbers as the general one)
   static bool raised = false;  // the canvas plus staged records can pass 64 KB (gfx950: 160 KB per CU)
   if (!raised) {
-    const void* forms[4] = {(const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, true>,
+    const void* forms[8] = {(const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, true>,
                             (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, false>,
                             (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, true>,
-                            (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, false>};
+                            (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, false>,
+                            (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, true, true>,
+                            (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, false, true>,
+                            (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, true, true>,
+                            (const void*)k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, false, true>};
     for (const void* f : forms)
       (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kStageLdsLimit);
     raised = true;
   }
+  const int fl = (parent_dev && L.nregion == 1) ? fload : 0;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3(per * kXcds), dim3(kRegionThreads), lds, s, L,
                        parent ? (const T*)parent->Q : (const T*)nullptr, Qout ? Qout : L.Qs, 2 * nsweeps, frame,
-                       omega, homog, per, ghosts, Rq, Rdup, (ring && parent) ? 1 : 0, cr);
+                       omega, homog, per, ghosts, Rq, Rdup, (ring && parent) ? 1 : 0, cr, parent_dev, fl);
   };
+  // the fused load is a form of its own, so the other launches run the
+  // kernel without its code
   const bool w1 = omega == T(1);
-  if (L.pow2 && w1) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, true>);
+  if (fl) {
+    if (L.pow2 && w1) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, true, true>);
+    else if (L.pow2) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, false, true>);
+    else if (w1) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, true, true>);
+    else go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, false, true>);
+  } else if (L.pow2 && w1) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, true>);
   else if (L.pow2) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, true, false>);
   else if (w1) go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, true>);
   else go(k_relax_region<T, kRegionThreads, kRegionK, kRegionKL, false, false>);
@@ -3295,7 +3323,7 @@ void launch_polygon_fields(const double* seg5, int ns, const double* x, const do
   template void launch_relax_tiled<T>(const LevelDev<T>&, const LevelDev<T>*, int, T, int, hipStream_t); \
   template void launch_fill<T>(const LevelDev<T>&, const LevelDev<T>*, int, hipStream_t);               \
   template void launch_relax_region<T>(const LevelDev<T>&, const LevelDev<T>*, int, int, T, int, hipStream_t, T*, int, \
-                                       T*, T*, int, int);                                                   \
+                                       T*, T*, int, int, const LevelDev<T>*, int);                          \
   template void launch_fill_copy<T>(const LevelDev<T>&, const LevelDev<T>*, int, const T*, hipStream_t); \
   template void launch_fill_copy_restrict<T>(const LevelDev<T>&, const LevelDev<T>&, int, const T*, T*,        \
                                              hipStream_t);                                                  \

@@ -2016,8 +2016,10 @@ struct Seq {
   // ring refresh, which the FAS load's gather redoes
   // corr: the launch applies the pending coarse-grid correction of level k
   // first (swap 1 only, levels with L.rcorr)
+  // fload: k_relax_region's fused FAS load into the parent (a level of one
+  // tile smoothing into its other buffer with the restriction)
   void relax(int k, int nsweeps, int first_inline = 0, bool cf_current = false, bool defer_fill = false,
-             int swap = 0, T* rdup = nullptr, bool corr = false) {
+             int swap = 0, T* rdup = nullptr, bool corr = false, int fload = 0) {
     const bool qs_current = qs_ready == k;
     qs_ready = 0;
     if (region(k, nsweeps)) {
@@ -2031,7 +2033,8 @@ struct Seq {
         T* rq = swap == 2 ? cur[k - 2].Q : nullptr;
         const bool cr = corr && swap == 1;
         launch_relax_region<T>(Lk, par(k), nsweeps, first_inline == 0 ? 1 : 0, (T)h->opts.omega, homog, s, Lk.Qs, 1,
-                               rq, swap == 2 ? rdup : nullptr, swap == 1 && !cr ? 1 : 0, cr ? 1 : 0);
+                               rq, swap == 2 ? rdup : nullptr, swap == 1 && !cr ? 1 : 0, cr ? 1 : 0,
+                               fload ? LVd + (k - 2) : nullptr, fload);
         if (cr) {
           if ((int)ringp.size() < h->nlev) ringp.assign((size_t)h->nlev, 0);
           ringp[(size_t)k - 1] = 1;
@@ -2164,7 +2167,19 @@ struct Seq {
     // pre-smoothing from assembled ghosts (what that fill would have written)
     const bool first_assembled = unfilled == k;
     unfilled = 0;
-    relax(k, h->opts.npre, first_assembled ? 2 : 0, k < h->nlev || top_cf_current, fuse, sw ? 2 : 0, dup);
+    // the parent's fill after the restriction is left out when none of its
+    // readers needs it: the FAS load evaluates the parent's physical ghosts
+    // from the interior (pgh_ok: every parent ghost it reads is physical), and
+    // the parent's pre-smoothing is a buffer-swapping region launch that
+    // assembles its first pass's ghosts, or the parent is the coarsest level,
+    // whose solve reads no ghost (it is filled after the solve)
+    const bool pskip = lev(k).pgh_ok && (k - 1 == 1 ? kc <= 1 : (k - 1 > kc && swaps(k - 1)));
+    // a one-tile level above the coarsest does the FAS load in its
+    // pre-smoothing launch (the coarsest level's descriptor never swaps)
+    const bool fl = sw && k == 2 && pskip && !dup && lev(k).nregion == 1 && region(k, h->opts.npre);
+    const int fload = fl ? 1 | (h->opts.mass_correction ? 2 : 0) | 4 | (reflux_ft ? 8 : 0) : 0;
+    relax(k, h->opts.npre, first_assembled ? 2 : 0, k < h->nlev || top_cf_current, fuse, sw ? 2 : 0, dup, false,
+          fload);
     if (dup) qs_ready = k - 1;
     if (sw) {
     } else if (fuse) {
@@ -2174,16 +2189,10 @@ struct Seq {
     } else {
       restrict_(k);
     }
-    // the parent's fill after the restriction is left out when none of its
-    // readers needs it: the FAS load evaluates the parent's physical ghosts
-    // from the interior (pgh_ok: every parent ghost it reads is physical), and
-    // the parent's pre-smoothing is a buffer-swapping region launch that
-    // assembles its first pass's ghosts, or the parent is the coarsest level,
-    // whose solve reads no ghost (it is filled after the solve)
-    const bool pskip = lev(k).pgh_ok && (k - 1 == 1 ? kc <= 1 : (k - 1 > kc && swaps(k - 1)));
     if (!pskip) fill(k - 1);
-    launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s, pskip ? 1 : 0, homog,
-                   pskip && k - 1 > 1 ? (const T*)lev(k - 2).Q : nullptr, reflux_ft);
+    if (!fl)
+      launch_load<T>(lev(k), lev(k - 1), h->opts.mass_correction, s, pskip ? 1 : 0, homog,
+                     pskip && k - 1 > 1 ? (const T*)lev(k - 2).Q : nullptr, reflux_ft);
     unfilled = pskip ? k - 1 : 0;
     vcycle(k - 1, kc);
     const bool prep = h->opts.npost > 0 && lev(k).arith && lev(k).aux && h->opts.smoother != 1;
