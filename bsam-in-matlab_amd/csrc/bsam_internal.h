@@ -255,10 +255,22 @@ struct LevelDev {
   FaceAux<T>* aux;   // per ghost record (null: parent parts not kept)
 };
 
+// A level-1 ghost record whose source cells are all rows of one 64-row block
+// of the coarsest solve: rows[0..1] the record's sources (a Neumann record
+// repeats rows[0]); for a physical corner rows[0..1] and [2..3] those of its
+// two side records and rows[4] the diagonal cell; else rows[2..4] = rows[0]
+struct CoarsestGhost {
+  int rec;
+  int rows[5];
+};
 struct CoarsestDev {
   int N, M;          // unknowns, matrix order (N or N+1 with the KKT row)
   double* Ainv;      // M x M column-major
   double* bcRhs;     // N
+  // level 1's ghost records by 64-row block (gofs: CSR offsets), evaluated
+  // by the block that computes their sources; null: a fill launch follows
+  CoarsestGhost* gtab;
+  int* gofs;
 };
 
 // scalars of the composite residual

@@ -1932,10 +1932,10 @@ __global__ void __launch_bounds__(kTile) k_correct(LevelDev<T> L, const T* __res
 
 // ---------------------------------------------------------------- coarsest
 // u = Ainv (F + bcRhs), rows accumulated in ascending column order; the
-// caller has staged rhs in LDS
+// caller has staged rhs in LDS.  Returns the value stored.
 template <typename T>
-__device__ __forceinline__ void coarsest_row(const CoarsestDev& C, const double* rhs, int i, int n1,
-                                             T* __restrict__ Q) {
+__device__ __forceinline__ T coarsest_row(const CoarsestDev& C, const double* rhs, int i, int n1,
+                                          T* __restrict__ Q) {
   // ascending-j accumulation as in the oracle; the loads of 16 columns are
   // issued before their products are added so that they overlap
   const double* __restrict__ a = C.Ainv + i;
@@ -1952,16 +1952,46 @@ __device__ __forceinline__ void coarsest_row(const CoarsestDev& C, const double*
   for (; j < C.M; ++j) acc = acc + a[(size_t)j * M] * rhs[j];
   const int ci = i % n1, cj = i / n1;
   Q[(ci + 1) + (cj + 1) * (n1 + 2)] = (T)acc;
+  return (T)acc;
 }
 
+// one wave per block of the coarsest apply
+constexpr int kCoarsestThreads = 64;
+
+// With C.gtab, the block (one wave) also writes the level-1 ghost records
+// whose sources are its rows, as the fill after the solve would
+// (ebFillGhosts): the rows' values come from the lanes that computed them
+// (shuffles every lane takes part in), the expressions are region_ghost's and
+// fill_item's corner rule.
 template <typename T>
-__global__ void __launch_bounds__(64) k_coarsest(CoarsestDev C, LevelDev<T> L1, int homog) {
+__global__ void __launch_bounds__(kCoarsestThreads) k_coarsest(CoarsestDev C, LevelDev<T> L1, int homog) {
   extern __shared__ double rhs[];
+  const int g0 = C.gtab ? C.gofs[blockIdx.x] : 0, g1 = C.gtab ? C.gofs[blockIdx.x + 1] : 0;
   for (int j = threadIdx.x; j < C.M; j += blockDim.x)
     rhs[j] = j < C.N ? (homog ? (double)L1.F[j] : (double)L1.F[j] + C.bcRhs[j]) : 0.0;
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < C.N) coarsest_row<T>(C, rhs, i, L1.n[0], L1.Q);
+  T qv = T(0);
+  if (i < C.N) qv = coarsest_row<T>(C, rhs, i, L1.n[0], L1.Q);
+  const int base = blockIdx.x * blockDim.x;
+  for (int e0 = g0; e0 < g1; e0 += blockDim.x) {  // uniform in the block
+    const int lane = threadIdx.x, e = min(e0 + lane, g1 - 1);
+    const CoarsestGhost G = C.gtab[e];
+    T v[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = __shfl(qv, G.rows[k] - base, 64);
+    const GhostRec r = L1.ghost[G.rec];
+    T val;
+    if (r.kind == G_PCORNER) {
+      const GhostRec A = L1.ghost[r.s[0]], B = L1.ghost[r.s[1]];
+      const T a = region_ghost<T>(A.kind, T(A.c), T(0), v[0], v[1], homog);
+      const T b = region_ghost<T>(B.kind, T(B.c), T(0), v[2], v[3], homog);
+      val = (a + b) - v[4];
+    } else {
+      val = region_ghost<T>(r.kind, T(r.c), T(0), v[0], v[1], homog);
+    }
+    if (e0 + lane < g1) L1.Q[r.dst] = val;
+  }
 }
 
 // ---------------------------------------------------------------- groups
@@ -2987,8 +3017,8 @@ void launch_correct(const LevelDev<T>& L, const LevelDev<T>& parent, int init, h
 
 template <typename T>
 void launch_coarsest(const CoarsestDev& C, const LevelDev<T>& L1, int homog, hipStream_t s) {
-  const unsigned nb = (unsigned)((C.N + 63) / 64);
-  hipLaunchKernelGGL(k_coarsest<T>, dim3(nb), dim3(64), C.M * sizeof(double), s, C, L1, homog);
+  const unsigned nb = (unsigned)((C.N + kCoarsestThreads - 1) / kCoarsestThreads);
+  hipLaunchKernelGGL(k_coarsest<T>, dim3(nb), dim3(kCoarsestThreads), C.M * sizeof(double), s, C, L1, homog);
 }
 
 template <typename T>

@@ -2127,7 +2127,7 @@ struct Seq {
   void correct(int k, bool prep = false) { launch_correct<T>(lev(k), lev(k - 1), 0, s, prep ? 1 : 0); }
   void coarsest() {
     launch_coarsest<T>(h->cst, lev(1), homog, s);
-    fill(1);
+    if (!h->cst.gtab) fill(1);  // else the solve's blocks wrote the ghosts
   }
   void vcycle_ref(int k) {  // ebVcycle.m:15-35, one launch per reference step
     if (k == 1) {
@@ -2345,6 +2345,8 @@ int bsam_ctx_create(int hip_device, bsam_ctx** out) {
 
 void bsam_ctx_destroy(bsam_ctx* ctx) { delete ctx; }
 
+int build_coarsest_ghosts(bsam_hier* h);
+
 static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_solve_opts* o, bsam_hier* h) {
   if (!d || d->nlev < 1 || d->root_idx < 1 || d->root_idx > d->nlev || !d->n || !d->np || !d->boxes ||
       !d->bc_code || !d->Q || !d->FTrue || !d->DeW || !d->DnS || !d->CC || !d->InvDiagS || !d->g)
@@ -2453,6 +2455,7 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
   }
   if ((rc = upload(h, &h->cst.Ainv, Ainv))) return rc;
   if ((rc = upload(h, &h->cst.bcRhs, bcRhs))) return rc;
+  if ((rc = build_coarsest_ghosts(h))) return rc;
   // device levels
   h->d64.resize(h->nlev);
   for (int k = 0; k < h->nlev; ++k)
@@ -3207,6 +3210,58 @@ int bsam_export(const bsam_hier* h, int k, int what, long long* out) {
     }
   }
   return 0;
+}
+
+// Level 1's ghost records for k_coarsest's blocks (CoarsestGhost): every
+// record must be physical (Dirichlet, Neumann, or a corner of two), with all
+// its source cells among the rows of one 64-row block of the solve; else no
+// table is built and the fill after the solve stays a launch of its own.
+int build_coarsest_ghosts(bsam_hier* h) {
+  h->cst.gtab = nullptr;
+  h->cst.gofs = nullptr;
+  const HLevel& L = h->H[0];
+  if (L.np != 1 || L.ghost.empty()) return 0;
+  const int n1 = L.n[0], n2 = L.n[1], ldq = n1 + 2, N = h->cst.N, B = 64;
+  const int nb = (N + B - 1) / B;
+  auto row = [&](int q) {
+    const int ci = q % ldq - 1, cj = q / ldq - 1;
+    return (q >= 0 && ci >= 0 && ci < n1 && cj >= 0 && cj < n2) ? ci + cj * n1 : -1;
+  };
+  auto side = [&](const GhostRec& s, int& a, int& b) {
+    if (s.kind == G_DIR) { a = row(s.s[0]); b = row(s.s[1]); return a >= 0 && b >= 0; }
+    if (s.kind == G_NEU) { a = row(s.s[0]); b = a; return a >= 0; }
+    return false;
+  };
+  std::vector<std::vector<CoarsestGhost>> per((size_t)nb);
+  const int ng = (int)L.ghost.size();
+  for (int g = 0; g < ng; ++g) {
+    const GhostRec& r = L.ghost[(size_t)g];
+    CoarsestGhost e{};
+    e.rec = g;
+    int* w = e.rows;
+    if (r.kind == G_PCORNER) {
+      if (r.s[0] < 0 || r.s[0] >= ng || r.s[1] < 0 || r.s[1] >= ng) return 0;
+      if (!side(L.ghost[(size_t)r.s[0]], w[0], w[1]) || !side(L.ghost[(size_t)r.s[1]], w[2], w[3])) return 0;
+      w[4] = row(r.s[2]);
+      if (w[4] < 0) return 0;
+    } else {
+      if (!side(r, w[0], w[1])) return 0;
+      w[2] = w[3] = w[4] = w[0];
+    }
+    const int b = w[0] / B;
+    for (int k = 0; k < 5; ++k)
+      if (w[k] / B != b) return 0;
+    per[(size_t)b].push_back(e);
+  }
+  std::vector<CoarsestGhost> tab;
+  std::vector<int> ofs{0};
+  for (const auto& v : per) {
+    tab.insert(tab.end(), v.begin(), v.end());
+    ofs.push_back((int)tab.size());
+  }
+  int rc;
+  if ((rc = upload(h, &h->cst.gtab, tab))) return rc;
+  return upload(h, &h->cst.gofs, ofs);
 }
 
 int bsam_hier_reset(bsam_hier* h) {
