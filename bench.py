@@ -642,6 +642,7 @@ def main():
                      "sweeps_per_launch": sweeps,
                      # the algorithmic bytes of the sweeps a launch performs, per second
                      "sweep_bytes_rate_GBps": round(sweeps * gbs64 if region else gbs64, 1),
+                     "sweep_bytes_rate_frac": round((sweeps * gbs64 if region else gbs64) / HBM_PEAK_GBS, 4),
                      "colour_pass_us": round(ms_pass * 1e3, 3),
                      "relax_call_us": {"launch_form": round(ms64 * 1e3 * (1 if region else 2 * opts.NPre), 2),
                                        "colour_passes": round(relax_call_pass_us, 2)}},
