@@ -149,9 +149,11 @@ constexpr int kResidualParts = 2048;
 template <typename T, typename TO>
 void launch_residual_all(const LevelDev<T>* LV_dev, int nlev, const int2* tl, int ntl, int mass_corr,
                          ResidualOut* parts, TO* const* rout, int k0, hipStream_t s);
-// out_host (pinned host memory), when given, receives the result as well
+// out_host (pinned host memory), when given, receives the result as well;
+// seq: a device counter the launch increments and writes into
+// out_host->pad2[0] after the result (the host polls it)
 void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s,
-                   ResidualOut* out_host = nullptr);
+                   ResidualOut* out_host = nullptr, unsigned long long* seq = nullptr);
 template <typename T> void launch_mass(const LevelDev<T>& L, ResidualOut* parts, hipStream_t s);
 void launch_add_correction(const LevelDev<double>& L, const LevelDev<float>& E, hipStream_t s);
 template <typename T> void launch_fill_value(T* a, long long n, T v, hipStream_t s);

@@ -2440,9 +2440,13 @@ __global__ void __launch_bounds__(kTile) k_residual_all(const LevelDev<T>* __res
 // time, then waves and the workgroup in a fixed order: the result is the
 // same on every run.
 constexpr int kReduceThreads = 1024;
+// seq (with out_host): a device counter, incremented per launch and written
+// into out_host->pad2[0] after the scalars (system-scope fence between), so
+// that the host can wait for the result by polling the pinned buffer
 __global__ void __launch_bounds__(kReduceThreads) k_reduce_parts(const ResidualOut* __restrict__ parts,
                                                                  int nparts, ResidualOut* __restrict__ out,
-                                                                 ResidualOut* __restrict__ out_host) {
+                                                                 ResidualOut* __restrict__ out_host,
+                                                                 unsigned long long* __restrict__ seq) {
   double a = 0.0, b = 0.0, f = 0.0;
   long long n = 0;
   for (int k0 = threadIdx.x; k0 < nparts; k0 += 8 * kReduceThreads) {
@@ -2485,7 +2489,18 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_parts(const ResidualO
     r.pad = 0.0;
     r.pad2[0] = r.pad2[1] = r.pad2[2] = 0;
     *out = r;
-    if (out_host) *out_host = r;
+    if (out_host) {  // the scalars, then (seq) the flag; pad2 is the flag's word
+      out_host->linf = r.linf;
+      out_host->l2sq = r.l2sq;
+      out_host->fnorm = r.fnorm;
+      out_host->ncell = r.ncell;
+      if (seq) {
+        const unsigned long long v = *seq + 1ULL;
+        *seq = v;
+        __threadfence_system();
+        *(volatile long long*)&out_host->pad2[0] = (long long)v;
+      }
+    }
   }
 }
 
@@ -3072,8 +3087,9 @@ void launch_reset_levels(const LevelDev<T>* LV_dev, const T* const* Q0_dev, int 
   hipLaunchKernelGGL(k_reset_levels<T>, dim3(gx, nlev), dim3(kTile), 0, s, LV_dev, Q0_dev);
 }
 
-void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s, ResidualOut* out_host) {
-  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kReduceThreads), 0, s, parts, nparts, out, out_host);
+void launch_reduce(const ResidualOut* parts, int nparts, ResidualOut* out, hipStream_t s, ResidualOut* out_host,
+                   unsigned long long* seq) {
+  hipLaunchKernelGGL(k_reduce_parts, dim3(1), dim3(kReduceThreads), 0, s, parts, nparts, out, out_host, seq);
 }
 
 template <typename T>

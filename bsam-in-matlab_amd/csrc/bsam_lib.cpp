@@ -11,6 +11,8 @@
 // replayed per cycle.
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -163,6 +165,10 @@ struct bsam_hier {
   long long nq32all = 0, nqc32all = 0;
   ResidualOut* dres = nullptr;
   ResidualOut* hres = nullptr;             // pinned; the reduction writes it
+  // the reductions' counter on the device and the count the host expects
+  // in hres->pad2[0] after the last one it enqueued (wait_residual)
+  unsigned long long* dseq = nullptr;
+  long long seq_expect = 0;
   std::vector<double*> Q0;                 // uploaded initial guess per level
   double** q0dev = nullptr;                // the same pointers, on the device (bsam_hier_reset)
   double* Uo = nullptr;                    // parabolic history (packed cells, all levels)
@@ -2264,7 +2270,7 @@ void enqueue_residual(bsam_hier* h, float* const* rout, hipStream_t s, bool qc_f
   if (h->opts.mass_correction && !qc_fresh) launch_qc_refresh_all<double>(h->lv64, h->qc_items, h->nqc_items, s);
   launch_residual_all<double, float>(h->lv64, h->nlev, h->res_tiles, h->nres_tiles, h->opts.mass_correction,
                                      h->parts, rout, h->root, s);
-  launch_reduce(h->parts, std::min(h->nres_tiles, kResidualParts), h->dres, s, h->hres);
+  launch_reduce(h->parts, std::min(h->nres_tiles, kResidualParts), h->dres, s, h->hres, h->dseq);
 }
 
 }  // namespace
@@ -2492,6 +2498,8 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
   if ((rc = upload(h, &h->lv64, h->d64))) return rc;
   if ((rc = h->mem.alloc(&h->parts, std::max(1, h->nparts)))) return rc;
   if ((rc = h->mem.alloc(&h->dres, 1))) return rc;
+  if ((rc = h->mem.alloc(&h->dseq, 1))) return rc;
+  HIPCHK(hipMemset(h->dseq, 0, sizeof(unsigned long long)));
   HIPCHK(hipHostMalloc((void**)&h->hres, sizeof(ResidualOut), hipHostMallocDefault));
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   h->cell_base.resize(h->nlev + 1);
@@ -2652,9 +2660,32 @@ static double residual_rel(const ResidualOut& r) {
   return r.linf / std::max(r.fnorm, 2.2250738585072014e-308);
 }
 
+// the scalars of the reduction the host enqueued last (one more than the
+// count so far): polls the flag the reduction writes into the pinned hres
+// after them, which returns as soon as the result lands rather than when the
+// runtime reports the stream idle; after kWaitSpinNs without it, waits for
+// the stream (which also reports a failed launch)
+constexpr long long kWaitSpinNs = 20000000;
+static int wait_residual(bsam_hier* h) {
+  const long long want = ++h->seq_expect;
+  volatile long long* flag = &h->hres->pad2[0];
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned n = 1; *flag != want; ++n) {
+    if ((n & 1023u) == 0 &&
+        std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count() >
+            kWaitSpinNs) {
+      HIPCHK(hipStreamSynchronize(h->stream));
+      break;
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return 0;
+}
+
 // after enqueue_residual, whose reduction wrote the scalars to hres
 static int read_residual(bsam_hier* h, bsam_residual* out) {
   HIPCHK(hipStreamSynchronize(h->stream));
+  ++h->seq_expect;  // the reduction it waited for
   if (out) {
     out->linf = h->hres->linf;
     out->l2 = std::sqrt(h->hres->l2sq);
@@ -2814,7 +2845,7 @@ int bsam_solve_cycles(bsam_hier* h, const bsam_solve_opts* o, bsam_stage_stats* 
     ++it;
     if (it == 1 && !mixed) enqueue_reset_force<double>(h, h->lv64, h->stream);  // see enqueue_cycle64
     if (int rc = run_cycle(h, mixed)) return rc;
-    HIPCHK(hipStreamSynchronize(h->stream));
+    if (int rc = wait_residual(h)) return rc;
     const ResidualOut r = *h->hres;
     hist.push_back(r.linf);
     rel = residual_rel(r);
