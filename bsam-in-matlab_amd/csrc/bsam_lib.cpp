@@ -2501,6 +2501,9 @@ static int hier_create_impl(bsam_ctx* ctx, const bsam_hier_desc* d, const bsam_s
   if ((rc = h->mem.alloc(&h->dseq, 1))) return rc;
   HIPCHK(hipMemset(h->dseq, 0, sizeof(unsigned long long)));
   HIPCHK(hipHostMalloc((void**)&h->hres, sizeof(ResidualOut), hipHostMallocDefault));
+  // a recycled pinned block can hold an earlier hierarchy's flag word, which
+  // the first wait_residual would take for this one's first result
+  std::memset(h->hres, 0, sizeof(ResidualOut));
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   h->cell_base.resize(h->nlev + 1);
   h->ncell_all = 0;
@@ -2684,8 +2687,7 @@ static int wait_residual(bsam_hier* h) {
 
 // after enqueue_residual, whose reduction wrote the scalars to hres
 static int read_residual(bsam_hier* h, bsam_residual* out) {
-  HIPCHK(hipStreamSynchronize(h->stream));
-  ++h->seq_expect;  // the reduction it waited for
+  if (int rc = wait_residual(h)) return rc;
   if (out) {
     out->linf = h->hres->linf;
     out->l2 = std::sqrt(h->hres->l2sq);
