@@ -170,6 +170,31 @@ def test_tiled_and_mixed_reach_the_same_solution(name, precision):
         assert err < 1e-10
 
 
+def test_first_residual_after_hierarchies_were_freed():
+    """The host waits for a residual by polling a sequence number that the
+    reduction writes into the hierarchy's pinned result after the scalars.  A
+    freed hierarchy's pinned block can be handed to the next one still holding
+    its last number (1 after a single residual); the next hierarchy's first
+    read must still return its own residual, not the scalars left there."""
+    for seed in (21, 22, 23):
+        old = GpuHierarchy(CASES["lshape"](seed=seed))
+        for k in range(1, old.nlev + 1):
+            old.fill_ghosts(k)
+        old.comp_residual()
+        old.close()
+        H = CASES["lshape"](seed=seed + 10)
+        Orc = O.OracleHierarchy(H)
+        G = GpuHierarchy(H)
+        for k in range(1, Orc.nlev + 1):
+            Orc.fill_ghosts(k)
+            G.fill_ghosts(k)
+        ro, rg = Orc.comp_residual(), G.comp_residual()
+        assert rg["linf"] == ro["linf"] and rg["fnorm"] == ro["fnorm"] and rg["ncell"] == ro["ncell"], seed
+        st = G.solve_cycles(max_vcycles=2)  # its baseline residual is the one above
+        assert st["resHist"][0] == ro["linf"] and 1 <= st["vcycles"] <= 2
+        G.close()
+
+
 def test_relax_omega_bit_exact():
     H = CASES["lshape"](seed=5)
     H.opts = H.opts.replace(Omega=1.3)
