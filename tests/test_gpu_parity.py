@@ -153,7 +153,7 @@ def test_tiled_and_mixed_reach_the_same_solution(name, precision):
     """The LDS-tiled smoother and the FP32-inner defect-correction form change
     the iteration, not the fixed point: converged to the round-off floor
     (rel 1e-11, or the oracle's own floor) they agree with the oracle's FP64
-    solution to 1e-10 relative L2."""
+    solution to 1e-10 relative L2 (the bound stated for FP64 results)."""
     H = CASES[name](seed=None)
     H.opts = H.opts.replace(RelTol=1e-11, MaxVCycles=40)
     Orc = O.OracleHierarchy(H)

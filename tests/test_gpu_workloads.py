@@ -92,7 +92,7 @@ def test_starfish_full_size_bit_exact(starfish):
         got = G.get_field(k, _lib.BSAM_Q)
         for p in range(H.L(k).np):
             assert np.array_equal(got[p], Orc.L(k).Q[p]), ("sweep", k, p)
-    # FP32-inner form: same fixed point (north-star bound 1e-10 rel L2, visible cells)
+    # FP32-inner form: same fixed point (1e-10 rel L2 on visible cells, the FP64 bound)
     G.reset()
     sm = G.solve_cycles(precision="mixed")
     num = den = 0.0

@@ -36,7 +36,7 @@ def test_problem_validation():
 
 
 def test_ladder_sizes():
-    """ebSetup.m:17-25 (matches the rootIdx table of the survey)."""
+    """ebSetup.m:17-25: the coarsening ladder below each base size."""
     for base, ladder in ((32, [16, 32]), (64, [16, 32, 64]), (128, [16, 32, 64, 128]), (48, [12, 24, 48]),
                          (33, [33])):
         got = [int(s[0]) for s in hierarchy.ladder_sizes(Options(BaseCells=base))]
