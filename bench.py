@@ -433,7 +433,7 @@ def main():
                     help="launch a plain device copy of MIB MiB 20 times (scale for the PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each box has one MI355X); "
+        print("bench.py measures one GPU: the library has no multi-GPU path; "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     # host threads for the CPU baseline: the process's CPU share (16 per GPU

@@ -1,12 +1,12 @@
 """Device path on the benchmark workloads at their real sizes.
 
-  README elliptic (config 1) through eb_solve: the published stage log.
-  Variable-D 256^2 (config 2): bit-exact against the oracle, and the finding
+  README elliptic (base 32, 3 levels) through eb_solve: the published stage log.
+  Variable-D 256^2: bit-exact against the oracle, and the finding
   that ulap at 1e-3 tags nothing there (one stage, one AMR level).
-  Starfish 64 -> 1024 (config 3, 544,664 cells): the whole FP64 solve equals
+  Starfish 64 -> 1024 (544,664 cells): the whole FP64 solve equals
   the oracle's bit for bit (residual history, every Q), and the mixed form
   reaches the same solution within 1e-10 relative L2.
-  Variable-D refined to 2048^2 spacing (config 2 with the ulap threshold at
+  Variable-D refined to 2048^2 spacing (the 256^2 problem with the ulap threshold at
   1e-6, 1.75M cells): the staged solve (tagging after each stage) builds the
   same hierarchy as the oracle-driven one and ends on the same Q bit for bit;
   the mixed form of the final stage is within 1e-10 of the FP64 one.
